@@ -1,0 +1,189 @@
+/*
+ * tmhip.h - C ABI of the MI355X cyclic-symmetry modal solve engine.
+ *
+ * This is the drop-in boundary below turbomodal's C++ cyclic solver.  The
+ * reference runs the whole pipeline of CyclicSymmetrySolver::solve_at_rpm
+ * (src/cyclic_solver.cpp:725-1357) on the CPU with Eigen/Spectra; here the
+ * host object model (Mesh, Material, CyclicSymmetrySolver, ModalResult) calls
+ * these entry points and every numeric stage runs as gfx950 HIP kernels:
+ *
+ *   reference operator                              replaced by
+ *   ---------------------------------------------   -------------------------
+ *   GlobalAssembler::assemble / assemble_rotating_  tm_operator_create
+ *     effects / assemble_centrifugal_load /           (element kernels, fixed-
+ *     assemble_stress_stiffening                      order segmented assembly,
+ *     (include/turbomodal/assembler.hpp:9-41)         static prestress solve)
+ *   CyclicSymmetrySolver::static_centrifugal         tm_operator_create
+ *     (src/cyclic_solver.cpp:456-558)
+ *   precompute_cyclic_projections                    tm_mesh_create (symbolic)
+ *     (src/cyclic_solver.cpp:560-723)                  + tm_operator_create
+ *   SymShiftInvertLDLT, HermitianLanczosEigen-       tm_solve_batch
+ *     Solver::solve, ModalSolver::solve_real /
+ *     solve_complex_hermitian, solve_harmonic
+ *     (include/turbomodal/sym_shift_invert_ldlt.hpp:25-112,
+ *      src/hermitian_lanczos.cpp:12-460,
+ *      src/modal_solver.cpp:12-147,
+ *      src/cyclic_solver.cpp:937-1224)
+ *
+ * Conventions: plain C types only, row-major host arrays, every call returns
+ * TM_OK (0) or a negative status; tm_last_error() holds the message.  No C++
+ * exception crosses this boundary.  A context owns one HIP stream on one
+ * device; calls on one context are serialised by the caller.
+ */
+#ifndef TMHIP_H
+#define TMHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TM_OK 0
+#define TM_ERR_ARG -1        /* invalid argument (reference: std::invalid_argument) */
+#define TM_ERR_RUNTIME -2    /* topology/runtime error (reference: std::runtime_error) */
+#define TM_ERR_HIP -3        /* HIP runtime failure */
+#define TM_ERR_NUMERIC -4    /* zero pivot / breakdown that could not be recovered */
+#define TM_ERR_MEMORY -5     /* device memory exhausted or admission refused */
+
+/* node classes, as CyclicSymmetrySolver::classify_dofs (src/cyclic_solver.cpp:214-238) */
+#define TM_NODE_INTERIOR 0
+#define TM_NODE_LEFT 1
+#define TM_NODE_RIGHT 2
+
+/* precision of the shift-invert factor (north-star mixed precision) */
+#define TM_PREC_FP64 0       /* complex FP64 factor, FP64 everywhere */
+#define TM_PREC_MIXED 1      /* complex FP32 factor, FP64 refinement + Rayleigh-Ritz */
+
+/* eigen-solver convergence policy */
+#define TM_CONV_TIGHT 0      /* relative Ritz residual <= tol_rel (default 1e-12) */
+#define TM_CONV_REFERENCE 1  /* |beta s_i| < 1e-8 max(1,|theta_i|) (src/hermitian_lanczos.cpp:414-424) */
+
+typedef struct tm_context tm_context;
+typedef struct tm_mesh tm_mesh;
+typedef struct tm_operator tm_operator;
+
+/* Context: one device, one stream.  device < 0 selects the current device. */
+int tm_context_create(int device, tm_context** out);
+void tm_context_destroy(tm_context* ctx);
+const char* tm_last_error(const tm_context* ctx);
+/* Per-kernel timing with HIP events on the context stream (0 = off). */
+int tm_set_profiling(tm_context* ctx, int enabled);
+
+typedef struct tm_kernel_stat {
+  char name[48];
+  int64_t launches;
+  double total_ms;         /* sum of event-timed durations */
+  double alg_bytes;        /* algorithmic HBM bytes, summed over launches */
+  double alg_flops;        /* algorithmic flops, summed over launches */
+} tm_kernel_stat;
+/* Copies up to max_stats entries; returns the number of entries available. */
+int tm_kernel_stats(tm_context* ctx, tm_kernel_stat* stats, int max_stats);
+int tm_reset_kernel_stats(tm_context* ctx);
+
+/*
+ * Mesh + cyclic topology upload and symbolic analysis (integer work, once per
+ * mesh): reduced/free node map, node-block CSR pattern of K(k), segmented
+ * assembly slot maps, nested-dissection ordering, assembly tree, supernodal
+ * fronts.  Mirrors Mesh (include/turbomodal/mesh.hpp:40-105) after
+ * match_boundary_nodes and CyclicSymmetrySolver::classify_dofs.
+ *   nodes           n_nodes x 3 float64
+ *   elements        n_elems x 10 int32, gmsh TET10 order
+ *   node_class      n_nodes int32, TM_NODE_*
+ *   right_partner   n_nodes int32: left partner of each right node, else -1
+ *   constrained     3*n_nodes uint8: 1 = DOF removed (build_constrained_dofs)
+ */
+int tm_mesh_create(tm_context* ctx, const double* nodes, int32_t n_nodes,
+                   const int32_t* elements, int32_t n_elems,
+                   const int32_t* node_class, const int32_t* right_partner,
+                   const uint8_t* constrained, int32_t num_sectors,
+                   int32_t rotation_axis, tm_mesh** out);
+void tm_mesh_destroy(tm_mesh* mesh);
+
+typedef struct tm_mesh_info {
+  int32_t n_nodes, n_elems, n_reduced_nodes, n_free_nodes;
+  int32_t n_free;            /* free DOFs (the reference's n_free_) */
+  int32_t n_dof_system;      /* 3 * n_free_nodes: order of the factored system */
+  int64_t nnz_blocks;        /* 3x3 blocks in the K(k) pattern (both triangles) */
+  int64_t nnz_L;             /* complex entries stored in the supernodal factor */
+  double factor_flops;       /* real flops of one complex LDL^H factorization */
+  int32_t n_supernodes, tree_levels, max_front;
+  int64_t front_bytes_fp64;  /* device bytes of one problem's factor workspace (FP64) */
+} tm_mesh_info;
+int tm_mesh_get_info(const tm_mesh* mesh, tm_mesh_info* info);
+
+/*
+ * Operator = mesh + material: element kernels (Ke, Me, Kw at unit omega, G,
+ * centrifugal load at unit omega), fixed-order segmented assembly straight
+ * into the phase-split cyclic blocks (K_c, K_p, M_c, M_p, G_c, G_p), and -
+ * when prestress != 0 - the k = 0 static solve K u = F at unit omega followed
+ * by the stress-stiffening kernel.  K_eff(omega) = K + omega^2 (Ks1 - Kw1)
+ * exactly (Ks is linear in u, u in F, F in omega^2), so one operator serves
+ * every rpm.  Material validation follows src/material.cpp:15-22.
+ */
+int tm_operator_create(tm_context* ctx, tm_mesh* mesh, double E, double nu,
+                       double rho, int32_t prestress, tm_operator** out);
+void tm_operator_destroy(tm_operator* op);
+
+/* One eigenproblem: (operator, rpm, harmonic k, num_modes). */
+typedef struct tm_problem {
+  tm_operator* op;
+  double rpm;
+  int32_t harmonic;
+  int32_t num_modes;
+  int32_t precision;         /* TM_PREC_* */
+  int32_t conv_policy;       /* TM_CONV_* */
+  double shift;              /* sigma; the reference uses 1.0 */
+  double tol_rel;            /* TM_CONV_TIGHT threshold (<= 0: 1e-12) */
+  int32_t max_restarts;      /* <= 0: 1000 as SolverConfig (modal_solver.hpp:39) */
+  int32_t reserved;
+  double min_frequency;      /* Hz; > 0: solve 6 extra modes and drop f <= min_frequency
+                                (src/cyclic_solver.cpp:1011, 1058-1084) */
+} tm_problem;
+
+/* Result of one problem, in the reference's ModalResult layout
+ * (include/turbomodal/modal_solver.hpp:8-31).  Caller-owned buffers sized
+ * from num_modes and 3*n_nodes; any pointer may be NULL to skip. */
+typedef struct tm_result {
+  double* frequencies;       /* [num_modes] Hz, after degenerate duplication */
+  double* mode_shapes;       /* [3*n_nodes][n_modes] complex128 (re,im pairs), row-major,
+                                row stride n_modes (<= num_modes) */
+  int32_t* whirl;            /* [num_modes] */
+  double* eigenvalues;       /* [num_modes] lambda of the reduced problem (before duplication) */
+  double* residuals;         /* [num_modes] ||K x - lambda M x|| / ||K x|| (free space, FP64) */
+  double* modal_mass;        /* [num_modes] N * Re(phi^H M_sector phi), kg (python/turbomodal/solver.py:91-103) */
+  int32_t n_modes;           /* out: modes written */
+  int32_t n_unique;          /* out: eigenpairs solved (nev_target) */
+  int32_t converged;         /* out */
+  int32_t restarts;          /* out: implicit restarts used */
+  int32_t status;            /* out: TM_OK or failure (harmonic dropped by caller) */
+  double seconds;            /* out: wall time of the batch this problem ran in */
+} tm_result;
+
+/* Solve a batch: problems sharing an operator run together (one symbolic
+ * analysis, batched numeric factorizations and Lanczos).  Blocking. */
+int tm_solve_batch(tm_context* ctx, const tm_problem* problems, int32_t n,
+                   tm_result* results);
+
+/* Device bytes one problem holds while in flight (factor + Lanczos basis);
+ * the serving loop admits requests against this. */
+int64_t tm_problem_footprint(const tm_operator* op, int32_t num_modes,
+                             int32_t precision, int32_t harmonic);
+int64_t tm_device_free_bytes(tm_context* ctx);
+
+/* Diagnostics used by the parity tests: the assembled free-space matrices
+ * K(k) (at rpm) and M(k) in COO form over free DOFs in the reference's free
+ * ordering (interior DOFs, then left DOFs, node-ascending; constrained DOFs
+ * removed).  Call with rows == NULL to get the entry count. */
+int tm_operator_export(tm_operator* op, double rpm, int32_t harmonic,
+                       int32_t which /* 0 = K_eff(k), 1 = M(k), 2 = G(k) */,
+                       int64_t* nnz, int32_t* rows, int32_t* cols, double* vals_c128);
+/* Static prestress displacement at unit omega (3*n_nodes), the reference's
+ * static_centrifugal(1.0). */
+int tm_operator_static_u(tm_operator* op, double* u_full);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TMHIP_H */

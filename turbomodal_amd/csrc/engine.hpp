@@ -1,0 +1,162 @@
+// Internal structures of the MI355X cyclic modal engine (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/tmhip.h"
+
+namespace tme {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ArgError : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+struct NumericError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define TM_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      throw ::tme::HipError(std::string(#call) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+// Device buffer owned by the engine.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t n) { alloc(n); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t n) {
+    release();
+    if (n == 0) return;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw HipError("hipMalloc(" + std::to_string(n) + " B): " + hipGetErrorString(e));
+    }
+    bytes = n;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+template <class T>
+inline DevBuf upload(const std::vector<T>& v, hipStream_t s) {
+  DevBuf b(v.size() * sizeof(T));
+  if (!v.empty()) {
+    TM_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    TM_HIP(hipStreamSynchronize(s));  // the host vector may be a temporary
+  }
+  return b;
+}
+
+// Per-kernel event timing (bench roofline) -----------------------------------
+struct KernelStat {
+  int64_t launches = 0;
+  double total_ms = 0, bytes = 0, flops = 0;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  bool profiling = false;
+  std::map<std::string, KernelStat> stats;
+  // pending timed launches: (name, start, stop, bytes, flops)
+  struct Pending { std::string name; hipEvent_t a, b; double bytes, flops; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  hipEvent_t get_event();
+  void flush_timing();  // synchronises and folds pending events into stats
+};
+
+// RAII scope that times everything launched on ctx->stream inside it.
+struct KTimer {
+  Context* ctx;
+  std::string name;
+  double bytes, flops;
+  hipEvent_t a = nullptr;
+  KTimer(Context* c, const char* n, double by = 0, double fl = 0);
+  ~KTimer();
+};
+
+// Symbolic analysis of one mesh ----------------------------------------------
+struct Snode {
+  int col0 = 0;        // first permuted free node of the pivot block
+  int ncol = 0;        // pivot nodes
+  int nstr = 0;        // nodes below the pivot block
+  int parent = -1;
+  int height = 0;
+  int child[2] = {-1, -1};
+  int64_t foff = 0;    // offset (complex entries) of the m x m front
+  int64_t toff = 0;    // offset (complex entries) of the r-long update vector
+  int str0 = 0;        // offset into struct_nodes
+  int rel0 = 0;        // offset into relmap (node-level positions in parent front)
+  int asm0 = 0, asm1 = 0;  // range in asm lists
+  int m() const { return 3 * (ncol + nstr); }
+  int w() const { return 3 * ncol; }
+  int r() const { return 3 * nstr; }
+};
+
+struct Symbolic {
+  int nn = 0, ne = 0, N = 0, axis = 2;
+  int c1 = 0, c2 = 1, cax = 2;
+  std::vector<int> elements;          // ne*10
+  std::vector<double> coords;         // nn*3
+  std::vector<int> node_class, partner;
+  std::vector<uint8_t> cmask;         // 3*nn
+  // full node -> permuted free node (interior/left nodes and right nodes via partner), -1 removed
+  std::vector<int> free_of_full;
+  std::vector<int> free_to_full;      // nf: interior/left full node id
+  std::vector<uint8_t> pinned;        // 3*nf
+  int nf = 0;
+  int n_free_dofs = 0;                // reference n_free
+  int n_reduced_nodes = 0;
+  // node-block CSR pattern of K(k) over permuted free nodes
+  std::vector<int> rowptr, colidx, tslot;
+  // segmented assembly: per slot, contributions eb*4 + flags  (eb = e*100 + a*10 + b)
+  std::vector<int> kc_ptr, kc_src, kp_ptr, kp_src;
+  // centrifugal load: per free node, contributions (e*10+a)*2 + rotated
+  std::vector<int> f_ptr, f_src;
+  // fronts
+  std::vector<Snode> sn;
+  std::vector<int> struct_nodes;      // concatenated struct lists (permuted free nodes)
+  std::vector<int> relmap;            // per snode struct node -> node position in parent front
+  std::vector<int> asm_slot, asm_pos; // per snode: (slot of (I,J)), (rowpos<<16 | colpos) node-level
+  std::vector<std::vector<int>> levels;   // snodes by height
+  int64_t front_entries = 0;          // sum m^2
+  int64_t tvec_entries = 0;           // sum r
+  int64_t nnz_L = 0;
+  double flops = 0;
+  int max_front = 0;
+  // reference free-DOF ordering (interior DOFs then left DOFs, constrained removed):
+  // for each such DOF, its system index 3*perm_node + c
+  std::vector<int> ref_free_to_sys;
+};
+
+void build_symbolic(Symbolic& S);
+
+}  // namespace tme

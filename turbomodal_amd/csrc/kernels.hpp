@@ -1,0 +1,136 @@
+// Launchers and device-side plan structures shared by the engine sources.
+#pragma once
+
+#include "engine.hpp"
+
+namespace tme {
+
+struct Mesh;
+// ---- FEM (fem.hip) ----------------------------------------------------------
+void launch_element_base(Context* ctx, Mesh& M, double E, double nu, double rho, DevBuf& ke,
+                         DevBuf& m_abs, DevBuf& m_raw, DevBuf& fe);
+void launch_element_sigma(Context* ctx, Mesh& M, double E, double nu, const DevBuf& u_full,
+                          DevBuf& s_ab);
+// kind: 0 = K (element stiffness), 1 = M, 2 = Ks1 - Kw1, 3 = G
+void launch_assemble(Context* ctx, Mesh& M, int kind, bool phase, const DevBuf& ke,
+                     const DevBuf& m_abs, const DevBuf& m_raw, const DevBuf* s_ab, DevBuf& out);
+void launch_assemble_load(Context* ctx, Mesh& M, const DevBuf& fe, DevBuf& f);
+void launch_expand_real(Context* ctx, Mesh& M, const double2* u, DevBuf& out);
+
+// ---- per-problem parameters --------------------------------------------------
+struct ProbParam {
+  double pre, pim;   // P = exp(i k alpha)
+  double s;          // omega^2 (rad^2/s^2): K_eff = K + s (Ks1 - Kw1)
+  double sigma;      // shift
+};
+
+// Operator arrays (9 doubles per pattern slot).
+struct OpArrays {
+  const double *Kc, *Kp, *dKc, *dKp, *Mc, *Mp;
+};
+
+// ---- plans (built once per mesh from the symbolic analysis) ------------------
+struct AsmTask {    // front assembly: node columns [n0, n1) of snode sn
+  int sn, n0, n1, a0, a1;
+  int clo[2], chi[2];  // child struct node ranges that land in [n0, n1)
+};
+struct RowTask {    // rows [r0, r1) (DOF) of snode sn
+  int sn, r0, r1;
+};
+struct TileTask {   // 64x64 lower tile of the trailing front
+  int sn, i0, j0;
+};
+
+// Dense partial factorization plan step at one (level, kb).
+struct FactorStep {
+  int kb;
+  int diag0, ndiag;    // range in diag task list (snode ids)
+  int trsm0, ntrsm;    // range in row tasks
+  int tile0, ntile;    // range in tile tasks
+};
+
+struct LevelPlan {
+  int asm0, nasm;          // front assembly tasks
+  std::vector<FactorStep> steps;
+  int sn0, nsn;            // snode ids of this level (range in level_sn)
+  int fup0, nfup;          // forward off-diagonal row tasks
+  int bup0, nbup;          // backward column tasks (RowTask used as column range)
+};
+
+struct Plan {
+  int nb = 32;             // diagonal block width
+  int tile = 64;           // Schur tile
+  std::vector<LevelPlan> levels;
+  std::vector<AsmTask> asm_tasks;
+  std::vector<int> diag_tasks;
+  std::vector<RowTask> row_tasks;
+  std::vector<TileTask> tile_tasks;
+  std::vector<int> level_sn;
+  std::vector<RowTask> fup_tasks, bup_tasks;
+  // device copies
+  DevBuf d_asm, d_diag, d_row, d_tile, d_level_sn, d_fup, d_bup;
+  // per-snode device data
+  DevBuf d_sn;             // DevSn[]
+  int64_t panel_entries = 0, upool_entries = 0, tvec_entries = 0;
+  double trisolve_bytes_c16 = 0;  // algorithmic bytes of one L/D/L^H pair (complex fp64)
+};
+
+struct DevSn {
+  int col0, ncol, nstr, str0, rel0, child0, child1, pad;
+  long long poff, uoff, toff;
+};
+
+void build_plan(const Symbolic& S, Plan& P, hipStream_t stream);
+
+// Device-resident mesh: topology, symbolic analysis and plan.
+struct Mesh {
+  Context* ctx = nullptr;
+  Symbolic S;
+  Plan plan;
+  DevBuf d_coords, d_elements, d_rowptr, d_colidx, d_tslot;
+  DevBuf d_kc_ptr, d_kc_src, d_kp_ptr, d_kp_src, d_f_ptr, d_f_src;
+  DevBuf d_pinned, d_free_of_full, d_node_class;
+  DevBuf d_struct, d_relmap, d_asm_slot, d_asm_pos;
+};
+
+// Assembled operator (mesh + material): FP64 phase-split node blocks, 9 doubles
+// per pattern slot.  K_eff(omega) = K + omega^2 dK with dK = Ks1 - Kw1.
+struct Operator {
+  Mesh* mesh = nullptr;
+  double E = 0, nu = 0, rho = 0;
+  bool prestress = false;
+  size_t nblk = 0;
+  DevBuf Kc, Kp, dKc, dKp, Mc, Mp, Gc, Gp;
+  std::vector<double> u_static;       // 3*nn at unit omega
+  std::map<int, DevBuf> Mk_cache;     // harmonic -> complex M(k) blocks
+};
+
+// ---- factorization (factor.hip) ------------------------------------------------
+struct BatchBufs {
+  double2* L;        // B * panel_entries
+  double2* U;        // B * upool_entries
+  double2* T;        // B * tvec_entries
+  int* status;       // B
+  const ProbParam* params;  // B
+  int B;
+};
+
+void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& ops,
+                  const BatchBufs& bb);
+// in-place solve A x = b for each problem's vector x_b = X + b*ldx
+void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs& bb, double2* X,
+                    long long ldx, const int* active);
+
+struct HostProblem;
+void create_operator(Context* ctx, Mesh& M, Operator& op);
+OpArrays op_arrays(const Operator& op);
+const double2* harmonic_mass(Context* ctx, Operator& op, int k);
+// solves every problem; results written through each problem's tm_result
+void solve_problems(Context* ctx, const tm_problem* problems, int n, tm_result* results);
+int64_t problem_footprint(const Operator& op, int num_modes, int harmonic);
+
+}  // namespace tme
+
+struct tm_context : tme::Context {};
+struct tm_mesh : tme::Mesh {};
+struct tm_operator : tme::Operator {};

@@ -1,0 +1,209 @@
+// Execution plan of the supernodal multifrontal LDL^H: per tree level, the
+// task tables of every kernel (front assembly, diagonal-block factorization,
+// panel solve, Schur tiles, triangular-solve row/column chunks) and the
+// memory layout of one problem (L panels, a level-scheduled pool for the
+// update matrices, update vectors of the triangular solves).  Built once per
+// mesh; every numeric factorization and solve of every problem that shares
+// the mesh replays it.
+#include <algorithm>
+
+#include "kernels.hpp"
+
+namespace tme {
+
+namespace {
+
+// First-fit interval allocator (deterministic) used to lay out update matrices.
+class Pool {
+ public:
+  int64_t alloc(int64_t n) {
+    if (n == 0) return 0;
+    for (size_t i = 0; i < free_.size(); ++i)
+      if (free_[i].second >= n) {
+        int64_t off = free_[i].first;
+        free_[i].first += n;
+        free_[i].second -= n;
+        if (free_[i].second == 0) free_.erase(free_.begin() + i);
+        return off;
+      }
+    int64_t off = top_;
+    top_ += n;
+    return off;
+  }
+  void release(int64_t off, int64_t n) {
+    if (n == 0) return;
+    free_.push_back({off, n});
+    std::sort(free_.begin(), free_.end());
+    std::vector<std::pair<int64_t, int64_t>> merged;
+    for (auto& f : free_) {
+      if (!merged.empty() && merged.back().first + merged.back().second == f.first)
+        merged.back().second += f.second;
+      else
+        merged.push_back(f);
+    }
+    free_ = merged;
+    // shrink the top when its last block is free
+    while (!free_.empty() && free_.back().first + free_.back().second == top_) {
+      top_ = free_.back().first;
+      free_.pop_back();
+    }
+  }
+  int64_t peak() const { return peak_; }
+  void note() { peak_ = std::max(peak_, top_); }
+
+ private:
+  std::vector<std::pair<int64_t, int64_t>> free_;
+  int64_t top_ = 0, peak_ = 0;
+};
+
+}  // namespace
+
+void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
+  const int nsn = static_cast<int>(S.sn.size());
+  std::vector<DevSn> dsn(nsn);
+  int64_t poff = 0, toff = 0;
+  for (int id = 0; id < nsn; ++id) {
+    const Snode& s = S.sn[id];
+    DevSn& d = dsn[id];
+    d.col0 = s.col0;
+    d.ncol = s.ncol;
+    d.nstr = s.nstr;
+    d.str0 = s.str0;
+    d.rel0 = s.rel0;
+    d.child0 = s.child[0];
+    d.child1 = s.child[1];
+    d.pad = 0;
+    d.poff = poff;
+    poff += int64_t(s.m()) * s.w();
+    d.toff = toff;
+    toff += s.r();
+  }
+  Pool pool;
+  for (const auto& lvl : S.levels) {
+    for (int id : lvl) {
+      const Snode& s = S.sn[id];
+      dsn[id].uoff = pool.alloc(int64_t(s.r()) * s.r());
+    }
+    pool.note();
+    for (int id : lvl)
+      for (int c : S.sn[id].child)
+        if (c >= 0) pool.release(dsn[c].uoff, int64_t(S.sn[c].r()) * S.sn[c].r());
+  }
+  P.panel_entries = poff;
+  P.tvec_entries = std::max<int64_t>(toff, 1);
+  P.upool_entries = std::max<int64_t>(pool.peak(), 1);
+
+  P.levels.clear();
+  P.asm_tasks.clear();
+  P.diag_tasks.clear();
+  P.row_tasks.clear();
+  P.tile_tasks.clear();
+  P.level_sn.clear();
+  P.fup_tasks.clear();
+  P.bup_tasks.clear();
+  const int NB = P.nb, TL = P.tile;
+  constexpr int kAsmChunk = 16;   // node columns per assembly task
+  constexpr int kRowChunk = 256;  // rows per panel-solve / forward-update task
+  constexpr int kColChunk = 4;    // columns per backward-update task (one wave each)
+  double tri_bytes = 0;
+  for (const auto& lvl : S.levels) {
+    LevelPlan L;
+    L.sn0 = static_cast<int>(P.level_sn.size());
+    L.nsn = static_cast<int>(lvl.size());
+    P.level_sn.insert(P.level_sn.end(), lvl.begin(), lvl.end());
+    L.asm0 = static_cast<int>(P.asm_tasks.size());
+    int maxw = 0;
+    for (int id : lvl) {
+      const Snode& s = S.sn[id];
+      maxw = std::max(maxw, s.w());
+      const int nodes = s.ncol + s.nstr;
+      for (int n0 = 0; n0 < nodes; n0 += kAsmChunk) {
+        AsmTask t{};
+        t.sn = id;
+        t.n0 = n0;
+        t.n1 = std::min(nodes, n0 + kAsmChunk);
+        // asm entries are ordered by column node (colpos = pos & 0xffff)
+        int a = s.asm0, b = s.asm1;
+        auto colpos = [&](int q) { return S.asm_pos[q] & 0xffff; };
+        int lo = a, hi = a;
+        while (lo < b && colpos(lo) < t.n0) ++lo;
+        hi = lo;
+        while (hi < b && colpos(hi) < t.n1) ++hi;
+        t.a0 = lo;
+        t.a1 = hi;
+        for (int k = 0; k < 2; ++k) {
+          int c = s.child[k];
+          t.clo[k] = t.chi[k] = 0;
+          if (c < 0) continue;
+          const Snode& cs = S.sn[c];
+          const int* rm = S.relmap.data() + cs.rel0;
+          int l = static_cast<int>(std::lower_bound(rm, rm + cs.nstr, t.n0) - rm);
+          int h = static_cast<int>(std::lower_bound(rm, rm + cs.nstr, t.n1) - rm);
+          t.clo[k] = l;
+          t.chi[k] = h;
+        }
+        P.asm_tasks.push_back(t);
+      }
+      for (int r0 = 0; r0 < s.r(); r0 += kRowChunk)
+        P.fup_tasks.push_back({id, r0, std::min(s.r(), r0 + kRowChunk)});
+      for (int c0 = 0; c0 < s.w(); c0 += kColChunk)
+        P.bup_tasks.push_back({id, c0, std::min(s.w(), c0 + kColChunk)});
+      tri_bytes += 2.0 * (double(s.m()) * s.w() - double(s.w()) * (s.w() - 1) / 2) * 20.0;
+    }
+    L.nasm = static_cast<int>(P.asm_tasks.size()) - L.asm0;
+    for (int kb = 0; kb < maxw; kb += NB) {
+      FactorStep st{};
+      st.kb = kb;
+      st.diag0 = static_cast<int>(P.diag_tasks.size());
+      st.trsm0 = static_cast<int>(P.row_tasks.size());
+      st.tile0 = static_cast<int>(P.tile_tasks.size());
+      for (int id : lvl) {
+        const Snode& s = S.sn[id];
+        const int w = s.w(), m = s.m();
+        if (w <= kb) continue;
+        const int jb = std::min(NB, w - kb);
+        P.diag_tasks.push_back(id);
+        const int t0 = kb + jb;
+        for (int r0 = t0; r0 < m; r0 += kRowChunk)
+          P.row_tasks.push_back({id, r0, std::min(m, r0 + kRowChunk)});
+        for (int i0 = t0; i0 < m; i0 += TL)
+          for (int j0 = t0; j0 <= i0; j0 += TL) P.tile_tasks.push_back({id, i0, j0});
+      }
+      st.ndiag = static_cast<int>(P.diag_tasks.size()) - st.diag0;
+      st.ntrsm = static_cast<int>(P.row_tasks.size()) - st.trsm0;
+      st.ntile = static_cast<int>(P.tile_tasks.size()) - st.tile0;
+      L.steps.push_back(st);
+    }
+    P.levels.push_back(L);
+  }
+  // forward/backward task ranges per level
+  {
+    int f = 0, b = 0;
+    for (size_t h = 0; h < S.levels.size(); ++h) {
+      LevelPlan& L = P.levels[h];
+      L.fup0 = f;
+      L.bup0 = b;
+      int nf = 0, nbk = 0;
+      for (int id : S.levels[h]) {
+        const Snode& s = S.sn[id];
+        nf += (s.r() + kRowChunk - 1) / kRowChunk;
+        nbk += (s.w() + kColChunk - 1) / kColChunk;
+      }
+      L.nfup = nf;
+      L.nbup = nbk;
+      f += nf;
+      b += nbk;
+    }
+  }
+  P.trisolve_bytes_c16 = tri_bytes + 5.0 * 3 * S.nf * 16.0;
+  P.d_sn = upload(dsn, stream);
+  P.d_asm = upload(P.asm_tasks, stream);
+  P.d_diag = upload(P.diag_tasks, stream);
+  P.d_row = upload(P.row_tasks, stream);
+  P.d_tile = upload(P.tile_tasks, stream);
+  P.d_level_sn = upload(P.level_sn, stream);
+  P.d_fup = upload(P.fup_tasks, stream);
+  P.d_bup = upload(P.bup_tasks, stream);
+}
+
+}  // namespace tme

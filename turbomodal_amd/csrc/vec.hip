@@ -1,0 +1,367 @@
+// Batched vector kernels of the shift-invert Lanczos (gfx950).
+//
+// Every kernel works on a batch of problems (grid.y = problem) whose Lanczos
+// step index j_b may differ; per-problem state lives on the device (jcur,
+// act, frozen) so a whole step is launched without a host round trip.
+// Reductions are two-phase with a fixed chunking and a fixed combine order,
+// so a problem's arithmetic does not depend on what else is in the batch.
+#include "vec.hpp"
+
+namespace tme {
+
+namespace {
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cconjmul(double2 a, double2 b) {  // conj(a) b
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < (int)(blockDim.x / 64); ++q) t += sh[q];
+  return t;  // valid in thread 0
+}
+
+}  // namespace
+
+__global__ void k_update_act(LzState st, int B) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  st.act[b] = (st.live[b] && !st.frozen[b] && st.jcur[b] < st.ncv[b]) ? 1 : 0;
+}
+
+// W_b = MV_b[:, j_b]
+__global__ void k_copy_mv(LzState st, long long n) {
+  const int b = blockIdx.y;
+  if (!st.act[b]) return;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2* src = st.MV + b * st.ldb + (long long)st.jcur[b] * n;
+  st.W[b * n + i] = src[i];
+}
+
+// partial dots: part[b][k][chunk] = sum_chunk conj(A_k[i]) x[i]
+// mode 0: k in [0, j_b] over MV (reorthogonalisation coefficients)
+// mode 1: k = j_b only over MV (alpha)
+// mode 2: single dot conj(W) MW (beta^2)
+__global__ __launch_bounds__(256) void k_mdot(LzState st, long long n, int mode, int kmax) {
+  const int b = blockIdx.y;
+  if (!st.act[b]) return;
+  const int chunk = blockIdx.x;
+  const long long i0 = (long long)chunk * kDotChunk;
+  const long long i1 = i0 + kDotChunk < n ? i0 + kDotChunk : n;
+  __shared__ double sh[8];
+  const double2* x = (mode == 2 ? st.MW : st.W) + b * n;
+  const int j = st.jcur[b];
+  int k0, k1;
+  if (mode == 0) { k0 = 0; k1 = j + 1; }
+  else if (mode == 1) { k0 = j; k1 = j + 1; }
+  else { k0 = 0; k1 = 1; }
+  // cache this thread's x values
+  double2 xv[kDotChunk / 256];
+  for (int q = 0; q < kDotChunk / 256; ++q) {
+    long long i = i0 + threadIdx.x + q * 256;
+    xv[q] = i < i1 ? x[i] : make_double2(0.0, 0.0);
+  }
+  for (int k = k0; k < k1; ++k) {
+    const double2* a = mode == 2 ? st.W + b * n : st.MV + b * st.ldb + (long long)k * n;
+    double re = 0.0, im = 0.0;
+    for (int q = 0; q < kDotChunk / 256; ++q) {
+      long long i = i0 + threadIdx.x + q * 256;
+      if (i < i1) {
+        double2 u = cconjmul(a[i], xv[q]);
+        re += u.x;
+        im += u.y;
+      }
+    }
+    double tre = block_sum(re, sh);
+    double tim = block_sum(im, sh);
+    if (threadIdx.x == 0) {
+      double2* p = st.part + ((long long)b * kmax + k) * st.nchunk + chunk;
+      *p = make_double2(tre, tim);
+    }
+  }
+}
+
+// h[b][k] = sum over chunks (fixed order)
+__global__ void k_mdot_reduce(LzState st, int mode, int kmax) {
+  const int b = blockIdx.x;
+  if (!st.act[b]) return;
+  const int j = st.jcur[b];
+  int k0, k1;
+  if (mode == 0) { k0 = 0; k1 = j + 1; }
+  else if (mode == 1) { k0 = j; k1 = j + 1; }
+  else { k0 = 0; k1 = 1; }
+  for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+    const double2* p = st.part + ((long long)b * kmax + k) * st.nchunk;
+    double re = 0.0, im = 0.0;
+    for (int c = 0; c < st.nchunk; ++c) {
+      re += p[c].x;
+      im += p[c].y;
+    }
+    st.h[(long long)b * kmax + k] = make_double2(re, im);
+    if (mode == 1) st.alpha[(long long)b * st.ncvmax + j] = re;
+  }
+}
+
+// W -= alpha_j V_j + beta_{j-1} V_{j-1}
+__global__ void k_three_term(LzState st, long long n) {
+  const int b = blockIdx.y;
+  if (!st.act[b]) return;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = st.jcur[b];
+  const double a = st.alpha[(long long)b * st.ncvmax + j];
+  const double2* V = st.V + b * st.ldb;
+  double2 w = st.W[b * n + i];
+  double2 vj = V[(long long)j * n + i];
+  w.x -= a * vj.x;
+  w.y -= a * vj.y;
+  if (j > 0) {
+    const double bt = st.beta[(long long)b * st.ncvmax + j - 1];
+    double2 vm = V[(long long)(j - 1) * n + i];
+    w.x -= bt * vm.x;
+    w.y -= bt * vm.y;
+  }
+  st.W[b * n + i] = w;
+}
+
+// W -= sum_{k<=j} h_k V_k
+__global__ void k_axpy_multi(LzState st, long long n, int kmax) {
+  const int b = blockIdx.y;
+  if (!st.act[b]) return;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = st.jcur[b];
+  const double2* V = st.V + b * st.ldb;
+  const double2* h = st.h + (long long)b * kmax;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int k = 0; k <= j; ++k) acc = cadd(acc, cmul(h[k], V[(long long)k * n + i]));
+  double2 w = st.W[b * n + i];
+  st.W[b * n + i] = make_double2(w.x - acc.x, w.y - acc.y);
+}
+
+// y_b = M(k_b) x_b, complex node-block CSR, one thread per DOF row
+__global__ void k_spmv_bsr(int nf, const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                           const double2* const* __restrict__ blocks, const double2* __restrict__ X,
+                           long long ldx, double2* __restrict__ Y, long long ldy,
+                           const int* __restrict__ act) {
+  const int b = blockIdx.y;
+  if (act && !act[b]) return;
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= 3 * nf) return;
+  const int I = row / 3, c = row % 3;
+  const double2* Mb = blocks[b];
+  const double2* x = X + b * ldx;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int s = rowptr[I]; s < rowptr[I + 1]; ++s) {
+    const int J = colidx[s];
+    const double2* blk = Mb + (long long)s * 9 + 3 * c;
+    acc = cadd(acc, cmul(blk[0], x[3LL * J]));
+    acc = cadd(acc, cmul(blk[1], x[3LL * J + 1]));
+    acc = cadd(acc, cmul(blk[2], x[3LL * J + 2]));
+  }
+  Y[b * ldy + row] = acc;
+}
+
+// beta_j = sqrt|Re(W^H MW)|.  Breakdown (an invariant Krylov subspace) is
+// declared when beta_j <= 1e-12 max_k |alpha_k|: the reference's absolute
+// 1e-14 (src/hermitian_lanczos.cpp:350) depends on the units of K and M and
+// fires on healthy SI-unit problems once theta drops below 1e-14.  A broken
+// down problem is frozen; the host restarts it with a fresh vector.
+__global__ void k_finish_beta(LzState st, int B, int kmax) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B || !st.act[b]) return;
+  const int j = st.jcur[b];
+  const double bb = sqrt(fabs(st.h[(long long)b * kmax].x));
+  double amax = 0.0;
+  for (int k = 0; k <= j; ++k) amax = fmax(amax, fabs(st.alpha[(long long)b * st.ncvmax + k]));
+  st.bnew[b] = bb;
+  if (!(bb > 1e-12 * amax)) {
+    st.frozen[b] = 1;
+    st.act[b] = 0;
+  } else {
+    st.beta[(long long)b * st.ncvmax + j] = bb;
+  }
+}
+
+// bnew = ||W||_M only (start vector and restart continuation vector)
+__global__ void k_finish_beta_start(LzState st, int B, int kmax) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B || !st.act[b]) return;
+  st.bnew[b] = sqrt(fabs(st.h[(long long)b * kmax].x));
+}
+
+// V_{j+1} = W / beta, MV_{j+1} = MW / beta
+__global__ void k_normalize_next(LzState st, long long n) {
+  const int b = blockIdx.y;
+  if (!st.act[b]) return;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = st.jcur[b];
+  const double inv = 1.0 / st.bnew[b];
+  double2 w = st.W[b * n + i], mw = st.MW[b * n + i];
+  st.V[b * st.ldb + (long long)(j + 1) * n + i] = make_double2(w.x * inv, w.y * inv);
+  st.MV[b * st.ldb + (long long)(j + 1) * n + i] = make_double2(mw.x * inv, mw.y * inv);
+}
+
+__global__ void k_advance(LzState st, int B) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B || !st.act[b]) return;
+  st.jcur[b] += 1;
+}
+
+// out_b[:, c] = sum_{k < nk_b} V_b[:, k] C_b[k][c]  for c < nc_b (C row-major, ldc)
+__global__ void k_combine(const double2* __restrict__ V, long long ldb, long long n,
+                          const double* __restrict__ C, int ldc, long long cstride,
+                          const int* __restrict__ nk, const int* __restrict__ nc,
+                          double2* __restrict__ out, long long ldo) {
+  const int b = blockIdx.y;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int K = nk[b], NC = nc[b];
+  const double2* v = V + b * ldb;
+  const double* cb = C + b * cstride;
+  for (int c = 0; c < NC; ++c) {
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < K; ++k) {
+      const double q = cb[(long long)k * ldc + c];
+      const double2 x = v[(long long)k * n + i];
+      re += q * x.x;
+      im += q * x.y;
+    }
+    out[b * ldo + (long long)c * n + i] = make_double2(re, im);
+  }
+}
+
+// M(k) blocks: M_c + P M_p + conj(P) M_p^T(t), pinned rows/cols zero
+__global__ void k_build_phase_blocks(int nf, const int* __restrict__ rowptr,
+                                     const int* __restrict__ colidx, const int* __restrict__ tslot,
+                                     const unsigned char* __restrict__ pinned,
+                                     const double* __restrict__ Xc, const double* __restrict__ Xp,
+                                     const double* __restrict__ dXc, const double* __restrict__ dXp,
+                                     double s, double pre, double pim, double2* __restrict__ out) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nf) return;
+  for (int sl = rowptr[I]; sl < rowptr[I + 1]; ++sl) {
+    const int J = colidx[sl];
+    const long long tt = tslot[sl];
+    for (int ci = 0; ci < 3; ++ci)
+      for (int cj = 0; cj < 3; ++cj) {
+        const long long e1 = (long long)sl * 9 + 3 * ci + cj, e2 = tt * 9 + 3 * cj + ci;
+        double c = Xc[e1], p = Xp[e1], q = Xp[e2];
+        if (dXc) {
+          c += s * dXc[e1];
+          p += s * dXp[e1];
+          q += s * dXp[e2];
+        }
+        double2 v = make_double2(c + pre * (p + q), pim * (p - q));
+        if (pinned[3 * I + ci] || pinned[3 * J + cj]) v = make_double2(0.0, 0.0);
+        out[(long long)sl * 9 + 3 * ci + cj] = v;
+      }
+  }
+}
+
+// multi-column SpMV with one complex block array: Y[:, c] = A X[:, c], c < ncol_b
+__global__ void k_spmv_multi(int nf, const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                             const double2* const* __restrict__ blocks, const double2* __restrict__ X,
+                             long long ldb, long long n, const int* __restrict__ ncol,
+                             double2* __restrict__ Y) {
+  const int b = blockIdx.y;
+  const int c = blockIdx.z;
+  if (c >= ncol[b]) return;
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= 3 * nf) return;
+  const int I = row / 3, cc = row % 3;
+  const double2* Mb = blocks[b];
+  const double2* x = X + b * ldb + (long long)c * n;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int s = rowptr[I]; s < rowptr[I + 1]; ++s) {
+    const int J = colidx[s];
+    const double2* blk = Mb + (long long)s * 9 + 3 * cc;
+    acc = cadd(acc, cmul(blk[0], x[3LL * J]));
+    acc = cadd(acc, cmul(blk[1], x[3LL * J + 1]));
+    acc = cadd(acc, cmul(blk[2], x[3LL * J + 2]));
+  }
+  Y[b * ldb + (long long)c * n + row] = acc;
+}
+
+// per column: num = ||K x - lam M x||^2, den = ||K x||^2, xmx = Re(x^H M x)
+// (chunk partials, combined on the host in fixed order)
+__global__ __launch_bounds__(256) void k_residual_partial(const double2* __restrict__ X,
+                                                          const double2* __restrict__ KX,
+                                                          const double2* __restrict__ MX,
+                                                          long long ldb, long long n,
+                                                          const double* __restrict__ lam, int lam_ld,
+                                                          const int* __restrict__ ncol,
+                                                          double* __restrict__ part, int nchunk) {
+  const int b = blockIdx.y, c = blockIdx.z;
+  if (c >= ncol[b]) return;
+  __shared__ double sh[8];
+  const long long i0 = (long long)blockIdx.x * kDotChunk;
+  const double l = lam[(long long)b * lam_ld + c];
+  double num = 0.0, den = 0.0, xm = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i0 + kDotChunk && i < n; i += 256) {
+    const long long o = b * ldb + (long long)c * n + i;
+    double2 k = KX[o], m = MX[o], x = X[o];
+    double rx = k.x - l * m.x, ry = k.y - l * m.y;
+    num += rx * rx + ry * ry;
+    den += k.x * k.x + k.y * k.y;
+    xm += x.x * m.x + x.y * m.y;
+  }
+  double tn = block_sum(num, sh);
+  double td = block_sum(den, sh);
+  double tx = block_sum(xm, sh);
+  if (threadIdx.x == 0) {
+    double* p = part + (((long long)b * lam_ld + c) * nchunk + blockIdx.x) * 3;
+    p[0] = tn;
+    p[1] = td;
+    p[2] = tx;
+  }
+}
+
+// full-sector mode shapes: out[dof][q] (row-major, ld = nm_ld) = T0 x_q with right
+// rows times P; x_q = X[:, src_q], conjugated when dup_q (degenerate duplication)
+__global__ void k_expand_modes(int nn, const int* __restrict__ free_of_full,
+                               const int* __restrict__ node_class, double cr, double sr, int c1,
+                               int c2, int cax, const double2* __restrict__ X, long long n,
+                               const int* __restrict__ src, const int* __restrict__ conjq, int nm,
+                               double pre, double pim, double2* __restrict__ out) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)nn * nm) return;
+  const int i = (int)(idx / nm), q = (int)(idx % nm);
+  const int p = free_of_full[i];
+  double2 v[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+  if (p >= 0) {
+    const double2* x = X + (long long)src[q] * n + 3LL * p;
+    for (int c = 0; c < 3; ++c) {
+      v[c] = x[c];
+      if (conjq[q]) v[c].y = -v[c].y;
+    }
+    if (node_class[i] == TM_NODE_RIGHT) {
+      double2 w[3];
+      w[c1] = make_double2(cr * v[c1].x - sr * v[c2].x, cr * v[c1].y - sr * v[c2].y);
+      w[c2] = make_double2(sr * v[c1].x + cr * v[c2].x, sr * v[c1].y + cr * v[c2].y);
+      w[cax] = v[cax];
+      const double2 P = make_double2(pre, pim);
+      for (int c = 0; c < 3; ++c) v[c] = cmul(P, w[c]);
+    }
+  }
+  for (int c = 0; c < 3; ++c) out[(3LL * i + c) * nm + q] = v[c];
+}
+
+__global__ void k_real_to_complex(const double* __restrict__ x, double2* __restrict__ y, long long n) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = make_double2(x[i], 0.0);
+}
+
+}  // namespace tme

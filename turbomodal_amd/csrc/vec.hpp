@@ -1,0 +1,65 @@
+// Batched Lanczos state and vector kernels (declarations).
+#pragma once
+
+#include "kernels.hpp"
+
+namespace tme {
+
+constexpr int kDotChunk = 2048;  // rows per partial-dot block (8 per thread)
+
+// Device-side state of a batch of Lanczos runs.
+struct LzState {
+  double2* V;        // B x (ncvmax+1) x n
+  double2* MV;       // same layout
+  double2* W;        // B x n work vector
+  double2* MW;       // B x n
+  double2* h;        // B x kmax coefficients
+  double2* part;     // B x kmax x nchunk partial dots
+  double* alpha;     // B x ncvmax
+  double* beta;      // B x ncvmax
+  double* bnew;      // B
+  int* jcur;         // B current step
+  int* ncv;          // B subspace size
+  int* live;         // B still iterating
+  int* frozen;       // B breakdown at jcur
+  int* act;          // B derived each step
+  long long ldb;     // (ncvmax+1) * n
+  int ncvmax;
+  int nchunk;
+};
+
+__global__ void k_update_act(LzState st, int B);
+__global__ void k_copy_mv(LzState st, long long n);
+__global__ void k_mdot(LzState st, long long n, int mode, int kmax);
+__global__ void k_mdot_reduce(LzState st, int mode, int kmax);
+__global__ void k_three_term(LzState st, long long n);
+__global__ void k_axpy_multi(LzState st, long long n, int kmax);
+__global__ void k_spmv_bsr(int nf, const int* rowptr, const int* colidx, const double2* const* blocks,
+                           const double2* X, long long ldx, double2* Y, long long ldy, const int* act);
+__global__ void k_finish_beta(LzState st, int B, int kmax);
+__global__ void k_finish_beta_start(LzState st, int B, int kmax);
+__global__ void k_normalize_next(LzState st, long long n);
+__global__ void k_advance(LzState st, int B);
+__global__ void k_combine(const double2* V, long long ldb, long long n, const double* C, int ldc,
+                          long long cstride, const int* nk, const int* nc, double2* out, long long ldo);
+__global__ void k_build_phase_blocks(int nf, const int* rowptr, const int* colidx, const int* tslot,
+                                     const unsigned char* pinned, const double* Xc, const double* Xp,
+                                     const double* dXc, const double* dXp, double s, double pre,
+                                     double pim, double2* out);
+__global__ void k_spmv_multi(int nf, const int* rowptr, const int* colidx, const double2* const* blocks,
+                             const double2* X, long long ldb, long long n, const int* ncol, double2* Y);
+__global__ void k_residual_partial(const double2* X, const double2* KX, const double2* MX, long long ldb, long long n,
+                                   const double* lam, int lam_ld, const int* ncol, double* part,
+                                   int nchunk);
+__global__ void k_expand_modes(int nn, const int* free_of_full, const int* node_class, double cr,
+                               double sr, int c1, int c2, int cax, const double2* X, long long n,
+                               const int* src, const int* conjq, int nm, double pre, double pim,
+                               double2* out);
+__global__ void k_real_to_complex(const double* x, double2* y, long long n);
+
+// Host-side symmetric tridiagonal eigensolver (implicit QL, Wilkinson shifts):
+// d (n) diagonal, e (n) off-diagonal e[i] = T(i, i+1); on exit d holds the
+// eigenvalues in ascending order and z (n x n, column-major) the eigenvectors.
+void tridiag_eigen(int n, std::vector<double>& d, std::vector<double> e, std::vector<double>& z);
+
+}  // namespace tme
