@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark: ND modal solves/sec of the cyclic-symmetry modal solve on one MI355X.
+
+Workload (default ``c4``): rpm sweep on the ~45k-DOF bladed sector (disk
+(24,12,4) + twisted blade (24,8,2) Kuhn TET10 lattices, 36 sectors, steel),
+20 rpm points linspace(0, 15000) x harmonic indices 0..18 = 380 complex
+Hermitian eigenproblems, 10 modes each, centrifugal prestress + spin
+softening.  One step = one full sweep through the drop-in
+``CyclicSymmetrySolver.solve_rpm_sweep`` (results incl. full-sector mode
+shapes delivered to host memory).  The mesh, its symbolic analysis and the
+assembled operator are resident on the device before the timed region, as
+the engine keeps them across calls.
+
+Prints one JSON line (see the keys below).  ``--gpus N`` with N > 1 exits:
+this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ND modal solves/sec + p99 latency, TET10 bladed sector, freq err ≤1e-8, MAC≥.999"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def build_solver(workload, precision):
+    import turbomodal_amd as tm
+    from turbomodal_amd import meshgen
+    name = {"c4": "bladed_c2", "c2": "bladed_c2", "c1": "flat_disk_c1"}[workload]
+    sm = meshgen.config_mesh(name)
+    mesh = meshgen.to_core_mesh(sm, tm._core)
+    mat = tm.Material(**meshgen.STEEL)
+    solver = tm.CyclicSymmetrySolver(mesh, mat)
+    return tm, sm, mesh, mat, solver
+
+
+def workload_spec(workload):
+    if workload == "c4":
+        return np.linspace(0.0, 15000.0, 20), 10
+    if workload == "c2":
+        return np.array([10000.0]), 10
+    return np.array([0.0]), 5
+
+
+def cpu_baseline(sm, rpm, num_modes, ks=(0, 1)):
+    """The oracle (numpy/scipy restatement of the reference algorithm, the
+    reference's own nev/ncv/tolerance policy) on a bounded sample, one core."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    from oracle import cyclic
+    t0 = time.perf_counter()
+    m = cyclic.build_model(sm.nodes, sm.elements, sm.node_sets, sm.num_sectors)
+    ops = cyclic.sector_operators(m, 200e9, 0.3, 7800.0, rpm)
+    t_ops = time.perf_counter() - t0
+    sols = {}
+    t1 = time.perf_counter()
+    for k in ks:
+        sols[k] = cyclic.solve_harmonic(m, ops, k, num_modes, rpm, "reference")
+    t_solve = (time.perf_counter() - t1) / len(ks)
+    n_h = sm.num_sectors // 2 + 1
+    rate = n_h / (t_ops + n_h * t_solve)
+    return rate, t_ops, t_solve, sols, m, ops
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c4", choices=["c4", "c2", "c1"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one MI355X; --gpus {args.gpus} is not supported "
+              "(the (rpm, harmonic) problems shard across GPUs with no collectives; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    tm, sm, mesh, mat, solver = build_solver(args.workload, args.precision)
+    rpms, num_modes = workload_spec(args.workload)
+    n_problems = len(rpms) * (sm.num_sectors // 2 + 1)
+    core = tm._core
+
+    print(f"[bench] workload={args.workload} problems/step={n_problems} nodes={sm.num_nodes}", file=sys.stderr,
+          flush=True)
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        solver.solve_rpm_sweep(list(rpms), num_modes)
+        print(f"[bench] warmup {w} {time.perf_counter() - t:.2f}s", file=sys.stderr, flush=True)
+    info = solver.engine_info()
+
+    core.engine_set_profiling(True)
+    core.engine_reset_kernel_stats()
+    lat = []
+    results = None
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ts = time.perf_counter()
+        results = solver.solve_rpm_sweep(list(rpms), num_modes)
+        lat += [r.solve_seconds for row in results for r in row]
+        print(f"[bench] step {s} {time.perf_counter() - ts:.2f}s", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    stats = core.engine_kernel_stats()
+    core.engine_set_profiling(False)
+
+    n_ok = sum(len(row) for row in results)
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = n_problems / (elapsed / args.steps)
+    max_res = max(float(np.max(r.residuals)) for row in results for r in row)
+
+    tri = stats.get("trisolve", {})
+    fac = stats.get("factor", {})
+    roofline = None
+    if tri.get("launches"):
+        avg_ms = tri["total_ms"] / tri["launches"]
+        bytes_per = tri["alg_bytes"] / tri["launches"]
+        ach = bytes_per / (avg_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": "trisolve (L, D, L^H sweep pair)", "achieved": round(ach, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "traffic": None, "avg_launch_ms": round(avg_ms, 3), "alg_bytes_per_launch": bytes_per}
+    factor_rate = None
+    if fac.get("launches"):
+        factor_rate = fac["alg_flops"] / (fac["total_ms"] * 1e-3) / 1e12
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "ND solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128 (complex f64)",
+        "data": "synthetic (seeded Kuhn TET10 lattice meshes, steel)",
+        "config": {"workload": {"c4": "rpm sweep bladed sector 20 rpm x 19 ND", "c2": "bladed sector 19 ND @10000 rpm",
+                                "c1": "flat disk 19 ND @0 rpm"}[args.workload],
+                   "n_dof": int(sm.num_dof), "n_free": int(info["n_free"]), "num_sectors": int(sm.num_sectors),
+                   "num_modes": num_modes, "problems_per_step": n_problems, "precision": args.precision,
+                   "nnz_L": int(info["nnz_L"]), "factor_flops": float(info["factor_flops"])},
+        "p50_latency_ms": round(1e3 * float(np.percentile(lat, 50)), 1),
+        "p99_latency_ms": round(1e3 * float(np.percentile(lat, 99)), 1),
+        "solved": n_ok,
+        "max_residual": max_res,
+        "roofline": roofline,
+        "factor_tflops": None if factor_rate is None else round(factor_rate, 2),
+        "kernels": {k: {"launches": v["launches"], "ms": round(v["total_ms"], 2)} for k, v in stats.items()},
+        "cpu_baseline": None,
+    }
+
+    if not args.no_cpu_baseline:
+        rpm_s = float(rpms[len(rpms) * 2 // 3])
+        print(f"[bench] cpu baseline sample at {rpm_s:.0f} rpm", file=sys.stderr, flush=True)
+        rate, t_ops, t_solve, sols, m, ops = cpu_baseline(sm, rpm_s, num_modes)
+        out["cpu_baseline"] = {
+            "value": round(rate, 4), "unit": "ND solves/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle restatement (scipy SuperLU + numpy Lanczos, reference policy) on the same mesh at "
+                       f"{rpm_s:.0f} rpm: operator build {t_ops:.1f}s + ND 0,1 solves {t_solve:.1f}s each; rate = "
+                       f"19 ND / (build + 19 solves)")}
+        # accuracy of the GPU results on the sampled problems
+        ir = list(rpms).index(rpm_s)
+        errs, macs = [], []
+        from turbomodal_amd.mac import compute_mac
+        for r in results[ir]:
+            if r.harmonic_index in sols:
+                ref = sols[r.harmonic_index]
+                nm = min(len(ref.frequencies), len(r.frequencies))
+                errs.append(float(np.max(np.abs(np.asarray(r.frequencies)[:nm] / ref.frequencies[:nm] - 1))))
+                sh = np.asarray(r.mode_shapes)
+                for q in range(nm):
+                    macs.append(float(compute_mac(sh[:, q], ref.mode_shapes[:, q])[0, 0]))
+        out["accuracy_vs_oracle"] = {"problems": len(errs), "max_freq_rel_err": max(errs) if errs else None,
+                                     "min_mac": min(macs) if macs else None}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

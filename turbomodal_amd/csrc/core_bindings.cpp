@@ -902,6 +902,37 @@ PYBIND11_MODULE(_core, m) {
       .def_readwrite("disk_thickness", &FluidConfig::disk_thickness)
       .def_readwrite("speed_of_sound", &FluidConfig::speed_of_sound);
 
+  // engine-wide timing of every kernel group with HIP events on the engine stream
+  m.def("engine_set_profiling", [](bool on) {
+    tm_context* c = Engine::get().ctx();
+    std::lock_guard<std::mutex> g(Engine::get().mutex());
+    Engine::check(c, tm_set_profiling(c, on ? 1 : 0));
+  });
+  m.def("engine_reset_kernel_stats", []() {
+    tm_context* c = Engine::get().ctx();
+    std::lock_guard<std::mutex> g(Engine::get().mutex());
+    Engine::check(c, tm_reset_kernel_stats(c));
+  });
+  m.def("engine_kernel_stats", []() {
+    tm_context* c = Engine::get().ctx();
+    std::lock_guard<std::mutex> g(Engine::get().mutex());
+    int n = tm_kernel_stats(c, nullptr, 0);
+    if (n < 0) Engine::check(c, n);
+    std::vector<tm_kernel_stat> v(std::max(n, 1));
+    tm_kernel_stats(c, v.data(), n);
+    py::dict d;
+    for (int i = 0; i < n; ++i) {
+      py::dict e;
+      e["launches"] = v[i].launches;
+      e["total_ms"] = v[i].total_ms;
+      e["alg_bytes"] = v[i].alg_bytes;
+      e["alg_flops"] = v[i].alg_flops;
+      d[py::str(v[i].name)] = e;
+    }
+    return d;
+  });
+  m.def("engine_free_bytes", []() { return tm_device_free_bytes(Engine::get().ctx()); });
+
   py::class_<CyclicSymmetrySolver>(m, "CyclicSymmetrySolver")
       .def(py::init<const Mesh&, const Material&, const FluidConfig&, bool>(), py::arg("mesh"), py::arg("material"),
            py::arg("fluid") = FluidConfig(), py::arg("apply_hub_constraint") = true, py::keep_alive<1, 2>())

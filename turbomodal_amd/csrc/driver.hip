@@ -12,6 +12,7 @@
 // reference obtains from its dense fallback.
 #include <algorithm>
 #include <chrono>
+#include <complex>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -138,6 +139,106 @@ void tridiag_eigen(int n, std::vector<double>& d, std::vector<double> e, std::ve
 }
 
 // ---------------------------------------------------------------------------
+// small generalised Hermitian eigenproblem (Cholesky + cyclic complex Jacobi)
+// ---------------------------------------------------------------------------
+void herm_gen_eig(int n, const std::vector<double>& A, const std::vector<double>& B, std::vector<double>& mu,
+                  std::vector<double>& Z) {
+  using cd = std::complex<double>;
+  auto at = [&](const std::vector<double>& M, int i, int j) { return cd(M[2 * (i * n + j)], M[2 * (i * n + j) + 1]); };
+  std::vector<cd> a(n * n), bm(n * n), L(n * n, cd(0, 0));
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      a[i * n + j] = 0.5 * (at(A, i, j) + std::conj(at(A, j, i)));
+      bm[i * n + j] = 0.5 * (at(B, i, j) + std::conj(at(B, j, i)));
+    }
+  for (int j = 0; j < n; ++j) {
+    double dj = bm[j * n + j].real();
+    for (int k = 0; k < j; ++k) dj -= std::norm(L[j * n + k]);
+    if (!(dj > 0)) throw NumericError("Rayleigh-Ritz mass Gram matrix is not positive definite");
+    L[j * n + j] = std::sqrt(dj);
+    for (int i = j + 1; i < n; ++i) {
+      cd s = bm[i * n + j];
+      for (int k = 0; k < j; ++k) s -= L[i * n + k] * std::conj(L[j * n + k]);
+      L[i * n + j] = s / L[j * n + j];
+    }
+  }
+  // W = L^-1 A (column by column), C = W L^-H = (L^-1 W^H)^H
+  auto fwd = [&](std::vector<cd>& X) {  // X <- L^-1 X, X n x n row-major
+    for (int c = 0; c < n; ++c)
+      for (int i = 0; i < n; ++i) {
+        cd s = X[i * n + c];
+        for (int k = 0; k < i; ++k) s -= L[i * n + k] * X[k * n + c];
+        X[i * n + c] = s / L[i * n + i];
+      }
+  };
+  std::vector<cd> W = a;
+  fwd(W);
+  std::vector<cd> Wh(n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) Wh[i * n + j] = std::conj(W[j * n + i]);
+  fwd(Wh);
+  std::vector<cd> C(n * n), V(n * n, cd(0, 0));
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) C[i * n + j] = std::conj(Wh[j * n + i]);
+  for (int i = 0; i < n; ++i) V[i * n + i] = 1.0;
+  double fro = 0;
+  for (auto& x : C) fro += std::norm(x);
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) off += std::norm(C[p * n + q]);
+    if (off <= 1e-32 * fro) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double g = std::abs(C[p * n + q]);
+        if (g <= 1e-300) continue;
+        const cd e = C[p * n + q] / g;  // phase; D = diag(1, conj(e)) makes (p,q) real
+        for (int k = 0; k < n; ++k) C[k * n + q] *= std::conj(e);
+        for (int k = 0; k < n; ++k) C[q * n + k] *= e;
+        for (int k = 0; k < n; ++k) V[k * n + q] *= std::conj(e);
+        const double app = C[p * n + p].real(), aqq = C[q * n + q].real();
+        const double zeta = (aqq - app) / (2.0 * g);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(zeta * zeta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {  // columns
+          const cd kp = C[k * n + p], kq = C[k * n + q];
+          C[k * n + p] = c * kp - s * kq;
+          C[k * n + q] = s * kp + c * kq;
+          const cd vp = V[k * n + p], vq = V[k * n + q];
+          V[k * n + p] = c * vp - s * vq;
+          V[k * n + q] = s * vp + c * vq;
+        }
+        for (int k = 0; k < n; ++k) {  // rows
+          const cd pk = C[p * n + k], qk = C[q * n + k];
+          C[p * n + k] = c * pk - s * qk;
+          C[q * n + k] = s * pk + c * qk;
+        }
+        C[p * n + q] = C[q * n + p] = 0.0;
+      }
+  }
+  // Z = L^-H V (back substitution with L^H)
+  std::vector<cd> Zc = V;
+  for (int c = 0; c < n; ++c)
+    for (int i = n - 1; i >= 0; --i) {
+      cd s = Zc[i * n + c];
+      for (int k = i + 1; k < n; ++k) s -= std::conj(L[k * n + i]) * Zc[k * n + c];
+      Zc[i * n + c] = s / std::conj(L[i * n + i]);
+    }
+  std::vector<int> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return C[x * n + x].real() < C[y * n + y].real(); });
+  mu.resize(n);
+  Z.assign(2 * (size_t)n * n, 0.0);
+  for (int q = 0; q < n; ++q) {
+    mu[q] = C[idx[q] * n + idx[q]].real();
+    for (int k = 0; k < n; ++k) {
+      Z[2 * (k * n + q)] = Zc[k * n + idx[q]].real();
+      Z[2 * (k * n + q) + 1] = Zc[k * n + idx[q]].imag();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // operator: assembly at unit omega + static prestress
 // ---------------------------------------------------------------------------
 struct BatchMem {
@@ -255,6 +356,7 @@ struct HostProblem {
   std::vector<double> theta_sorted_lambda;  // extracted eigenvalues
   std::vector<double> S_extract;            // ncv x n_extract (row-major k, c)
   std::vector<double> residual, xmx;        // per extracted pair
+  int refinements = 0;
   int ncv_final = 0;
   std::string message;
 };
@@ -708,6 +810,105 @@ class BatchLanczos {
     TM_HIP(hipStreamSynchronize(stream()));
   }
 
+  static constexpr double kRefineTol = 5e-9;  // keeps a 2x margin under the 1e-8 bar
+
+  // residual, ||K x||, x^H M x per Ritz vector (fixed-order chunk reduction)
+  void residuals(DevBuf& X, DevBuf& KX, DevBuf& MX, long long ldx, int nemax, const std::vector<int>& nc,
+                 const DevBuf& kp) {
+    DevBuf dnc = upload(nc, stream());
+    hipLaunchKernelGGL(k_spmv_multi, dim3((3 * S_.nf + 255) / 256, B_, nemax), dim3(256), 0, stream(), S_.nf,
+                       M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), kp.as<const double2*>(),
+                       X.as<double2>(), ldx, n_, dnc.as<int>(), KX.as<double2>());
+    hipLaunchKernelGGL(k_spmv_multi, dim3((3 * S_.nf + 255) / 256, B_, nemax), dim3(256), 0, stream(), S_.nf,
+                       M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), mk_ptrs_.as<const double2*>(),
+                       X.as<double2>(), ldx, n_, dnc.as<int>(), MX.as<double2>());
+    std::vector<double> lam((size_t)B_ * nemax, 0.0);
+    for (int b = 0; b < B_; ++b)
+      for (int c = 0; c < nc[b]; ++c) lam[(size_t)b * nemax + c] = probs_[b]->theta_sorted_lambda[c];
+    DevBuf dl = upload(lam, stream());
+    DevBuf part(sizeof(double) * 3 * (size_t)B_ * nemax * nchunk_);
+    hipLaunchKernelGGL(k_residual_partial, dim3(nchunk_, B_, nemax), dim3(256), 0, stream(), X.as<double2>(),
+                       KX.as<double2>(), MX.as<double2>(), ldx, n_, dl.as<double>(), nemax, dnc.as<int>(),
+                       part.as<double>(), nchunk_);
+    std::vector<double> hp_part(3 * (size_t)B_ * nemax * nchunk_);
+    TM_HIP(hipMemcpyAsync(hp_part.data(), part.p, part.bytes, hipMemcpyDeviceToHost, stream()));
+    TM_HIP(hipStreamSynchronize(stream()));
+    for (int b = 0; b < B_; ++b) {
+      HostProblem* hp = probs_[b];
+      hp->residual.assign(nc[b], 0.0);
+      hp->xmx.assign(nc[b], 0.0);
+      for (int c = 0; c < nc[b]; ++c) {
+        double num = 0, den = 0, xm = 0;
+        for (int q = 0; q < nchunk_; ++q) {
+          const double* pq = &hp_part[(((size_t)b * nemax + c) * nchunk_ + q) * 3];
+          num += pq[0];
+          den += pq[1];
+          xm += pq[2];
+        }
+        hp->residual[c] = den > 0 ? std::sqrt(num / den) : std::sqrt(num);
+        hp->xmx[c] = xm;
+      }
+    }
+  }
+
+  // block inverse iteration + Rayleigh-Ritz for the problems with nr[b] > 0
+  void refine(DevBuf& X, DevBuf& Y, DevBuf& KY, DevBuf& MY, long long ldx, int nemax, const std::vector<int>& nr,
+              const DevBuf& kp) {
+    DevBuf dnr = upload(nr, stream());
+    // Y = M X, then solve in place column by column
+    hipLaunchKernelGGL(k_spmv_multi, dim3((3 * S_.nf + 255) / 256, B_, nemax), dim3(256), 0, stream(), S_.nf,
+                       M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), mk_ptrs_.as<const double2*>(),
+                       X.as<double2>(), ldx, n_, dnr.as<int>(), Y.as<double2>());
+    BatchBufs bb = bm_.bufs(P_, B_);
+    for (int q = 0; q < nemax; ++q) {
+      hipLaunchKernelGGL(k_flag_cols, dim3((B_ + 63) / 64), dim3(64), 0, stream(), dnr.as<int>(), q, B_,
+                         act_.as<int>());
+      trisolve_batch(ctx_, M_, P_, bb, Y.as<double2>() + (long long)q * n_, ldx, act_.as<int>());
+    }
+    hipLaunchKernelGGL(k_spmv_multi, dim3((3 * S_.nf + 255) / 256, B_, nemax), dim3(256), 0, stream(), S_.nf,
+                       M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), kp.as<const double2*>(),
+                       Y.as<double2>(), ldx, n_, dnr.as<int>(), KY.as<double2>());
+    hipLaunchKernelGGL(k_spmv_multi, dim3((3 * S_.nf + 255) / 256, B_, nemax), dim3(256), 0, stream(), S_.nf,
+                       M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), mk_ptrs_.as<const double2*>(),
+                       Y.as<double2>(), ldx, n_, dnr.as<int>(), MY.as<double2>());
+    const size_t gsz = (size_t)B_ * nemax * nemax * nchunk_;
+    DevBuf pk(sizeof(double2) * gsz), pm(sizeof(double2) * gsz);
+    hipLaunchKernelGGL(k_gram_partial, dim3(nchunk_, B_, nemax * nemax), dim3(256), 0, stream(), Y.as<double2>(),
+                       KY.as<double2>(), ldx, n_, dnr.as<int>(), nemax, pk.as<double2>(), nchunk_);
+    hipLaunchKernelGGL(k_gram_partial, dim3(nchunk_, B_, nemax * nemax), dim3(256), 0, stream(), Y.as<double2>(),
+                       MY.as<double2>(), ldx, n_, dnr.as<int>(), nemax, pm.as<double2>(), nchunk_);
+    std::vector<double2> hk(gsz), hm(gsz);
+    TM_HIP(hipMemcpyAsync(hk.data(), pk.p, pk.bytes, hipMemcpyDeviceToHost, stream()));
+    TM_HIP(hipMemcpyAsync(hm.data(), pm.p, pm.bytes, hipMemcpyDeviceToHost, stream()));
+    TM_HIP(hipStreamSynchronize(stream()));
+    std::vector<double2> zc((size_t)B_ * nemax * nemax, make_double2(0.0, 0.0));
+    for (int b = 0; b < B_; ++b) {
+      const int ne = nr[b];
+      if (!ne) continue;
+      std::vector<double> A(2 * ne * ne), Bm(2 * ne * ne), mu, Z;
+      for (int p = 0; p < ne; ++p)
+        for (int q = 0; q < ne; ++q) {
+          double ar = 0, ai = 0, br = 0, bi = 0;
+          for (int c = 0; c < nchunk_; ++c) {
+            const size_t o = (((size_t)b * nemax + p) * nemax + q) * nchunk_ + c;
+            ar += hk[o].x; ai += hk[o].y;
+            br += hm[o].x; bi += hm[o].y;
+          }
+          A[2 * (p * ne + q)] = ar; A[2 * (p * ne + q) + 1] = ai;
+          Bm[2 * (p * ne + q)] = br; Bm[2 * (p * ne + q) + 1] = bi;
+        }
+      herm_gen_eig(ne, A, Bm, mu, Z);
+      HostProblem* hp = probs_[b];
+      for (int q = 0; q < ne; ++q) hp->theta_sorted_lambda[q] = mu[q];
+      for (int k = 0; k < ne; ++k)
+        for (int q = 0; q < ne; ++q)
+          zc[((size_t)b * nemax + k) * nemax + q] = make_double2(Z[2 * (k * ne + q)], Z[2 * (k * ne + q) + 1]);
+    }
+    DevBuf dz = upload(zc, stream());
+    hipLaunchKernelGGL(k_combine_c, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), Y.as<double2>(), ldx, n_,
+                       dz.as<double2>(), nemax, (long long)nemax * nemax, dnr.as<int>(), X.as<double2>(), ldx);
+  }
+
   void extract() {
     // Ritz vectors X = V S for every finished problem
     int nemax = 0;
@@ -737,57 +938,46 @@ class BatchLanczos {
     hipLaunchKernelGGL(k_combine, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), V_.as<double2>(), ldb_,
                        n_, coef_.as<double>(), ldc, (long long)(ncvmax_ + 1) * ldc, nk_.as<int>(),
                        nc_.as<int>(), X.as<double2>(), ldx);
-    // residuals ||K x - lambda M x|| / ||K x|| with FP64 K(k), M(k)
-    {
-      std::vector<const double2*> kptr(B_);
-      std::vector<DevBuf> kb(B_);
-      const double alpha_s = 2.0 * M_PI / S_.N;
-      (void)alpha_s;
+    // FP64 K(k) per problem for residuals and refinement
+    std::vector<const double2*> kptr(B_);
+    std::vector<DevBuf> kb(B_);
+    for (int b = 0; b < B_; ++b) {
+      HostProblem* hp = probs_[b];
+      if (nc[b] == 0) { kptr[b] = mk_[b]; continue; }
+      kb[b].alloc(sizeof(double2) * 9 * op_.nblk);
+      hipLaunchKernelGGL(k_build_phase_blocks, dim3((S_.nf + 127) / 128), dim3(128), 0, stream(), S_.nf,
+                         M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), M_.d_tslot.as<int>(),
+                         M_.d_pinned.as<unsigned char>(), op_.Kc.as<double>(), op_.Kp.as<double>(),
+                         op_.dKc.as<double>(), op_.dKp.as<double>(), hp->s, hp->pre, hp->pim,
+                         kb[b].as<double2>());
+      kptr[b] = kb[b].as<double2>();
+    }
+    DevBuf kp = upload(kptr, stream());
+    residuals(X, KX, MX, ldx, nemax, nc, kp);
+    // Refinement of Ritz vectors whose residual ||K x - lambda M x|| / ||K x|| is
+    // above kRefineTol: one block inverse iteration Y = (K - sigma M)^-1 M X
+    // followed by an FP64 Rayleigh-Ritz step on span(Y).  Inverse iteration damps
+    // the roundoff-level high-mode content of x by lambda_i / lambda_high.
+    DevBuf Y;
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<int> nr(B_, 0);
+      bool any = false;
       for (int b = 0; b < B_; ++b) {
         HostProblem* hp = probs_[b];
-        if (nc[b] == 0) { kptr[b] = mk_[b]; continue; }
-        kb[b].alloc(sizeof(double2) * 9 * op_.nblk);
-        hipLaunchKernelGGL(k_build_phase_blocks, dim3((S_.nf + 127) / 128), dim3(128), 0, stream(), S_.nf,
-                           M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), M_.d_tslot.as<int>(),
-                           M_.d_pinned.as<unsigned char>(), op_.Kc.as<double>(), op_.Kp.as<double>(),
-                           op_.dKc.as<double>(), op_.dKp.as<double>(), hp->s, hp->pre, hp->pim,
-                           kb[b].as<double2>());
-        kptr[b] = kb[b].as<double2>();
-      }
-      DevBuf kp = upload(kptr, stream());
-      hipLaunchKernelGGL(k_spmv_multi, dim3((3 * S_.nf + 255) / 256, B_, nemax), dim3(256), 0, stream(), S_.nf,
-                         M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), kp.as<const double2*>(),
-                         X.as<double2>(), ldx, n_, nc_.as<int>(), KX.as<double2>());
-      hipLaunchKernelGGL(k_spmv_multi, dim3((3 * S_.nf + 255) / 256, B_, nemax), dim3(256), 0, stream(), S_.nf,
-                         M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), mk_ptrs_.as<const double2*>(),
-                         X.as<double2>(), ldx, n_, nc_.as<int>(), MX.as<double2>());
-      std::vector<double> lam((size_t)B_ * nemax, 0.0);
-      for (int b = 0; b < B_; ++b)
-        for (int c = 0; c < nc[b]; ++c) lam[(size_t)b * nemax + c] = probs_[b]->theta_sorted_lambda[c];
-      DevBuf dl = upload(lam, stream());
-      DevBuf part(sizeof(double) * 3 * (size_t)B_ * nemax * nchunk_);
-      hipLaunchKernelGGL(k_residual_partial, dim3(nchunk_, B_, nemax), dim3(256), 0, stream(), X.as<double2>(),
-                         KX.as<double2>(), MX.as<double2>(), ldx, n_, dl.as<double>(), nemax, nc_.as<int>(),
-                         part.as<double>(), nchunk_);
-      std::vector<double> hp_part(3 * (size_t)B_ * nemax * nchunk_);
-      TM_HIP(hipMemcpyAsync(hp_part.data(), part.p, part.bytes, hipMemcpyDeviceToHost, stream()));
-      TM_HIP(hipStreamSynchronize(stream()));
-      for (int b = 0; b < B_; ++b) {
-        HostProblem* hp = probs_[b];
-        hp->residual.assign(nc[b], 0.0);
-        hp->xmx.assign(nc[b], 0.0);
-        for (int c = 0; c < nc[b]; ++c) {
-          double num = 0, den = 0, xm = 0;
-          for (int q = 0; q < nchunk_; ++q) {
-            const double* pq = &hp_part[(((size_t)b * nemax + c) * nchunk_ + q) * 3];
-            num += pq[0];
-            den += pq[1];
-            xm += pq[2];
-          }
-          hp->residual[c] = den > 0 ? std::sqrt(num / den) : std::sqrt(num);
-          hp->xmx[c] = xm;
+        if (nc[b] == 0) continue;
+        double worst = 0;
+        for (int c = 0; c < nc[b]; ++c) worst = std::max(worst, hp->residual[c]);
+        if (worst > kRefineTol || !(worst == worst)) {
+          nr[b] = nc[b];
+          any = true;
         }
       }
+      if (!any) break;
+      if (!Y.p) Y.alloc(sizeof(double2) * ldx * B_);
+      refine(X, Y, KX, MX, ldx, nemax, nr, kp);
+      residuals(X, KX, MX, ldx, nemax, nc, kp);
+      for (int b = 0; b < B_; ++b)
+        if (nr[b]) probs_[b]->refinements += 1;
     }
     // results: degenerate duplication + expansion (src/cyclic_solver.cpp:1153-1211)
     for (int b = 0; b < B_; ++b) {

@@ -359,6 +359,61 @@ __global__ void k_expand_modes(int nn, const int* __restrict__ free_of_full,
   for (int c = 0; c < 3; ++c) out[(3LL * i + c) * nm + q] = v[c];
 }
 
+// Gram blocks: G[b][p][q] = sum_i conj(Y_p[i]) Z_q[i], p, q < ncol_b (chunk partials)
+__global__ __launch_bounds__(256) void k_gram_partial(const double2* __restrict__ Y,
+                                                      const double2* __restrict__ Z, long long ldb,
+                                                      long long n, const int* __restrict__ ncol, int nmax,
+                                                      double2* __restrict__ part, int nchunk) {
+  const int b = blockIdx.y, pq = blockIdx.z;
+  const int p = pq / nmax, q = pq % nmax;
+  if (p >= ncol[b] || q >= ncol[b]) return;
+  __shared__ double sh[8];
+  const long long i0 = (long long)blockIdx.x * kDotChunk;
+  const double2* y = Y + b * ldb + (long long)p * n;
+  const double2* z = Z + b * ldb + (long long)q * n;
+  double re = 0.0, im = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i0 + kDotChunk && i < n; i += 256) {
+    double2 u = cconjmul(y[i], z[i]);
+    re += u.x;
+    im += u.y;
+  }
+  double tr = block_sum(re, sh);
+  double ti = block_sum(im, sh);
+  if (threadIdx.x == 0) part[(((long long)b * nmax + p) * nmax + q) * nchunk + blockIdx.x] = make_double2(tr, ti);
+}
+
+// out_b[:, c] = sum_{k < nk_b} V_b[:, k] Z_b[k][c] with complex coefficients (row-major, ldc)
+__global__ void k_combine_c(const double2* __restrict__ V, long long ldb, long long n,
+                            const double2* __restrict__ Zc, int ldc, long long cstride,
+                            const int* __restrict__ nk, double2* __restrict__ out, long long ldo) {
+  const int b = blockIdx.y;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int K = nk[b];
+  const double2* v = V + b * ldb;
+  const double2* zb = Zc + b * cstride;
+  for (int c = 0; c < K; ++c) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int k = 0; k < K; ++k) acc = cadd(acc, cmul(zb[(long long)k * ldc + c], v[(long long)k * n + i]));
+    out[b * ldo + (long long)c * n + i] = acc;
+  }
+}
+
+// copy column q of every problem: dst_b[:] = src_b[:, q] (problems with q < ncol_b)
+__global__ void k_copy_col(const double2* __restrict__ src, long long lds, long long n, int q,
+                           const int* __restrict__ ncol, double2* __restrict__ dst, long long ldd) {
+  const int b = blockIdx.y;
+  if (q >= ncol[b]) return;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dst[b * ldd + i] = src[b * lds + (long long)q * n + i];
+}
+
+__global__ void k_flag_cols(const int* __restrict__ ncol, int q, int B, int* __restrict__ act) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < B) act[b] = q < ncol[b] ? 1 : 0;
+}
+
 __global__ void k_real_to_complex(const double* __restrict__ x, double2* __restrict__ y, long long n) {
   long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i < n) y[i] = make_double2(x[i], 0.0);

@@ -56,6 +56,19 @@ __global__ void k_expand_modes(int nn, const int* free_of_full, const int* node_
                                const int* src, const int* conjq, int nm, double pre, double pim,
                                double2* out);
 __global__ void k_real_to_complex(const double* x, double2* y, long long n);
+__global__ void k_gram_partial(const double2* Y, const double2* Z, long long ldb, long long n, const int* ncol,
+                               int nmax, double2* part, int nchunk);
+__global__ void k_combine_c(const double2* V, long long ldb, long long n, const double2* Zc, int ldc,
+                            long long cstride, const int* nk, double2* out, long long ldo);
+__global__ void k_copy_col(const double2* src, long long lds, long long n, int q, const int* ncol, double2* dst,
+                           long long ldd);
+__global__ void k_flag_cols(const int* ncol, int q, int B, int* act);
+
+// Host: generalised Hermitian eigenproblem A z = mu B z (n <= a few dozen),
+// B positive definite; A, B row-major n x n complex (re, im interleaved).
+// mu ascending; Z column q (Z[k * n + q]) is B-orthonormal.
+void herm_gen_eig(int n, const std::vector<double>& A, const std::vector<double>& B, std::vector<double>& mu,
+                  std::vector<double>& Z);
 
 // Host-side symmetric tridiagonal eigensolver (implicit QL, Wilkinson shifts):
 // d (n) diagonal, e (n) off-diagonal e[i] = T(i, i+1); on exit d holds the
