@@ -82,7 +82,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one MI355X; --gpus {args.gpus} is not supported "
-              "(the (rpm, harmonic) problems shard across GPUs with no collectives; see DESIGN.md)",
+              "(the (rpm, harmonic) problems would shard across GPUs with no collectives)",
               file=sys.stderr)
         sys.exit(2)
 
