@@ -139,6 +139,83 @@ void tridiag_eigen(int n, std::vector<double>& d, std::vector<double> e, std::ve
 }
 
 // ---------------------------------------------------------------------------
+// Hermitian eigenproblem of the projected block-Lanczos matrix: Householder
+// reduction to complex tridiagonal form, a diagonal phase change to a real
+// tridiagonal, implicit QL (tridiag_eigen), back-transformation.
+// A: m x m row-major complex (re, im); theta ascending; S column q = S[k*m+q].
+// ---------------------------------------------------------------------------
+void herm_eig(int m, const std::vector<double>& A, std::vector<double>& theta, std::vector<double>& S) {
+  using cd = std::complex<double>;
+  std::vector<cd> a(m * m), Q(m * m, cd(0, 0));
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) {
+      const cd x(A[2 * (i * m + j)], A[2 * (i * m + j) + 1]);
+      const cd y(A[2 * (j * m + i)], A[2 * (j * m + i) + 1]);
+      a[i * m + j] = 0.5 * (x + std::conj(y));
+    }
+  for (int i = 0; i < m; ++i) Q[i * m + i] = 1.0;
+  std::vector<cd> v(m), p(m), wv(m);
+  for (int k = 0; k + 2 < m; ++k) {
+    double xn = 0;
+    for (int i = k + 1; i < m; ++i) xn += std::norm(a[i * m + k]);
+    xn = std::sqrt(xn);
+    if (xn == 0.0) continue;
+    const cd x0 = a[(k + 1) * m + k];
+    const cd ph = std::abs(x0) > 0 ? x0 / std::abs(x0) : cd(1, 0);
+    const cd alpha = -ph * xn;
+    std::fill(v.begin(), v.end(), cd(0, 0));
+    for (int i = k + 1; i < m; ++i) v[i] = a[i * m + k];
+    v[k + 1] -= alpha;
+    double vn = 0;
+    for (int i = k + 1; i < m; ++i) vn += std::norm(v[i]);
+    vn = std::sqrt(vn);
+    if (vn == 0.0) continue;
+    for (int i = k + 1; i < m; ++i) v[i] /= vn;
+    // p = A v (rows/cols k..m-1 touched by the reflector act on k+1..m-1)
+    for (int i = k; i < m; ++i) {
+      cd s(0, 0);
+      for (int j = k + 1; j < m; ++j) s += a[i * m + j] * v[j];
+      p[i] = s;
+    }
+    cd K(0, 0);
+    for (int i = k + 1; i < m; ++i) K += std::conj(v[i]) * p[i];
+    for (int i = k; i < m; ++i) wv[i] = 2.0 * p[i] - 2.0 * K * (i > k ? v[i] : cd(0, 0));
+    // A <- A - v w^H - w v^H (v is zero at index k)
+    for (int i = k; i < m; ++i)
+      for (int j = k; j < m; ++j) {
+        const cd vi = i > k ? v[i] : cd(0, 0), vj = j > k ? v[j] : cd(0, 0);
+        a[i * m + j] -= vi * std::conj(wv[j]) + wv[i] * std::conj(vj);
+      }
+    // Q <- Q H
+    for (int i = 0; i < m; ++i) {
+      cd s(0, 0);
+      for (int j = k + 1; j < m; ++j) s += Q[i * m + j] * v[j];
+      for (int j = k + 1; j < m; ++j) Q[i * m + j] -= 2.0 * s * std::conj(v[j]);
+    }
+  }
+  std::vector<double> d(m), e(m, 0.0);
+  std::vector<cd> phase(m, cd(1, 0));
+  for (int k = 0; k < m; ++k) d[k] = a[k * m + k].real();
+  for (int k = 0; k + 1 < m; ++k) {
+    const cd ek = a[(k + 1) * m + k];
+    const double r = std::abs(ek);
+    e[k] = r;
+    phase[k + 1] = r > 0 ? phase[k] * ek / r : phase[k];
+  }
+  std::vector<double> z;
+  tridiag_eigen(m, d, e, z);
+  theta = d;
+  S.assign(2 * (size_t)m * m, 0.0);
+  for (int i = 0; i < m; ++i)
+    for (int q = 0; q < m; ++q) {
+      cd s(0, 0);
+      for (int k = 0; k < m; ++k) s += Q[i * m + k] * phase[k] * z[(size_t)q * m + k];
+      S[2 * ((size_t)i * m + q)] = s.real();
+      S[2 * ((size_t)i * m + q) + 1] = s.imag();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // small generalised Hermitian eigenproblem (Cholesky + cyclic complex Jacobi)
 // ---------------------------------------------------------------------------
 void herm_gen_eig(int n, const std::vector<double>& A, const std::vector<double>& B, std::vector<double>& mu,
@@ -242,21 +319,25 @@ void herm_gen_eig(int n, const std::vector<double>& A, const std::vector<double>
 // operator: assembly at unit omega + static prestress
 // ---------------------------------------------------------------------------
 struct BatchMem {
-  DevBuf L, U, T, status, params;
+  DevBuf L, U, T, S, status, params;
+  int pcols = 1;
   BatchBufs bufs(const Plan&, int B) {
     BatchBufs bb;
     bb.L = L.as<double2>();
     bb.U = U.as<double2>();
     bb.T = T.as<double2>();
+    bb.S = S.as<double2>();
     bb.status = status.as<int>();
     bb.params = params.as<ProbParam>();
     bb.B = B;
     return bb;
   }
-  void alloc(const Plan& P, int B) {
+  void alloc(const Plan& P, int B, int p = 1) {
+    pcols = p;
     L.alloc(sizeof(double2) * P.panel_entries * B);
     U.alloc(sizeof(double2) * P.upool_entries * B);
-    T.alloc(sizeof(double2) * P.tvec_entries * B);
+    T.alloc(sizeof(double2) * P.tvec_entries * B * p);
+    S.alloc(sizeof(double2) * P.inv_scratch_entries * B);
     status.alloc(sizeof(int) * B);
     params.alloc(sizeof(ProbParam) * B);
   }
@@ -1082,7 +1163,7 @@ int64_t problem_footprint(const Operator& op, int num_modes, int harmonic) {
   hp.in.max_restarts = 0;
   fill_policy(S, hp);
   const int64_t n = 3LL * S.nf;
-  int64_t bytes = 16 * (P.panel_entries + P.upool_entries + P.tvec_entries);
+  int64_t bytes = 16 * (P.panel_entries + P.upool_entries + P.tvec_entries + P.inv_scratch_entries);
   bytes += 2 * 16 * (int64_t)(hp.ncv + 1) * n;       // V, MV
   bytes += 2 * 16 * n;                               // W, MW
   bytes += 3 * 16 * (int64_t)(hp.nev + 1) * n;       // restart / Ritz / residual vectors

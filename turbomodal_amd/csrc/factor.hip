@@ -246,8 +246,143 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
 }
 
 // ---------------------------------------------------------------------------
-// triangular solves
+// in-place inversion of the unit lower L11 of every front (after its
+// factorization), 64-column blocks from the last: X_JJ = L_JJ^-1 (row by row
+// in LDS), X_IJ = -(sum_{J<K<=I} X_IK L_KJ) X_JJ.  The diagonal keeps D.
 // ---------------------------------------------------------------------------
+struct InvArgs {
+  const DevSn* sn;
+  double2* L;
+  double2* S;
+  long long PE, SE;
+};
+
+__global__ __launch_bounds__(64) void k_inv_diag(InvArgs ia, const int* __restrict__ tasks, int t0, int J) {
+  const int id = tasks[t0 + blockIdx.x];
+  const int b = blockIdx.y;
+  const DevSn s = ia.sn[id];
+  const int w = 3 * s.ncol, m = w + 3 * s.nstr;
+  const int j0 = 64 * J, nb = min(64, w - j0);
+  const double2* Pn = ia.L + b * ia.PE + s.poff;
+  double2* Sc = ia.S + b * ia.SE + s.soff;  // w x 128, column-major (ld = w): X | Z
+  __shared__ double2 A[64 * 64];
+  const int t = threadIdx.x;
+  for (int i = 0; i < nb; ++i) A[i * 64 + t] = (t < i) ? Pn[(j0 + i) + (long long)(j0 + t) * m] : make_double2(0.0, 0.0);
+  __syncthreads();
+  // row i of X: X[i][j] = -L[i][j] - sum_{j<k<i} L[i][k] X[k][j]; rows k < i already hold X
+  for (int i = 1; i < nb; ++i) {
+    double2 v = make_double2(0.0, 0.0);
+    if (t < i) {
+      v = make_double2(-A[i * 64 + t].x, -A[i * 64 + t].y);
+      for (int k = t + 1; k < i; ++k) v = csub(v, cmul(A[i * 64 + k], A[k * 64 + t]));
+    }
+    __syncthreads();
+    if (t < i) A[i * 64 + t] = v;
+    __syncthreads();
+  }
+  for (int i = 0; i < nb; ++i)
+    if (t < nb) Sc[(j0 + i) + (long long)t * w] = (t < i) ? A[i * 64 + t] : make_double2(t == i ? 1.0 : 0.0, 0.0);
+}
+
+// 64x64 tile product acc += A[rows, k] B[k, cols] over k in [k0, k1); A and B
+// are read through callbacks so the same loop serves both inversion products.
+template <class FA, class FB>
+__device__ __forceinline__ void tile_gemm(double2 (&acc)[4][4], int k0, int k1, FA fa, FB fb) {
+  __shared__ double2 As[64][17];
+  __shared__ double2 Bs[16][65];
+  const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+  for (int kc = k0; kc < k1; kc += 16) {
+    const int kn = min(16, k1 - kc);
+    __syncthreads();
+    for (int idx = tid; idx < 64 * 16; idx += 256) {
+      const int r = idx % 64, kk = idx / 64;
+      As[r][kk] = kk < kn ? fa(r, kc + kk) : make_double2(0.0, 0.0);
+      const int c = idx % 64, kr = idx / 64;
+      Bs[kr][c] = kr < kn ? fb(kc + kr, c) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int kk = 0; kk < kn; ++kk) {
+      double2 a[4], bb[4];
+      for (int p = 0; p < 4; ++p) a[p] = As[tx + 16 * p][kk];
+      for (int q = 0; q < 4; ++q) bb[q] = Bs[kk][ty + 16 * q];
+      for (int p = 0; p < 4; ++p)
+        for (int q = 0; q < 4; ++q) acc[p][q] = cadd(acc[p][q], cmul(a[p], bb[q]));
+    }
+  }
+}
+
+// Z_IJ = L_IJ X_JJ (scratch columns 64..127), I > J
+__global__ __launch_bounds__(256) void k_inv_z(InvArgs ia, const TileTask* __restrict__ tasks, int t0) {
+  const TileTask tk = tasks[t0 + blockIdx.x];
+  const int b = blockIdx.y;
+  const DevSn s = ia.sn[tk.sn];
+  const int w = 3 * s.ncol, m = w + 3 * s.nstr;
+  const int i0 = 64 * tk.i0, ni = min(64, w - i0), jj0 = 64 * tk.j0, nj = min(64, w - jj0);
+  const double2* Pn = ia.L + b * ia.PE + s.poff;
+  double2* Sc = ia.S + b * ia.SE + s.soff;
+  double2 acc[4][4];
+  for (int p = 0; p < 4; ++p)
+    for (int q = 0; q < 4; ++q) acc[p][q] = make_double2(0.0, 0.0);
+  tile_gemm(acc, 0, nj,
+            [&](int r, int k) { return r < ni ? Pn[(i0 + r) + (long long)(jj0 + k) * m] : make_double2(0.0, 0.0); },
+            [&](int k, int c) { return c < nj ? Sc[(jj0 + k) + (long long)c * w] : make_double2(0.0, 0.0); });
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  for (int p = 0; p < 4; ++p)
+    for (int q = 0; q < 4; ++q) {
+      const int r = tx + 16 * p, c = ty + 16 * q;
+      if (r < ni && c < nj) Sc[(i0 + r) + (long long)(64 + c) * w] = acc[p][q];
+    }
+}
+
+// X_IJ = -sum_{J<K<=I} X_IK Z_KJ (scratch columns 0..63), I > J
+__global__ __launch_bounds__(256) void k_inv_tile(InvArgs ia, const TileTask* __restrict__ tasks, int t0) {
+  const TileTask tk = tasks[t0 + blockIdx.x];
+  const int b = blockIdx.y;
+  const DevSn s = ia.sn[tk.sn];
+  const int w = 3 * s.ncol, m = w + 3 * s.nstr;
+  const int i0 = 64 * tk.i0, ni = min(64, w - i0), jj0 = 64 * tk.j0, nj = min(64, w - jj0);
+  const double2* Pn = ia.L + b * ia.PE + s.poff;
+  double2* Sc = ia.S + b * ia.SE + s.soff;
+  double2 acc[4][4];
+  for (int p = 0; p < 4; ++p)
+    for (int q = 0; q < 4; ++q) acc[p][q] = make_double2(0.0, 0.0);
+  tile_gemm(acc, jj0 + 64, i0 + ni,
+            [&](int r, int k) {
+              if (r >= ni) return make_double2(0.0, 0.0);
+              const int gi = i0 + r;
+              if (k < gi) return Pn[gi + (long long)k * m];  // X_IK, already inverted
+              return make_double2(k == gi ? 1.0 : 0.0, 0.0);
+            },
+            [&](int k, int c) { return c < nj ? Sc[k + (long long)(64 + c) * w] : make_double2(0.0, 0.0); });
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  for (int p = 0; p < 4; ++p)
+    for (int q = 0; q < 4; ++q) {
+      const int r = tx + 16 * p, c = ty + 16 * q;
+      if (r < ni && c < nj) Sc[(i0 + r) + (long long)c * w] = make_double2(-acc[p][q].x, -acc[p][q].y);
+    }
+}
+
+// write the inverted block column J back into the panel (diagonal keeps D)
+__global__ __launch_bounds__(256) void k_inv_store(InvArgs ia, const int* __restrict__ tasks, int t0, int J) {
+  const int id = tasks[t0 + blockIdx.x];
+  const int b = blockIdx.y;
+  const DevSn s = ia.sn[id];
+  const int w = 3 * s.ncol, m = w + 3 * s.nstr;
+  const int j0 = 64 * J, nj = min(64, w - j0);
+  double2* Pn = ia.L + b * ia.PE + s.poff;
+  const double2* Sc = ia.S + b * ia.SE + s.soff;
+  for (int c = 0; c < nj; ++c)
+    for (int i = j0 + c + 1 + threadIdx.x; i < w; i += 256)
+      Pn[i + (long long)(j0 + c) * m] = Sc[i + (long long)c * w];
+}
+
+// ---------------------------------------------------------------------------
+// batched triangular solves with p right-hand sides.  Column c of problem b is
+// X + b*ldx + c*n; the update vector of front s and its pivot temp live at
+// T + b*TE*p + c*TE + toff (r entries, then w entries).
+// ---------------------------------------------------------------------------
+constexpr int PMAX = 16;
+
 struct SolveArgs {
   const DevSn* sn;
   const int* struct_nodes;
@@ -256,158 +391,187 @@ struct SolveArgs {
   double2* T;
   long long PE, TE;
   double2* X;
-  long long ldx;
+  long long ldx, n;
   const int* active;
+  int p;
 };
+
+__device__ __forceinline__ double2* tcol(const SolveArgs& sa, int b, int c, long long off) {
+  return sa.T + ((long long)b * sa.p + c) * sa.TE + off;
+}
 
 __global__ __launch_bounds__(256) void k_fwd_asm(SolveArgs sa, const int* __restrict__ level_sn, int s0) {
   const int b = blockIdx.y;
   if (sa.active && !sa.active[b]) return;
   const DevSn s = sa.sn[level_sn[s0 + blockIdx.x]];
-  const int w = 3 * s.ncol, r = 3 * s.nstr;
-  double2* x = sa.X + b * sa.ldx + 3LL * s.col0;
-  double2* ts = sa.T + b * sa.TE + s.toff;
+  const int w = 3 * s.ncol, r = 3 * s.nstr, p = sa.p;
   const int tid = threadIdx.x;
-  for (int q = tid; q < r; q += 256) ts[q] = make_double2(0.0, 0.0);
+  for (int idx = tid; idx < r * p; idx += 256) *tcol(sa, b, idx / r, s.toff + idx % r) = make_double2(0.0, 0.0);
   __syncthreads();
   for (int k = 0; k < 2; ++k) {
     const int c = k == 0 ? s.child0 : s.child1;
     if (c < 0) continue;
     const DevSn cs = sa.sn[c];
     const int rc = 3 * cs.nstr;
-    const double2* tc = sa.T + b * sa.TE + cs.toff;
     const int* rm = sa.relmap + cs.rel0;
-    for (int q = tid; q < rc; q += 256) {
+    for (int idx = tid; idx < rc * p; idx += 256) {
+      const int q = idx % rc, col = idx / rc;
       const int pos = 3 * rm[q / 3] + q % 3;
-      if (pos < w) x[pos] = cadd(x[pos], tc[q]);
-      else ts[pos - w] = cadd(ts[pos - w], tc[q]);
+      const double2 v = *tcol(sa, b, col, cs.toff + q);
+      if (pos < w) {
+        double2* x = sa.X + b * sa.ldx + (long long)col * sa.n + 3LL * s.col0 + pos;
+        *x = cadd(*x, v);
+      } else {
+        double2* t = tcol(sa, b, col, s.toff + pos - w);
+        *t = cadd(*t, v);
+      }
     }
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(256) void k_fwd_trsv(SolveArgs sa, const int* __restrict__ level_sn, int s0) {
-  const int b = blockIdx.y;
-  if (sa.active && !sa.active[b]) return;
-  const DevSn s = sa.sn[level_sn[s0 + blockIdx.x]];
-  const int w = 3 * s.ncol, m = w + 3 * s.nstr;
-  const double2* Pn = sa.L + b * sa.PE + s.poff;
-  double2* y = sa.X + b * sa.ldx + 3LL * s.col0;
-  __shared__ double2 yb[64];
-  const int tid = threadIdx.x;
-  for (int jb0 = 0; jb0 < w; jb0 += 64) {
-    const int nb = min(64, w - jb0);
-    if (tid < 64) {
-      const int lane = tid;
-      double2 yi = lane < nb ? y[jb0 + lane] : make_double2(0.0, 0.0);
-      for (int j = 0; j < nb; ++j) {
-        double2 yj = make_double2(shfl_d(yi.x, j), shfl_d(yi.y, j));
-        if (lane > j && lane < nb) yi = csub(yi, cmul(Pn[(jb0 + lane) + (long long)(jb0 + j) * m], yj));
-      }
-      if (lane < nb) {
-        y[jb0 + lane] = yi;
-        yb[lane] = yi;
-      }
-    }
-    __syncthreads();
-    for (int i = jb0 + nb + tid; i < w; i += 256) {
-      double2 acc = make_double2(0.0, 0.0);
-      for (int j = 0; j < nb; ++j) acc = cadd(acc, cmul(Pn[i + (long long)(jb0 + j) * m], yb[j]));
-      y[i] = csub(y[i], acc);
-    }
-    __syncthreads();
-  }
-}
-
-__global__ __launch_bounds__(256) void k_fwd_update(SolveArgs sa, const RowTask* __restrict__ tasks, int t0) {
+// y[i] = x[i] + sum_{j<i} Linv[i, j] x[j]  -> pivot temp (rows [r0, r1) of the pivot block)
+template <int PM>
+__global__ __launch_bounds__(64) void k_fwd_inv(SolveArgs sa, const RowTask* __restrict__ tasks, int t0) {
   const int b = blockIdx.y;
   if (sa.active && !sa.active[b]) return;
   const RowTask t = tasks[t0 + blockIdx.x];
   const DevSn s = sa.sn[t.sn];
-  const int w = 3 * s.ncol, m = w + 3 * s.nstr;
+  const int w = 3 * s.ncol, m = w + 3 * s.nstr, p = sa.p;
   const double2* Pn = sa.L + b * sa.PE + s.poff;
-  const double2* y = sa.X + b * sa.ldx + 3LL * s.col0;
-  double2* ts = sa.T + b * sa.TE + s.toff;
-  __shared__ double2 ys[512];
-  const int tid = threadIdx.x;
-  const int q = t.r0 + tid;
-  double2 acc = make_double2(0.0, 0.0);
-  for (int c0 = 0; c0 < w; c0 += 512) {
-    const int nc = min(512, w - c0);
+  const double2* x = sa.X + b * sa.ldx + 3LL * s.col0;
+  __shared__ double2 xs[64][PM];
+  const int i = t.r0 + threadIdx.x;
+  double2 acc[PM];
+  for (int c = 0; c < PM; ++c) acc[c] = make_double2(0.0, 0.0);
+  for (int j0 = 0; j0 < t.r1; j0 += 64) {
+    const int nj = min(64, t.r1 - j0);
     __syncthreads();
-    for (int j = tid; j < nc; j += 256) ys[j] = y[c0 + j];
+    for (int idx = threadIdx.x; idx < 64 * PM; idx += 64) {
+      const int jj = idx % 64, c = idx / 64;
+      xs[jj][c] = (jj < nj && c < p) ? x[(long long)c * sa.n + j0 + jj] : make_double2(0.0, 0.0);
+    }
     __syncthreads();
-    if (q < t.r1)
-      for (int j = 0; j < nc; ++j) acc = cadd(acc, cmul(Pn[(w + q) + (long long)(c0 + j) * m], ys[j]));
+    if (i < t.r1) {
+      const int jend = min(nj, i - j0 + 1);
+      for (int jj = 0; jj < jend; ++jj) {
+        const int j = j0 + jj;
+        const double2 l = j == i ? make_double2(1.0, 0.0) : Pn[i + (long long)j * m];
+        for (int c = 0; c < PM; ++c)
+          if (c < p) acc[c] = cadd(acc[c], cmul(l, xs[jj][c]));
+      }
+    }
   }
-  if (q < t.r1) ts[q] = csub(ts[q], acc);
+  if (i < t.r1)
+    for (int c = 0; c < PM; ++c)
+      if (c < p) *tcol(sa, b, c, s.toff + 3 * s.nstr + i) = acc[c];
 }
 
-// x_j = y_j / d_j - sum_q conj(L[w+q, j]) x[struct(q)]  (one wave per pivot column)
-__global__ __launch_bounds__(256) void k_bwd_update(SolveArgs sa, const RowTask* __restrict__ tasks, int t0) {
+// rows < w: write the pivot solution back into x; rows >= w: t -= L21 y
+template <int PM>
+__global__ __launch_bounds__(256) void k_fwd_upd(SolveArgs sa, const RowTask* __restrict__ tasks, int t0) {
   const int b = blockIdx.y;
   if (sa.active && !sa.active[b]) return;
   const RowTask t = tasks[t0 + blockIdx.x];
   const DevSn s = sa.sn[t.sn];
-  const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r;
+  const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r, p = sa.p;
   const double2* Pn = sa.L + b * sa.PE + s.poff;
-  double2* xb = sa.X + b * sa.ldx;
+  constexpr int CH = 2048 / PM;
+  __shared__ double2 ys[CH][PM];
+  const int i = t.r0 + threadIdx.x;
+  const bool upd = i < t.r1 && i >= w;
+  double2 acc[PM];
+  for (int c = 0; c < PM; ++c) acc[c] = make_double2(0.0, 0.0);
+  if (t.r1 > w) {
+    for (int j0 = 0; j0 < w; j0 += CH) {
+      const int nj = min(CH, w - j0);
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < CH * PM; idx += 256) {
+        const int jj = idx % CH, c = idx / CH;
+        ys[jj][c] = (jj < nj && c < p) ? *tcol(sa, b, c, s.toff + r + j0 + jj) : make_double2(0.0, 0.0);
+      }
+      __syncthreads();
+      if (upd)
+        for (int jj = 0; jj < nj; ++jj) {
+          const double2 l = Pn[i + (long long)(j0 + jj) * m];
+          for (int c = 0; c < PM; ++c)
+            if (c < p) acc[c] = cadd(acc[c], cmul(l, ys[jj][c]));
+        }
+    }
+  }
+  if (i < t.r1 && i < w) {
+    for (int c = 0; c < PM; ++c)
+      if (c < p) sa.X[b * sa.ldx + (long long)c * sa.n + 3LL * s.col0 + i] = *tcol(sa, b, c, s.toff + r + i);
+  } else if (upd) {
+    for (int c = 0; c < PM; ++c)
+      if (c < p) {
+        double2* tq = tcol(sa, b, c, s.toff + i - w);
+        *tq = csub(*tq, acc[c]);
+      }
+  }
+}
+
+// stage 0: z_j = y_j / d_j - sum_q conj(L[w+q, j]) x[struct(q)]      -> pivot temp
+// stage 1: x_j = z_j + sum_{i>j} conj(Linv[i, j]) z_i                -> x
+// one wave per pivot column j, lanes over the sum index
+template <int PM>
+__global__ __launch_bounds__(256) void k_bwd_col(SolveArgs sa, const RowTask* __restrict__ tasks, int t0,
+                                                 int stage) {
+  const int b = blockIdx.y;
+  if (sa.active && !sa.active[b]) return;
+  const RowTask t = tasks[t0 + blockIdx.x];
+  const DevSn s = sa.sn[t.sn];
+  const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r, p = sa.p;
+  const double2* Pn = sa.L + b * sa.PE + s.poff;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int j = t.r0 + wave;
   if (j >= t.r1) return;
-  double re = 0.0, im = 0.0;
-  const double2* col = Pn + (long long)j * m + w;
-  for (int q = lane; q < r; q += 64) {
-    const int node = sa.struct_nodes[s.str0 + q / 3];
-    const double2 xv = xb[3LL * node + q % 3];
-    const double2 l = col[q];
-    double2 u = cconjmul(l, xv);
-    re += u.x;
-    im += u.y;
+  const double2* col = Pn + (long long)j * m;
+  double re[PM], im[PM];
+  for (int c = 0; c < PM; ++c) re[c] = im[c] = 0.0;
+  if (stage == 0) {
+    const double2* xb = sa.X + b * sa.ldx;
+    for (int q = lane; q < r; q += 64) {
+      const double2 l = col[w + q];
+      const long long g = 3LL * sa.struct_nodes[s.str0 + q / 3] + q % 3;
+      for (int c = 0; c < PM; ++c)
+        if (c < p) {
+          const double2 u = cconjmul(l, xb[(long long)c * sa.n + g]);
+          re[c] += u.x;
+          im[c] += u.y;
+        }
+    }
+  } else {
+    for (int i = j + 1 + lane; i < w; i += 64) {
+      const double2 l = col[i];
+      for (int c = 0; c < PM; ++c)
+        if (c < p) {
+          const double2 u = cconjmul(l, *tcol(sa, b, c, s.toff + r + i));
+          re[c] += u.x;
+          im[c] += u.y;
+        }
+    }
   }
-  re = wave_sum(re);
-  im = wave_sum(im);
+  for (int c = 0; c < PM; ++c)
+    if (c < p) {
+      re[c] = wave_sum(re[c]);
+      im[c] = wave_sum(im[c]);
+    }
   if (lane == 0) {
-    double2* xj = xb + 3LL * s.col0 + j;
-    const double d = Pn[j + (long long)j * m].x;
-    *xj = make_double2(xj->x / d - re, xj->y / d - im);
-  }
-}
-
-__global__ __launch_bounds__(256) void k_bwd_trsv(SolveArgs sa, const int* __restrict__ level_sn, int s0) {
-  const int b = blockIdx.y;
-  if (sa.active && !sa.active[b]) return;
-  const DevSn s = sa.sn[level_sn[s0 + blockIdx.x]];
-  const int w = 3 * s.ncol, m = w + 3 * s.nstr;
-  const double2* Pn = sa.L + b * sa.PE + s.poff;
-  double2* x = sa.X + b * sa.ldx + 3LL * s.col0;
-  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-  const int nblk = (w + 63) / 64;
-  for (int bi = nblk - 1; bi >= 0; --bi) {
-    const int jb0 = bi * 64, nb = min(64, w - jb0), ib0 = jb0 + nb;
-    // subtract contributions of the already solved rows below the block
-    for (int jj = wave; jj < nb; jj += 4) {
-      const int j = jb0 + jj;
-      double re = 0.0, im = 0.0;
-      for (int i = ib0 + lane; i < w; i += 64) {
-        double2 u = cconjmul(Pn[i + (long long)j * m], x[i]);
-        re += u.x;
-        im += u.y;
-      }
-      re = wave_sum(re);
-      im = wave_sum(im);
-      if (lane == 0) x[j] = make_double2(x[j].x - re, x[j].y - im);
+    if (stage == 0) {
+      const double d = col[j].x;
+      for (int c = 0; c < PM; ++c)
+        if (c < p) {
+          const double2 y = sa.X[b * sa.ldx + (long long)c * sa.n + 3LL * s.col0 + j];
+          *tcol(sa, b, c, s.toff + r + j) = make_double2(y.x / d - re[c], y.y / d - im[c]);
+        }
+    } else {
+      for (int c = 0; c < PM; ++c)
+        if (c < p) {
+          const double2 z = *tcol(sa, b, c, s.toff + r + j);
+          sa.X[b * sa.ldx + (long long)c * sa.n + 3LL * s.col0 + j] = make_double2(z.x + re[c], z.y + im[c]);
+        }
     }
-    __syncthreads();
-    if (wave == 0) {
-      double2 v = lane < nb ? x[jb0 + lane] : make_double2(0.0, 0.0);
-      for (int j = nb - 1; j >= 0; --j) {
-        double2 xj = make_double2(shfl_d(v.x, j), shfl_d(v.y, j));
-        if (lane < j) v = csub(v, cconjmul(Pn[(jb0 + j) + (long long)(jb0 + lane) * m], xj));
-      }
-      if (lane < nb) x[jb0 + lane] = v;
-    }
-    __syncthreads();
   }
 }
 
@@ -435,6 +599,7 @@ static FactorArgs factor_args(const Mesh& M, const Plan& P, const OpArrays& ops,
 void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& ops,
                   const BatchBufs& bb) {
   FactorArgs fa = factor_args(M, P, ops, bb);
+  InvArgs ia{P.d_sn.as<DevSn>(), bb.L, bb.S, P.panel_entries, P.inv_scratch_entries};
   const int B = bb.B;
   KTimer t(ctx, "factor", 0, M.S.flops * B);
   for (const LevelPlan& L : P.levels) {
@@ -452,12 +617,47 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
         hipLaunchKernelGGL(k_schur_tile, dim3(st.ntile, B), dim3(256), 0, ctx->stream, fa,
                            P.d_tile.as<TileTask>(), st.tile0, st.kb);
     }
+    for (const InvStep& iv : L.inv) {
+      hipLaunchKernelGGL(k_inv_diag, dim3(iv.ndiag, B), dim3(64), 0, ctx->stream, ia, P.d_inv_diag.as<int>(),
+                         iv.diag0, iv.J);
+      if (iv.ntile) {
+        hipLaunchKernelGGL(k_inv_z, dim3(iv.ntile, B), dim3(256), 0, ctx->stream, ia,
+                           P.d_inv_tiles.as<TileTask>(), iv.tile0);
+        hipLaunchKernelGGL(k_inv_tile, dim3(iv.ntile, B), dim3(256), 0, ctx->stream, ia,
+                           P.d_inv_tiles.as<TileTask>(), iv.tile0);
+      }
+      hipLaunchKernelGGL(k_inv_store, dim3(iv.ndiag, B), dim3(256), 0, ctx->stream, ia, P.d_inv_diag.as<int>(),
+                         iv.diag0, iv.J);
+    }
   }
   TM_HIP(hipGetLastError());
 }
 
+template <int PM>
+static void trisolve_levels(Context* ctx, const Plan& P, const SolveArgs& sa, int B) {
+  const int* lsn = P.d_level_sn.as<int>();
+  for (const LevelPlan& L : P.levels) {
+    hipLaunchKernelGGL(k_fwd_asm, dim3(L.nsn, B), dim3(256), 0, ctx->stream, sa, lsn, L.sn0);
+    if (L.nfinv)
+      hipLaunchKernelGGL(k_fwd_inv<PM>, dim3(L.nfinv, B), dim3(64), 0, ctx->stream, sa, P.d_finv.as<RowTask>(),
+                         L.finv0);
+    if (L.nfup)
+      hipLaunchKernelGGL(k_fwd_upd<PM>, dim3(L.nfup, B), dim3(256), 0, ctx->stream, sa, P.d_fup.as<RowTask>(),
+                         L.fup0);
+  }
+  for (int h = static_cast<int>(P.levels.size()) - 1; h >= 0; --h) {
+    const LevelPlan& L = P.levels[h];
+    if (!L.nbcol) continue;
+    hipLaunchKernelGGL(k_bwd_col<PM>, dim3(L.nbcol, B), dim3(256), 0, ctx->stream, sa, P.d_bcol.as<RowTask>(),
+                       L.bcol0, 0);
+    hipLaunchKernelGGL(k_bwd_col<PM>, dim3(L.nbcol, B), dim3(256), 0, ctx->stream, sa, P.d_bcol.as<RowTask>(),
+                       L.bcol0, 1);
+  }
+}
+
 void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs& bb, double2* X,
-                    long long ldx, const int* active) {
+                    long long ldx, const int* active, int p) {
+  if (p < 1 || p > PMAX) throw ArgError("trisolve: 1 <= p <= 16 right-hand sides");
   SolveArgs sa;
   sa.sn = P.d_sn.as<DevSn>();
   sa.struct_nodes = M.d_struct.as<int>();
@@ -468,24 +668,15 @@ void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs&
   sa.TE = P.tvec_entries;
   sa.X = X;
   sa.ldx = ldx;
+  sa.n = 3LL * M.S.nf;
   sa.active = active;
+  sa.p = p;
   const int B = bb.B;
-  KTimer t(ctx, "trisolve", P.trisolve_bytes_c16 * B, 0);
-  const int* lsn = P.d_level_sn.as<int>();
-  for (const LevelPlan& L : P.levels) {
-    hipLaunchKernelGGL(k_fwd_asm, dim3(L.nsn, B), dim3(256), 0, ctx->stream, sa, lsn, L.sn0);
-    hipLaunchKernelGGL(k_fwd_trsv, dim3(L.nsn, B), dim3(256), 0, ctx->stream, sa, lsn, L.sn0);
-    if (L.nfup)
-      hipLaunchKernelGGL(k_fwd_update, dim3(L.nfup, B), dim3(256), 0, ctx->stream, sa,
-                         P.d_fup.as<RowTask>(), L.fup0);
-  }
-  for (int h = static_cast<int>(P.levels.size()) - 1; h >= 0; --h) {
-    const LevelPlan& L = P.levels[h];
-    if (L.nbup)
-      hipLaunchKernelGGL(k_bwd_update, dim3(L.nbup, B), dim3(256), 0, ctx->stream, sa,
-                         P.d_bup.as<RowTask>(), L.bup0);
-    hipLaunchKernelGGL(k_bwd_trsv, dim3(L.nsn, B), dim3(256), 0, ctx->stream, sa, lsn, L.sn0);
-  }
+  KTimer t(ctx, "trisolve", P.trisolve_bytes_c16 * B, 8.0 * p * B * P.trisolve_bytes_c16 / 20.0);
+  if (p == 1) trisolve_levels<1>(ctx, P, sa, B);
+  else if (p <= 4) trisolve_levels<4>(ctx, P, sa, B);
+  else if (p <= 8) trisolve_levels<8>(ctx, P, sa, B);
+  else trisolve_levels<16>(ctx, P, sa, B);
   TM_HIP(hipGetLastError());
 }
 

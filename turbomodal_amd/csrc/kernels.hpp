@@ -41,6 +41,13 @@ struct TileTask {   // 64x64 lower tile of the trailing front
   int sn, i0, j0;
 };
 
+// Inversion of the unit lower L11 of every front, 64-column block J (from the last).
+struct InvStep {
+  int J;
+  int diag0, ndiag;    // snode ids (range in inv_diag)
+  int tile0, ntile;    // (snode, row block I > J) tasks (range in inv_tiles; TileTask.i0 = I, j0 = J)
+};
+
 // Dense partial factorization plan step at one (level, kb).
 struct FactorStep {
   int kb;
@@ -52,9 +59,11 @@ struct FactorStep {
 struct LevelPlan {
   int asm0, nasm;          // front assembly tasks
   std::vector<FactorStep> steps;
+  std::vector<InvStep> inv;
   int sn0, nsn;            // snode ids of this level (range in level_sn)
-  int fup0, nfup;          // forward off-diagonal row tasks
-  int bup0, nbup;          // backward column tasks (RowTask used as column range)
+  int finv0, nfinv;        // forward: rows of L11^-1 y (64-row chunks of the pivot block)
+  int fup0, nfup;          // forward: rows [0, m) (pivot write-back + update vector)
+  int bcol0, nbcol;        // backward: pivot columns (4 per task, one wave each)
 };
 
 struct Plan {
@@ -66,9 +75,12 @@ struct Plan {
   std::vector<RowTask> row_tasks;
   std::vector<TileTask> tile_tasks;
   std::vector<int> level_sn;
-  std::vector<RowTask> fup_tasks, bup_tasks;
+  std::vector<RowTask> finv_tasks, fup_tasks, bcol_tasks;
+  std::vector<int> inv_diag;
+  std::vector<TileTask> inv_tiles;
+  int64_t inv_scratch_entries = 0;  // complex entries per problem for the inversion
   // device copies
-  DevBuf d_asm, d_diag, d_row, d_tile, d_level_sn, d_fup, d_bup;
+  DevBuf d_asm, d_diag, d_row, d_tile, d_level_sn, d_finv, d_fup, d_bcol, d_inv_diag, d_inv_tiles;
   // per-snode device data
   DevBuf d_sn;             // DevSn[]
   int64_t panel_entries = 0, upool_entries = 0, tvec_entries = 0;
@@ -77,7 +89,9 @@ struct Plan {
 
 struct DevSn {
   int col0, ncol, nstr, str0, rel0, child0, child1, pad;
-  long long poff, uoff, toff;
+  long long poff, uoff;
+  long long toff;   // per column: [r update-vector entries][w pivot temp entries]
+  long long soff;   // inversion scratch (w x 64) within its level
 };
 
 void build_plan(const Symbolic& S, Plan& P, hipStream_t stream);
@@ -109,7 +123,8 @@ struct Operator {
 struct BatchBufs {
   double2* L;        // B * panel_entries
   double2* U;        // B * upool_entries
-  double2* T;        // B * tvec_entries
+  double2* S;        // B * inv_scratch_entries
+  double2* T;        // B * p * tvec_entries (p right-hand sides)
   int* status;       // B
   const ProbParam* params;  // B
   int B;
@@ -117,9 +132,10 @@ struct BatchBufs {
 
 void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& ops,
                   const BatchBufs& bb);
-// in-place solve A x = b for each problem's vector x_b = X + b*ldx
+// in-place solve A X = B for each problem's block X_b = X + b*ldx: p columns of
+// length n = 3 nf, column c at X_b + c*n.  bb.T must hold p * tvec_entries per problem.
 void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs& bb, double2* X,
-                    long long ldx, const int* active);
+                    long long ldx, const int* active, int p = 1);
 
 struct HostProblem;
 void create_operator(Context* ctx, Mesh& M, Operator& op);

@@ -76,8 +76,20 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
     d.poff = poff;
     poff += int64_t(s.m()) * s.w();
     d.toff = toff;
-    toff += s.r();
+    toff += s.r() + s.w();
+    d.soff = 0;
   }
+  // inversion scratch: w x 128 per front (X block column | Z = L X_JJ), laid out per level
+  int64_t scratch = 0;
+  for (const auto& lvl : S.levels) {
+    int64_t o = 0;
+    for (int id : lvl) {
+      dsn[id].soff = o;
+      o += int64_t(S.sn[id].w()) * 128;
+    }
+    scratch = std::max(scratch, o);
+  }
+  P.inv_scratch_entries = std::max<int64_t>(scratch, 1);
   Pool pool;
   for (const auto& lvl : S.levels) {
     for (int id : lvl) {
@@ -99,12 +111,16 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
   P.row_tasks.clear();
   P.tile_tasks.clear();
   P.level_sn.clear();
+  P.finv_tasks.clear();
   P.fup_tasks.clear();
-  P.bup_tasks.clear();
+  P.bcol_tasks.clear();
+  P.inv_diag.clear();
+  P.inv_tiles.clear();
   const int NB = P.nb, TL = P.tile;
   constexpr int kAsmChunk = 16;   // node columns per assembly task
   constexpr int kRowChunk = 256;  // rows per panel-solve / forward-update task
-  constexpr int kColChunk = 4;    // columns per backward-update task (one wave each)
+  constexpr int kInvChunk = 64;   // rows per L11^-1 task
+  constexpr int kColChunk = 4;    // columns per backward task (one wave each)
   double tri_bytes = 0;
   for (const auto& lvl : S.levels) {
     LevelPlan L;
@@ -112,6 +128,9 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
     L.nsn = static_cast<int>(lvl.size());
     P.level_sn.insert(P.level_sn.end(), lvl.begin(), lvl.end());
     L.asm0 = static_cast<int>(P.asm_tasks.size());
+    L.finv0 = static_cast<int>(P.finv_tasks.size());
+    L.fup0 = static_cast<int>(P.fup_tasks.size());
+    L.bcol0 = static_cast<int>(P.bcol_tasks.size());
     int maxw = 0;
     for (int id : lvl) {
       const Snode& s = S.sn[id];
@@ -144,13 +163,20 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
         }
         P.asm_tasks.push_back(t);
       }
-      for (int r0 = 0; r0 < s.r(); r0 += kRowChunk)
-        P.fup_tasks.push_back({id, r0, std::min(s.r(), r0 + kRowChunk)});
+      for (int r0 = 0; r0 < s.w(); r0 += kInvChunk)
+        P.finv_tasks.push_back({id, r0, std::min(s.w(), r0 + kInvChunk)});
+      for (int r0 = 0; r0 < s.m(); r0 += kRowChunk)
+        P.fup_tasks.push_back({id, r0, std::min(s.m(), r0 + kRowChunk)});
       for (int c0 = 0; c0 < s.w(); c0 += kColChunk)
-        P.bup_tasks.push_back({id, c0, std::min(s.w(), c0 + kColChunk)});
-      tri_bytes += 2.0 * (double(s.m()) * s.w() - double(s.w()) * (s.w() - 1) / 2) * 20.0;
+        P.bcol_tasks.push_back({id, c0, std::min(s.w(), c0 + kColChunk)});
+      // per sweep pair: L21 read twice (forward, backward), L11^-1 triangle read twice
+      const double w = s.w(), r = s.r();
+      tri_bytes += 2.0 * (r * w + w * (w - 1) / 2) * 20.0;
     }
     L.nasm = static_cast<int>(P.asm_tasks.size()) - L.asm0;
+    L.nfinv = static_cast<int>(P.finv_tasks.size()) - L.finv0;
+    L.nfup = static_cast<int>(P.fup_tasks.size()) - L.fup0;
+    L.nbcol = static_cast<int>(P.bcol_tasks.size()) - L.bcol0;
     for (int kb = 0; kb < maxw; kb += NB) {
       FactorStep st{};
       st.kb = kb;
@@ -174,26 +200,23 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
       st.ntile = static_cast<int>(P.tile_tasks.size()) - st.tile0;
       L.steps.push_back(st);
     }
-    P.levels.push_back(L);
-  }
-  // forward/backward task ranges per level
-  {
-    int f = 0, b = 0;
-    for (size_t h = 0; h < S.levels.size(); ++h) {
-      LevelPlan& L = P.levels[h];
-      L.fup0 = f;
-      L.bup0 = b;
-      int nf = 0, nbk = 0;
-      for (int id : S.levels[h]) {
-        const Snode& s = S.sn[id];
-        nf += (s.r() + kRowChunk - 1) / kRowChunk;
-        nbk += (s.w() + kColChunk - 1) / kColChunk;
+    const int maxblk = (maxw + 63) / 64;
+    for (int J = maxblk - 1; J >= 0; --J) {
+      InvStep iv{};
+      iv.J = J;
+      iv.diag0 = static_cast<int>(P.inv_diag.size());
+      iv.tile0 = static_cast<int>(P.inv_tiles.size());
+      for (int id : lvl) {
+        const int nblk = (S.sn[id].w() + 63) / 64;
+        if (nblk <= J) continue;
+        P.inv_diag.push_back(id);
+        for (int I = J + 1; I < nblk; ++I) P.inv_tiles.push_back({id, I, J});
       }
-      L.nfup = nf;
-      L.nbup = nbk;
-      f += nf;
-      b += nbk;
+      iv.ndiag = static_cast<int>(P.inv_diag.size()) - iv.diag0;
+      iv.ntile = static_cast<int>(P.inv_tiles.size()) - iv.tile0;
+      L.inv.push_back(iv);
     }
+    P.levels.push_back(L);
   }
   P.trisolve_bytes_c16 = tri_bytes + 5.0 * 3 * S.nf * 16.0;
   P.d_sn = upload(dsn, stream);
@@ -202,8 +225,11 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
   P.d_row = upload(P.row_tasks, stream);
   P.d_tile = upload(P.tile_tasks, stream);
   P.d_level_sn = upload(P.level_sn, stream);
+  P.d_finv = upload(P.finv_tasks, stream);
   P.d_fup = upload(P.fup_tasks, stream);
-  P.d_bup = upload(P.bup_tasks, stream);
+  P.d_bcol = upload(P.bcol_tasks, stream);
+  P.d_inv_diag = upload(P.inv_diag, stream);
+  P.d_inv_tiles = upload(P.inv_tiles, stream);
 }
 
 }  // namespace tme

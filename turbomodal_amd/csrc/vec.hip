@@ -414,6 +414,94 @@ __global__ void k_flag_cols(const int* __restrict__ ncol, int q, int B, int* __r
   if (b < B) act[b] = q < ncol[b] ? 1 : 0;
 }
 
+// Tall-skinny C = A^H B, partial over row chunks.  A_b: n x ka (columns ldA apart,
+// problem stride lda_b), B_b: n x kb.  grid (nchunk, problems, ceil(ka/16)); each
+// block owns 16 columns of A and all kb columns of B for one chunk of rows.
+__global__ __launch_bounds__(256) void k_tsgemm_partial(const double2* __restrict__ A, long long lda_b,
+                                                        const double2* __restrict__ Bm, long long ldb_b,
+                                                        long long n, const int* __restrict__ ka_b, int kb,
+                                                        const int* __restrict__ act, int kamax,
+                                                        double2* __restrict__ part, int nchunk) {
+  const int b = blockIdx.y;
+  if (act && !act[b]) return;
+  const int ka = ka_b[b];
+  const int a0 = blockIdx.z * 16;
+  if (a0 >= ka) return;
+  const int na = min(16, ka - a0);
+  __shared__ double2 As[64][17];
+  __shared__ double2 Bs[64][17];
+  const int tid = threadIdx.x;
+  const int oa = tid / 16, oc = tid % 16;  // output (a0 + oa, oc)
+  double re = 0.0, im = 0.0;
+  const long long r0 = (long long)blockIdx.x * kTsChunk;
+  const long long r1 = r0 + kTsChunk < n ? r0 + kTsChunk : n;
+  const double2* Ab = A + b * lda_b;
+  const double2* Bb = Bm + b * ldb_b;
+  for (long long rr = r0; rr < r1; rr += 64) {
+    const int nr = (int)(r1 - rr < 64 ? r1 - rr : 64);
+    __syncthreads();
+    for (int idx = tid; idx < 64 * 16; idx += 256) {
+      const int r = idx % 64, c = idx / 64;
+      As[r][c] = (r < nr && c < na) ? Ab[(long long)(a0 + c) * n + rr + r] : make_double2(0.0, 0.0);
+      Bs[r][c] = (r < nr && c < kb) ? Bb[(long long)c * n + rr + r] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int r = 0; r < nr; ++r) {
+      const double2 u = cconjmul(As[r][oa], Bs[r][oc]);
+      re += u.x;
+      im += u.y;
+    }
+  }
+  if (oa < na && oc < kb)
+    part[(((long long)b * kamax + a0 + oa) * 16 + oc) * nchunk + blockIdx.x] = make_double2(re, im);
+}
+
+// C[b][a][c] = sum over chunks (fixed order)
+__global__ void k_tsgemm_reduce(const double2* __restrict__ part, const int* __restrict__ ka_b, int kb,
+                                const int* __restrict__ act, int kamax, int nchunk, double2* __restrict__ C) {
+  const int b = blockIdx.y;
+  if (act && !act[b]) return;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = idx / 16, c = idx % 16;
+  if (a >= ka_b[b] || c >= kb) return;
+  const double2* p = part + (((long long)b * kamax + a) * 16 + c) * nchunk;
+  double re = 0.0, im = 0.0;
+  for (int q = 0; q < nchunk; ++q) {
+    re += p[q].x;
+    im += p[q].y;
+  }
+  C[((long long)b * kamax + a) * 16 + c] = make_double2(re, im);
+}
+
+// Y_b[:, c] = beta Y_b[:, c] + sum_{a < ka_b} A_b[:, a] C_b[a][c], c < kc
+// (C_b row-major kamax x 16); one thread per row
+__global__ __launch_bounds__(256) void k_tsmm(const double2* __restrict__ A, long long lda_b, long long n,
+                                              const double2* __restrict__ C, const int* __restrict__ ka_b,
+                                              int kc, const int* __restrict__ act, int kamax, double beta,
+                                              double2* __restrict__ Y, long long ldy_b) {
+  const int b = blockIdx.y;
+  if (act && !act[b]) return;
+  const int ka = ka_b[b];
+  __shared__ double2 Cs[2304];
+  for (int idx = threadIdx.x; idx < ka * 16; idx += 256) Cs[idx] = C[(long long)b * kamax * 16 + idx];
+  __syncthreads();
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2* Ab = A + b * lda_b;
+  double2 acc[16];
+  for (int c = 0; c < 16; ++c) acc[c] = make_double2(0.0, 0.0);
+  for (int a = 0; a < ka; ++a) {
+    const double2 x = Ab[(long long)a * n + i];
+    for (int c = 0; c < 16; ++c)
+      if (c < kc) acc[c] = cadd(acc[c], cmul(x, Cs[a * 16 + c]));
+  }
+  double2* Yb = Y + b * ldy_b;
+  for (int c = 0; c < kc; ++c) {
+    double2 y = beta != 0.0 ? Yb[(long long)c * n + i] : make_double2(0.0, 0.0);
+    Yb[(long long)c * n + i] = make_double2(beta * y.x + acc[c].x, beta * y.y + acc[c].y);
+  }
+}
+
 __global__ void k_real_to_complex(const double* __restrict__ x, double2* __restrict__ y, long long n) {
   long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i < n) y[i] = make_double2(x[i], 0.0);

@@ -6,6 +6,7 @@
 namespace tme {
 
 constexpr int kDotChunk = 2048;  // rows per partial-dot block (8 per thread)
+constexpr int kTsChunk = 4096;   // rows per tall-skinny A^H B block
 
 // Device-side state of a batch of Lanczos runs.
 struct LzState {
@@ -63,6 +64,16 @@ __global__ void k_combine_c(const double2* V, long long ldb, long long n, const 
 __global__ void k_copy_col(const double2* src, long long lds, long long n, int q, const int* ncol, double2* dst,
                            long long ldd);
 __global__ void k_flag_cols(const int* ncol, int q, int B, int* act);
+
+__global__ void k_tsgemm_partial(const double2* A, long long lda_b, const double2* Bm, long long ldb_b, long long n,
+                                 const int* ka_b, int kb, const int* act, int kamax, double2* part, int nchunk);
+__global__ void k_tsgemm_reduce(const double2* part, const int* ka_b, int kb, const int* act, int kamax, int nchunk,
+                                double2* C);
+__global__ void k_tsmm(const double2* A, long long lda_b, long long n, const double2* C, const int* ka_b, int kc,
+                       const int* act, int kamax, double beta, double2* Y, long long ldy_b);
+
+// Host: Hermitian eigenproblem (Householder + QL), theta ascending, S[k*m+q] (re, im)
+void herm_eig(int m, const std::vector<double>& A, std::vector<double>& theta, std::vector<double>& S);
 
 // Host: generalised Hermitian eigenproblem A z = mu B z (n <= a few dozen),
 // B positive definite; A, B row-major n x n complex (re, im interleaved).
