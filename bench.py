@@ -78,6 +78,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c4", choices=["c4", "c2", "c1"])
     ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--method", default="block", choices=["block", "lanczos"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -87,6 +88,7 @@ def main():
         sys.exit(2)
 
     tm, sm, mesh, mat, solver = build_solver(args.workload, args.precision)
+    solver.engine_method = args.method
     rpms, num_modes = workload_spec(args.workload)
     n_problems = len(rpms) * (sm.num_sectors // 2 + 1)
     core = tm._core
@@ -149,6 +151,7 @@ def main():
                                 "c1": "flat disk 19 ND @0 rpm"}[args.workload],
                    "n_dof": int(sm.num_dof), "n_free": int(info["n_free"]), "num_sectors": int(sm.num_sectors),
                    "num_modes": num_modes, "problems_per_step": n_problems, "precision": args.precision,
+                   "eigensolver": args.method,
                    "nnz_L": int(info["nnz_L"]), "factor_flops": float(info["factor_flops"])},
         "p50_latency_ms": round(1e3 * float(np.percentile(lat, 50)), 1),
         "p99_latency_ms": round(1e3 * float(np.percentile(lat, 99)), 1),
@@ -157,6 +160,11 @@ def main():
         "roofline": roofline,
         "factor_tflops": None if factor_rate is None else round(factor_rate, 2),
         "kernels": {k: {"launches": v["launches"], "ms": round(v["total_ms"], 2)} for k, v in stats.items()},
+        "sweeps_per_step": tri.get("launches", 0) / args.steps,
+        "restarts": {"max": int(max(r.restarts for row in results for r in row)),
+                     "mean": round(float(np.mean([r.restarts for row in results for r in row])), 2),
+                     "worst": sorted(((int(r.restarts), int(r.harmonic_index), float(r.rpm))
+                                      for row in results for r in row), reverse=True)[:5]},
         "cpu_baseline": None,
     }
 

@@ -60,6 +60,15 @@ extern "C" {
 #define TM_CONV_TIGHT 0      /* relative Ritz residual <= tol_rel (default 1e-12) */
 #define TM_CONV_REFERENCE 1  /* |beta s_i| < 1e-8 max(1,|theta_i|) (src/hermitian_lanczos.cpp:414-424) */
 
+/* Eigensolver (tm_problem.method) */
+#define TM_METHOD_LANCZOS 0  /* single-vector shift-invert Lanczos, exact-shift implicit restarts
+                                (src/hermitian_lanczos.cpp), conv_policy applies */
+#define TM_METHOD_BLOCK 1    /* block shift-invert Krylov (8 or 16 vectors per triangular sweep),
+                                FP64 Rayleigh-Ritz with K(k), M(k); converged when
+                                ||K x - lambda M x|| / ||K x|| <= max(1e-9, 4 x its FP64 floor).
+                                Problems with fewer than 256 free DOFs, and TM_CONV_REFERENCE,
+                                run the Lanczos method */
+
 typedef struct tm_context tm_context;
 typedef struct tm_mesh tm_mesh;
 typedef struct tm_operator tm_operator;
@@ -137,7 +146,7 @@ typedef struct tm_problem {
   double shift;              /* sigma; the reference uses 1.0 */
   double tol_rel;            /* TM_CONV_TIGHT threshold (<= 0: 1e-12) */
   int32_t max_restarts;      /* <= 0: 1000 as SolverConfig (modal_solver.hpp:39) */
-  int32_t reserved;
+  int32_t method;            /* TM_METHOD_* */
   double min_frequency;      /* Hz; > 0: solve 6 extra modes and drop f <= min_frequency
                                 (src/cyclic_solver.cpp:1011, 1058-1084) */
 } tm_problem;
@@ -166,10 +175,10 @@ typedef struct tm_result {
 int tm_solve_batch(tm_context* ctx, const tm_problem* problems, int32_t n,
                    tm_result* results);
 
-/* Device bytes one problem holds while in flight (factor + Lanczos basis);
+/* Device bytes one problem holds while in flight (factor + Krylov basis);
  * the serving loop admits requests against this. */
 int64_t tm_problem_footprint(const tm_operator* op, int32_t num_modes,
-                             int32_t precision, int32_t harmonic);
+                             int32_t precision, int32_t harmonic, int32_t method);
 int64_t tm_device_free_bytes(tm_context* ctx);
 
 /* Diagnostics used by the parity tests: the assembled free-space matrices

@@ -25,6 +25,7 @@ TM_ERR_MEMORY = -5
 TM_NODE_INTERIOR, TM_NODE_LEFT, TM_NODE_RIGHT = 0, 1, 2
 TM_PREC_FP64, TM_PREC_MIXED = 0, 1
 TM_CONV_TIGHT, TM_CONV_REFERENCE = 0, 1
+TM_METHOD_LANCZOS, TM_METHOD_BLOCK = 0, 1
 
 # Every function declared in include/tmhip.h.
 EXPORTS = (
@@ -53,7 +54,7 @@ class Problem(C.Structure):
     _fields_ = [("op", C.c_void_p), ("rpm", C.c_double), ("harmonic", C.c_int32),
                 ("num_modes", C.c_int32), ("precision", C.c_int32), ("conv_policy", C.c_int32),
                 ("shift", C.c_double), ("tol_rel", C.c_double), ("max_restarts", C.c_int32),
-                ("reserved", C.c_int32), ("min_frequency", C.c_double)]
+                ("method", C.c_int32), ("min_frequency", C.c_double)]
 
 
 class Result(C.Structure):
@@ -94,7 +95,7 @@ def lib():
     L.tm_operator_create.argtypes = [vp, vp, dbl, dbl, dbl, i32, P(vp)]
     L.tm_operator_destroy.argtypes = [vp]
     L.tm_solve_batch.argtypes = [vp, P(Problem), i32, P(Result)]
-    L.tm_problem_footprint.argtypes = [vp, i32, i32, i32]
+    L.tm_problem_footprint.argtypes = [vp, i32, i32, i32, i32]
     L.tm_problem_footprint.restype = i64
     L.tm_device_free_bytes.argtypes = [vp]
     L.tm_device_free_bytes.restype = i64
@@ -223,8 +224,9 @@ class EngineOperator:
         self.mesh.ctx.check(self.mesh.ctx.L.tm_operator_static_u(self.h, _ptr(u, C.c_double)))
         return u
 
-    def footprint(self, num_modes, harmonic, precision=TM_PREC_FP64):
-        return int(self.mesh.ctx.L.tm_problem_footprint(self.h, num_modes, precision, harmonic))
+    def footprint(self, num_modes, harmonic, precision=TM_PREC_FP64, method=None):
+        method = TM_METHOD_BLOCK if method is None else method
+        return int(self.mesh.ctx.L.tm_problem_footprint(self.h, num_modes, precision, harmonic, method))
 
 
 @dataclass
@@ -245,7 +247,7 @@ class SolveOutput:
 
 def solve_batch(ctx: Context, problems, want_shapes=True):
     """problems: list of dicts {op, rpm, harmonic, num_modes, [precision, conv_policy, shift,
-    tol_rel, max_restarts, min_frequency_extra]} -> list[SolveOutput]."""
+    tol_rel, max_restarts, method, min_frequency]} -> list[SolveOutput]."""
     n = len(problems)
     parr = (Problem * n)()
     rarr = (Result * n)()
@@ -261,6 +263,7 @@ def solve_batch(ctx: Context, problems, want_shapes=True):
         parr[i].shift = float(p.get("shift", 1.0))
         parr[i].tol_rel = float(p.get("tol_rel", 1e-12))
         parr[i].max_restarts = int(p.get("max_restarts", 1000))
+        parr[i].method = int(p.get("method", TM_METHOD_BLOCK))
         parr[i].min_frequency = float(p.get("min_frequency", 0.0))
         nm = parr[i].num_modes
         nn = op.mesh.n_nodes

@@ -251,11 +251,11 @@ int tm_solve_batch(tm_context* ctx, const tm_problem* problems, int32_t n, tm_re
 }
 
 int64_t tm_problem_footprint(const tm_operator* op, int32_t num_modes, int32_t precision,
-                             int32_t harmonic) {
+                             int32_t harmonic, int32_t method) {
   if (!op) return -1;
   (void)precision;
   try {
-    return problem_footprint(*op, num_modes, harmonic);
+    return problem_footprint(*op, num_modes, harmonic, method);
   } catch (...) {
     return -1;
   }

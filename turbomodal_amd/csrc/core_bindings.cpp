@@ -559,6 +559,12 @@ class CyclicSymmetrySolver {
   tm_mesh* emesh_ = nullptr;
   tm_operator* op_ = nullptr;
 
+ public:
+  // eigensolver of the engine: TM_METHOD_BLOCK (default) or TM_METHOD_LANCZOS
+  int method_ = TM_METHOD_BLOCK;
+
+ private:
+
   void check_fluid() const {
     if (fluid_.type != FluidConfig::Type::NONE && fluid_.fluid_density > 0.0)
       throw std::runtime_error("fluid coupling is not part of the MI355X cyclic solve (dry structures only)");
@@ -621,7 +627,10 @@ class CyclicSymmetrySolver {
     const size_t np = rpms.size() * ks.size();
     std::vector<tm_problem> probs(np);
     std::vector<tm_result> res(np);
-    std::vector<std::vector<double>> freq(np), shapes(np), ev(np), rsd(np), mass(np);
+    std::vector<std::vector<double>> freq(np), ev(np), rsd(np), mass(np);
+    // mode shapes land directly in the numpy arrays handed back (no staging copy)
+    std::vector<py::array_t<std::complex<double>>> shapes;
+    shapes.reserve(np);
     std::vector<std::vector<int32_t>> whirl(np);
     size_t q = 0;
     for (double rpm : rpms)
@@ -636,17 +645,17 @@ class CyclicSymmetrySolver {
         p.shift = 1.0;
         p.tol_rel = 1e-12;
         p.max_restarts = 1000;
-        p.reserved = 0;
+        p.method = method_;
         p.min_frequency = min_freq;
         freq[q].assign(num_modes, 0.0);
         ev[q].assign(num_modes, 0.0);
         rsd[q].assign(num_modes, 0.0);
         mass[q].assign(num_modes, 0.0);
         whirl[q].assign(num_modes, 0);
-        shapes[q].assign(2 * 3 * (size_t)nn * num_modes, 0.0);
+        shapes.emplace_back(std::vector<py::ssize_t>{3 * (py::ssize_t)nn, num_modes});
         tm_result& r = res[q];
         r.frequencies = freq[q].data();
-        r.mode_shapes = shapes[q].data();
+        r.mode_shapes = reinterpret_cast<double*>(shapes[q].mutable_data());
         r.whirl = whirl[q].data();
         r.eigenvalues = ev[q].data();
         r.residuals = rsd[q].data();
@@ -696,9 +705,13 @@ class CyclicSymmetrySolver {
         m.converged = r.converged != 0;
         m.restarts = r.restarts;
         m.solve_seconds = r.seconds;
-        py::array_t<std::complex<double>> a(std::vector<py::ssize_t>{3 * (py::ssize_t)nn, r.n_modes});
-        std::memcpy(a.mutable_data(), shapes[q].data(), sizeof(std::complex<double>) * 3 * (size_t)nn * r.n_modes);
-        m.mode_shapes = a;
+        if (r.n_modes == num_modes) {
+          m.mode_shapes = shapes[q];
+        } else {  // fewer modes: the engine wrote [3 nn][n_modes] at the front of the buffer
+          py::array_t<std::complex<double>> a(std::vector<py::ssize_t>{3 * (py::ssize_t)nn, r.n_modes});
+          std::memcpy(a.mutable_data(), shapes[q].data(), sizeof(std::complex<double>) * 3 * (size_t)nn * r.n_modes);
+          m.mode_shapes = a;
+        }
         out[ir].push_back(std::move(m));
       }
     return out;
@@ -947,6 +960,14 @@ PYBIND11_MODULE(_core, m) {
            py::arg("num_modes_per_harmonic"), py::arg("harmonic_indices") = std::vector<int>{},
            py::arg("max_threads") = 0, py::arg("include_coriolis") = false, py::arg("min_frequency") = 0.0)
       .def("engine_info", &CyclicSymmetrySolver::engine_info)
+      .def_property(
+          "engine_method",
+          [](const CyclicSymmetrySolver& s) { return std::string(s.method_ == TM_METHOD_BLOCK ? "block" : "lanczos"); },
+          [](CyclicSymmetrySolver& s, const std::string& v) {
+            if (v == "block") s.method_ = TM_METHOD_BLOCK;
+            else if (v == "lanczos") s.method_ = TM_METHOD_LANCZOS;
+            else throw std::invalid_argument("engine_method: 'block' or 'lanczos'");
+          })
       .def("build_constrained_dofs", &CyclicSymmetrySolver::build_constrained_dofs)
       .def_static("compute_stationary_frame", &CyclicSymmetrySolver::compute_stationary_frame,
                   py::arg("rotating_result"), py::arg("num_sectors"));

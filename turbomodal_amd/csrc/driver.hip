@@ -14,9 +14,12 @@
 #include <chrono>
 #include <complex>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <random>
+#include <thread>
 
 #include "vec.hpp"
 
@@ -51,6 +54,22 @@ void Context::flush_timing() {
     event_pool.push_back(p.b);
   }
   pending.clear();
+}
+
+static double wall_ms() {
+  return 1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+HostTimer::HostTimer(Context* c, const char* n) : ctx(c), name(n) {
+  if (ctx && ctx->profiling) t0 = wall_ms();
+}
+
+HostTimer::~HostTimer() {
+  if (ctx && ctx->profiling) {
+    KernelStat& s = ctx->stats[std::string("host:") + name];
+    s.launches += 1;
+    s.total_ms += wall_ms() - t0;
+  }
 }
 
 KTimer::KTimer(Context* c, const char* n, double by, double fl) : ctx(c), name(n), bytes(by), flops(fl) {
@@ -215,6 +234,88 @@ void herm_eig(int m, const std::vector<double>& A, std::vector<double>& theta, s
     }
 }
 
+bool gen_eig_chol(int m, const std::vector<double>& H, const std::vector<double>& G, std::vector<double>& lam,
+                  std::vector<double>& X) {
+  using cd = std::complex<double>;
+  auto at = [m](const std::vector<double>& A, int i, int j) {
+    return cd(A[2 * ((size_t)i * m + j)], A[2 * ((size_t)i * m + j) + 1]);
+  };
+  std::vector<cd> L((size_t)m * m, cd(0, 0));
+  double gmax = 0;
+  for (int i = 0; i < m; ++i) gmax = std::max(gmax, std::fabs(G[2 * ((size_t)i * m + i)]));
+  for (int j = 0; j < m; ++j) {
+    double dj = 0.5 * (at(G, j, j).real() + at(G, j, j).real());
+    for (int k = 0; k < j; ++k) dj -= std::norm(L[(size_t)j * m + k]);
+    if (!(dj > 1e-13 * gmax)) return false;
+    const double ljj = std::sqrt(dj);
+    L[(size_t)j * m + j] = ljj;
+    for (int i = j + 1; i < m; ++i) {
+      cd s = 0.5 * (at(G, i, j) + std::conj(at(G, j, i)));
+      for (int k = 0; k < j; ++k) s -= L[(size_t)i * m + k] * std::conj(L[(size_t)j * m + k]);
+      L[(size_t)i * m + j] = s / ljj;
+    }
+  }
+  // Y = L^-1 H (columns), then C = L^-1 Y^H (= L^-1 H L^-H, Hermitian)
+  std::vector<cd> Y((size_t)m * m), C((size_t)m * m);
+  for (int c = 0; c < m; ++c)
+    for (int i = 0; i < m; ++i) {
+      cd s = 0.5 * (at(H, i, c) + std::conj(at(H, c, i)));
+      for (int k = 0; k < i; ++k) s -= L[(size_t)i * m + k] * Y[(size_t)k * m + c];
+      Y[(size_t)i * m + c] = s / L[(size_t)i * m + i];
+    }
+  for (int c = 0; c < m; ++c)
+    for (int i = 0; i < m; ++i) {
+      cd s = std::conj(Y[(size_t)c * m + i]);
+      for (int k = 0; k < i; ++k) s -= L[(size_t)i * m + k] * C[(size_t)k * m + c];
+      C[(size_t)i * m + c] = s / L[(size_t)i * m + i];
+    }
+  std::vector<double> A(2 * (size_t)m * m), S;
+  for (size_t q = 0; q < (size_t)m * m; ++q) {
+    A[2 * q] = C[q].real();
+    A[2 * q + 1] = C[q].imag();
+  }
+  herm_eig(m, A, lam, S);
+  // X = L^-H S (back substitution with the upper factor L^H)
+  X.assign(2 * (size_t)m * m, 0.0);
+  std::vector<cd> x(m);
+  for (int q = 0; q < m; ++q) {
+    for (int i = m - 1; i >= 0; --i) {
+      cd s(S[2 * ((size_t)i * m + q)], S[2 * ((size_t)i * m + q) + 1]);
+      for (int k = i + 1; k < m; ++k) s -= std::conj(L[(size_t)k * m + i]) * x[k];
+      x[i] = s / L[(size_t)i * m + i];
+    }
+    for (int i = 0; i < m; ++i) {
+      X[2 * ((size_t)i * m + q)] = x[i].real();
+      X[2 * ((size_t)i * m + q) + 1] = x[i].imag();
+    }
+  }
+  return true;
+}
+
+// Runs f(i) for i in [0, n) on up to hardware_concurrency() threads (static split;
+// the result does not depend on the split).
+template <class F>
+static void parallel_for(int n, F&& f) {
+  const int nt = std::max(1, std::min<int>(n, (int)std::thread::hardware_concurrency()));
+  if (nt <= 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  std::vector<std::exception_ptr> err(nt);
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      try {
+        for (int i = t; i < n; i += nt) f(i);
+      } catch (...) {
+        err[t] = std::current_exception();
+      }
+    });
+  for (auto& x : th) x.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+}
+
 // ---------------------------------------------------------------------------
 // small generalised Hermitian eigenproblem (Cholesky + cyclic complex Jacobi)
 // ---------------------------------------------------------------------------
@@ -332,14 +433,14 @@ struct BatchMem {
     bb.B = B;
     return bb;
   }
-  void alloc(const Plan& P, int B, int p = 1) {
+  void alloc(DevPool* pool, const Plan& P, int B, int p = 1) {
     pcols = p;
-    L.alloc(sizeof(double2) * P.panel_entries * B);
-    U.alloc(sizeof(double2) * P.upool_entries * B);
-    T.alloc(sizeof(double2) * P.tvec_entries * B * p);
-    S.alloc(sizeof(double2) * P.inv_scratch_entries * B);
-    status.alloc(sizeof(int) * B);
-    params.alloc(sizeof(ProbParam) * B);
+    L.alloc(pool, sizeof(double2) * P.panel_entries * B);
+    U.alloc(pool, sizeof(double2) * P.upool_entries * B);
+    T.alloc(pool, sizeof(double2) * P.tvec_entries * B * p);
+    S.alloc(pool, sizeof(double2) * P.inv_scratch_entries * B);
+    status.alloc(pool, sizeof(int) * B);
+    params.alloc(pool, sizeof(ProbParam) * B);
   }
 };
 
@@ -378,7 +479,7 @@ void create_operator(Context* ctx, Mesh& M, Operator& op) {
     TM_HIP(hipMemsetAsync(op.dKc.p, 0, op.dKc.bytes, st));
     TM_HIP(hipMemsetAsync(op.dKp.p, 0, op.dKp.bytes, st));
     BatchMem bm;
-    bm.alloc(M.plan, 1);
+    bm.alloc(nullptr, M.plan, 1);
     ProbParam pp{1.0, 0.0, 0.0, 0.0};
     TM_HIP(hipMemcpyAsync(bm.params.p, &pp, sizeof(pp), hipMemcpyHostToDevice, st));
     TM_HIP(hipMemsetAsync(bm.status.p, 0, sizeof(int), st));
@@ -463,16 +564,32 @@ static void fill_policy(const Symbolic& S, HostProblem& hp) {
 
 class BatchLanczos {
  public:
-  BatchLanczos(Context* ctx, Operator& op, std::vector<HostProblem*>& probs)
-      : ctx_(ctx), op_(op), M_(*op.mesh), S_(M_.S), P_(M_.plan), probs_(probs) {}
+  BatchLanczos(Context* ctx, Operator& op, std::vector<HostProblem*>& probs, bool block)
+      : ctx_(ctx), op_(op), M_(*op.mesh), S_(M_.S), P_(M_.plan), probs_(probs), block_(block) {}
 
   void run() {
     auto t0 = std::chrono::steady_clock::now();
-    setup();
-    factor();
-    start_vectors();
-    iterate();
-    extract();
+    {
+      HostTimer h(ctx_, "setup");
+      setup();
+    }
+    {
+      HostTimer h(ctx_, "factor");
+      factor();
+    }
+    {
+      HostTimer h(ctx_, "iterate");
+      if (block_) {
+        block_iterate();
+      } else {
+        start_vectors();
+        iterate();
+      }
+    }
+    {
+      HostTimer h(ctx_, "extract");
+      extract();
+    }
     double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (auto* hp : probs_)
       if (hp->out) hp->out->seconds = sec;
@@ -495,6 +612,7 @@ class BatchLanczos {
   std::vector<int> h_jcur_, h_ncv_, h_live_;
 
   hipStream_t stream() const { return ctx_->stream; }
+  DevPool* pool() const { return &ctx_->pool; }
 
   void setup() {
     B_ = static_cast<int>(probs_.size());
@@ -508,19 +626,11 @@ class BatchLanczos {
     kmax_ = ncvmax_ + 1;
     ldb_ = (long long)(ncvmax_ + 1) * n_;
     nchunk_ = static_cast<int>((n_ + kDotChunk - 1) / kDotChunk);
-    bm_.alloc(P_, B_);
-    V_.alloc(sizeof(double2) * ldb_ * B_);
-    MV_.alloc(sizeof(double2) * ldb_ * B_);
-    W_.alloc(sizeof(double2) * n_ * B_);
-    MW_.alloc(sizeof(double2) * n_ * B_);
-    h_.alloc(sizeof(double2) * kmax_ * B_);
-    part_.alloc(sizeof(double2) * (size_t)kmax_ * nchunk_ * B_);
-    alpha_.alloc(sizeof(double) * ncvmax_ * B_);
-    beta_.alloc(sizeof(double) * ncvmax_ * B_);
-    bnew_.alloc(sizeof(double) * B_);
-    for (DevBuf* b : {&jcur_, &ncv_, &live_, &frozen_, &act_}) b->alloc(sizeof(int) * B_);
-    nk_.alloc(sizeof(int) * B_);
-    nc_.alloc(sizeof(int) * B_);
+    // right-hand sides per triangular sweep: the Krylov block, and the Ritz
+    // vectors of the refinement step (at most 16 per sweep)
+    const int pcols = std::min(16, std::max(block_ ? (nevmax_ <= 6 ? 8 : 16) : 1, nevmax_));
+    bm_.alloc(pool(), P_, B_, pcols);
+    act_.alloc(pool(), sizeof(int) * B_);
     // params and M(k)
     std::vector<ProbParam> pp(B_);
     mk_.resize(B_);
@@ -533,7 +643,23 @@ class BatchLanczos {
     }
     TM_HIP(hipMemcpyAsync(bm_.params.p, pp.data(), sizeof(ProbParam) * B_, hipMemcpyHostToDevice, stream()));
     TM_HIP(hipMemsetAsync(bm_.status.p, 0, sizeof(int) * B_, stream()));
-    mk_ptrs_ = upload(mk_, stream());
+    mk_ptrs_ = upload(pool(), mk_, stream());
+    if (block_) {
+      block_setup();
+      return;
+    }
+    V_.alloc(pool(), sizeof(double2) * ldb_ * B_);
+    MV_.alloc(pool(), sizeof(double2) * ldb_ * B_);
+    W_.alloc(pool(), sizeof(double2) * n_ * B_);
+    MW_.alloc(pool(), sizeof(double2) * n_ * B_);
+    h_.alloc(pool(), sizeof(double2) * kmax_ * B_);
+    part_.alloc(pool(), sizeof(double2) * (size_t)kmax_ * nchunk_ * B_);
+    alpha_.alloc(pool(), sizeof(double) * ncvmax_ * B_);
+    beta_.alloc(pool(), sizeof(double) * ncvmax_ * B_);
+    bnew_.alloc(pool(), sizeof(double) * B_);
+    for (DevBuf* b : {&jcur_, &ncv_, &live_, &frozen_}) b->alloc(pool(), sizeof(int) * B_);
+    nk_.alloc(pool(), sizeof(int) * B_);
+    nc_.alloc(pool(), sizeof(int) * B_);
     TM_HIP(hipMemcpyAsync(ncv_.p, h_ncv_.data(), sizeof(int) * B_, hipMemcpyHostToDevice, stream()));
     TM_HIP(hipMemsetAsync(alpha_.p, 0, alpha_.bytes, stream()));
     TM_HIP(hipMemsetAsync(beta_.p, 0, beta_.bytes, stream()));
@@ -840,8 +966,8 @@ class BatchLanczos {
       nc[b] = nev < ncv ? nev + 1 : nev;
     }
     if (tmp_.bytes < sizeof(double2) * (size_t)(nevmax_ + 1) * n_ * B_)
-      tmp_.alloc(sizeof(double2) * (size_t)(nevmax_ + 1) * n_ * B_);
-    coef_ = upload(coef, stream());
+      tmp_.alloc(pool(), sizeof(double2) * (size_t)(nevmax_ + 1) * n_ * B_);
+    coef_ = upload(pool(), coef, stream());
     TM_HIP(hipMemcpyAsync(nk_.p, nk.data(), sizeof(int) * B_, hipMemcpyHostToDevice, stream()));
     TM_HIP(hipMemcpyAsync(nc_.p, nc.data(), sizeof(int) * B_, hipMemcpyHostToDevice, stream()));
     const long long ldo = (long long)(nevmax_ + 1) * n_;
@@ -863,10 +989,8 @@ class BatchLanczos {
       }
     }
     // MV[:, 0:nev] = M V[:, 0:nev]
-    DevBuf ncol = upload(ncolv, stream());
-    hipLaunchKernelGGL(k_spmv_multi, dim3((3 * S_.nf + 255) / 256, B_, nevmax_), dim3(256), 0, stream(), S_.nf,
-                       M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), mk_ptrs_.as<const double2*>(),
-                       V_.as<double2>(), ldb_, n_, ncol.as<int>(), MV_.as<double2>());
+    DevBuf ncol = upload(pool(), ncolv, stream());
+    spmv_cols(mk_ptrs_, V_.as<double2>(), ldb_, ncol, nevmax_, MV_.as<double2>());
     // continuation vector f: beta_new = ||f||_M
     set_host_state(jc, lv, fz);
     spmv_m(W_.as<double2>(), n_, MW_.as<double2>(), n_, act_.as<int>());
@@ -894,25 +1018,21 @@ class BatchLanczos {
   static constexpr double kRefineTol = 5e-9;  // keeps a 2x margin under the 1e-8 bar
 
   // residual, ||K x||, x^H M x per Ritz vector (fixed-order chunk reduction)
-  void residuals(DevBuf& X, DevBuf& KX, DevBuf& MX, long long ldx, int nemax, const std::vector<int>& nc,
+  void residuals(double2* X, double2* KX, double2* MX, long long ldx, int nemax, const std::vector<int>& nc,
                  const DevBuf& kp) {
-    DevBuf dnc = upload(nc, stream());
-    hipLaunchKernelGGL(k_spmv_multi, dim3((3 * S_.nf + 255) / 256, B_, nemax), dim3(256), 0, stream(), S_.nf,
-                       M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), kp.as<const double2*>(),
-                       X.as<double2>(), ldx, n_, dnc.as<int>(), KX.as<double2>());
-    hipLaunchKernelGGL(k_spmv_multi, dim3((3 * S_.nf + 255) / 256, B_, nemax), dim3(256), 0, stream(), S_.nf,
-                       M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), mk_ptrs_.as<const double2*>(),
-                       X.as<double2>(), ldx, n_, dnc.as<int>(), MX.as<double2>());
+    DevBuf dnc = upload(pool(), nc, stream());
+    spmv_cols(kp, X, ldx, dnc, nemax, KX);
+    spmv_cols(mk_ptrs_, X, ldx, dnc, nemax, MX);
     std::vector<double> lam((size_t)B_ * nemax, 0.0);
     for (int b = 0; b < B_; ++b)
       for (int c = 0; c < nc[b]; ++c) lam[(size_t)b * nemax + c] = probs_[b]->theta_sorted_lambda[c];
-    DevBuf dl = upload(lam, stream());
-    DevBuf part(sizeof(double) * 3 * (size_t)B_ * nemax * nchunk_);
-    hipLaunchKernelGGL(k_residual_partial, dim3(nchunk_, B_, nemax), dim3(256), 0, stream(), X.as<double2>(),
-                       KX.as<double2>(), MX.as<double2>(), ldx, n_, dl.as<double>(), nemax, dnc.as<int>(),
+    DevBuf dl = upload(pool(), lam, stream());
+    DevBuf part(pool(), sizeof(double) * 3 * (size_t)B_ * nemax * nchunk_);
+    hipLaunchKernelGGL(k_residual_partial, dim3(nchunk_, B_, nemax), dim3(256), 0, stream(), X,
+                       KX, MX, ldx, n_, dl.as<double>(), nemax, dnc.as<int>(),
                        part.as<double>(), nchunk_);
     std::vector<double> hp_part(3 * (size_t)B_ * nemax * nchunk_);
-    TM_HIP(hipMemcpyAsync(hp_part.data(), part.p, part.bytes, hipMemcpyDeviceToHost, stream()));
+    TM_HIP(hipMemcpyAsync(hp_part.data(), part.p, sizeof(double) * hp_part.size(), hipMemcpyDeviceToHost, stream()));
     TM_HIP(hipStreamSynchronize(stream()));
     for (int b = 0; b < B_; ++b) {
       HostProblem* hp = probs_[b];
@@ -933,34 +1053,27 @@ class BatchLanczos {
   }
 
   // block inverse iteration + Rayleigh-Ritz for the problems with nr[b] > 0
-  void refine(DevBuf& X, DevBuf& Y, DevBuf& KY, DevBuf& MY, long long ldx, int nemax, const std::vector<int>& nr,
+  void refine(double2* X, double2* Y, double2* KY, double2* MY, long long ldx, int nemax, const std::vector<int>& nr,
               const DevBuf& kp) {
-    DevBuf dnr = upload(nr, stream());
+    DevBuf dnr = upload(pool(), nr, stream());
     // Y = M X, then solve in place column by column
-    hipLaunchKernelGGL(k_spmv_multi, dim3((3 * S_.nf + 255) / 256, B_, nemax), dim3(256), 0, stream(), S_.nf,
-                       M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), mk_ptrs_.as<const double2*>(),
-                       X.as<double2>(), ldx, n_, dnr.as<int>(), Y.as<double2>());
+    spmv_cols(mk_ptrs_, X, ldx, dnr, nemax, Y);
     BatchBufs bb = bm_.bufs(P_, B_);
-    for (int q = 0; q < nemax; ++q) {
-      hipLaunchKernelGGL(k_flag_cols, dim3((B_ + 63) / 64), dim3(64), 0, stream(), dnr.as<int>(), q, B_,
-                         act_.as<int>());
-      trisolve_batch(ctx_, M_, P_, bb, Y.as<double2>() + (long long)q * n_, ldx, act_.as<int>());
-    }
-    hipLaunchKernelGGL(k_spmv_multi, dim3((3 * S_.nf + 255) / 256, B_, nemax), dim3(256), 0, stream(), S_.nf,
-                       M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), kp.as<const double2*>(),
-                       Y.as<double2>(), ldx, n_, dnr.as<int>(), KY.as<double2>());
-    hipLaunchKernelGGL(k_spmv_multi, dim3((3 * S_.nf + 255) / 256, B_, nemax), dim3(256), 0, stream(), S_.nf,
-                       M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), mk_ptrs_.as<const double2*>(),
-                       Y.as<double2>(), ldx, n_, dnr.as<int>(), MY.as<double2>());
+    hipLaunchKernelGGL(k_flag_cols, dim3((B_ + 63) / 64), dim3(64), 0, stream(), dnr.as<int>(), 0, B_,
+                       act_.as<int>());
+    for (int q0 = 0; q0 < nemax; q0 += bm_.pcols)
+      trisolve_batch(ctx_, M_, P_, bb, Y + (long long)q0 * n_, ldx, act_.as<int>(), std::min(bm_.pcols, nemax - q0));
+    spmv_cols(kp, Y, ldx, dnr, nemax, KY);
+    spmv_cols(mk_ptrs_, Y, ldx, dnr, nemax, MY);
     const size_t gsz = (size_t)B_ * nemax * nemax * nchunk_;
-    DevBuf pk(sizeof(double2) * gsz), pm(sizeof(double2) * gsz);
-    hipLaunchKernelGGL(k_gram_partial, dim3(nchunk_, B_, nemax * nemax), dim3(256), 0, stream(), Y.as<double2>(),
-                       KY.as<double2>(), ldx, n_, dnr.as<int>(), nemax, pk.as<double2>(), nchunk_);
-    hipLaunchKernelGGL(k_gram_partial, dim3(nchunk_, B_, nemax * nemax), dim3(256), 0, stream(), Y.as<double2>(),
-                       MY.as<double2>(), ldx, n_, dnr.as<int>(), nemax, pm.as<double2>(), nchunk_);
+    DevBuf pk(pool(), sizeof(double2) * gsz), pm(pool(), sizeof(double2) * gsz);
+    hipLaunchKernelGGL(k_gram_partial, dim3(nchunk_, B_, nemax * nemax), dim3(256), 0, stream(), Y,
+                       KY, ldx, n_, dnr.as<int>(), nemax, pk.as<double2>(), nchunk_);
+    hipLaunchKernelGGL(k_gram_partial, dim3(nchunk_, B_, nemax * nemax), dim3(256), 0, stream(), Y,
+                       MY, ldx, n_, dnr.as<int>(), nemax, pm.as<double2>(), nchunk_);
     std::vector<double2> hk(gsz), hm(gsz);
-    TM_HIP(hipMemcpyAsync(hk.data(), pk.p, pk.bytes, hipMemcpyDeviceToHost, stream()));
-    TM_HIP(hipMemcpyAsync(hm.data(), pm.p, pm.bytes, hipMemcpyDeviceToHost, stream()));
+    TM_HIP(hipMemcpyAsync(hk.data(), pk.p, sizeof(double2) * hk.size(), hipMemcpyDeviceToHost, stream()));
+    TM_HIP(hipMemcpyAsync(hm.data(), pm.p, sizeof(double2) * hm.size(), hipMemcpyDeviceToHost, stream()));
     TM_HIP(hipStreamSynchronize(stream()));
     std::vector<double2> zc((size_t)B_ * nemax * nemax, make_double2(0.0, 0.0));
     for (int b = 0; b < B_; ++b) {
@@ -985,61 +1098,69 @@ class BatchLanczos {
         for (int q = 0; q < ne; ++q)
           zc[((size_t)b * nemax + k) * nemax + q] = make_double2(Z[2 * (k * ne + q)], Z[2 * (k * ne + q) + 1]);
     }
-    DevBuf dz = upload(zc, stream());
-    hipLaunchKernelGGL(k_combine_c, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), Y.as<double2>(), ldx, n_,
-                       dz.as<double2>(), nemax, (long long)nemax * nemax, dnr.as<int>(), X.as<double2>(), ldx);
+    DevBuf dz = upload(pool(), zc, stream());
+    hipLaunchKernelGGL(k_combine_c, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), Y, ldx, n_,
+                       dz.as<double2>(), nemax, (long long)nemax * nemax, dnr.as<int>(), X, ldx);
   }
 
   void extract() {
-    // Ritz vectors X = V S for every finished problem
     int nemax = 0;
-    for (auto* hp : probs_) nemax = std::max(nemax, hp->n_extract);
+    for (auto* hp : probs_)
+      if (hp->done && !hp->failed) nemax = std::max(nemax, hp->n_extract);
     if (nemax == 0) {
       finish_failed();
       return;
     }
-    const int ldc = nemax;
-    std::vector<double> coef((size_t)B_ * (ncvmax_ + 1) * ldc, 0.0);
-    std::vector<int> nk(B_, 0), nc(B_, 0);
+    std::vector<int> nc(B_, 0);
     for (int b = 0; b < B_; ++b) {
       HostProblem* hp = probs_[b];
-      if (!hp->done || hp->failed) continue;
-      const int ne = hp->n_extract, ncv = hp->ncv_final;
-      for (int k = 0; k < ncv; ++k)
-        for (int c = 0; c < ne; ++c)
-          coef[(size_t)b * (ncvmax_ + 1) * ldc + (size_t)k * ldc + c] = hp->S_extract[(size_t)k * ne + c];
-      nk[b] = ncv;
-      nc[b] = ne;
+      if (hp->done && !hp->failed) nc[b] = hp->n_extract;
     }
-    const long long ldx = (long long)nemax * n_;
-    DevBuf X(sizeof(double2) * ldx * B_), KX(sizeof(double2) * ldx * B_), MX(sizeof(double2) * ldx * B_);
-    coef_ = upload(coef, stream());
-    TM_HIP(hipMemcpyAsync(nk_.p, nk.data(), sizeof(int) * B_, hipMemcpyHostToDevice, stream()));
-    TM_HIP(hipMemcpyAsync(nc_.p, nc.data(), sizeof(int) * B_, hipMemcpyHostToDevice, stream()));
-    hipLaunchKernelGGL(k_combine, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), V_.as<double2>(), ldb_,
-                       n_, coef_.as<double>(), ldc, (long long)(ncvmax_ + 1) * ldc, nk_.as<int>(),
-                       nc_.as<int>(), X.as<double2>(), ldx);
+    DevBuf Xo, KXo, MXo, Yo;
+    double2 *X, *KX, *MX;
+    long long ldx;
+    if (block_) {
+      // Ritz vectors of the last Rayleigh-Ritz step of each problem
+      X = Xb_.as<double2>();
+      KX = KXb_.as<double2>();
+      MX = MXb_.as<double2>();
+      ldx = ldk_;
+    } else {
+      // Ritz vectors X = V S for every finished problem
+      const int ldc = nemax;
+      std::vector<double> coef((size_t)B_ * (ncvmax_ + 1) * ldc, 0.0);
+      std::vector<int> nk(B_, 0);
+      for (int b = 0; b < B_; ++b) {
+        HostProblem* hp = probs_[b];
+        if (!nc[b]) continue;
+        const int ne = hp->n_extract, ncv = hp->ncv_final;
+        for (int k = 0; k < ncv; ++k)
+          for (int c = 0; c < ne; ++c)
+            coef[(size_t)b * (ncvmax_ + 1) * ldc + (size_t)k * ldc + c] = hp->S_extract[(size_t)k * ne + c];
+        nk[b] = ncv;
+      }
+      ldx = (long long)nemax * n_;
+      Xo.alloc(pool(), sizeof(double2) * ldx * B_);
+      KXo.alloc(pool(), sizeof(double2) * ldx * B_);
+      MXo.alloc(pool(), sizeof(double2) * ldx * B_);
+      X = Xo.as<double2>();
+      KX = KXo.as<double2>();
+      MX = MXo.as<double2>();
+      coef_ = upload(pool(), coef, stream());
+      TM_HIP(hipMemcpyAsync(nk_.p, nk.data(), sizeof(int) * B_, hipMemcpyHostToDevice, stream()));
+      TM_HIP(hipMemcpyAsync(nc_.p, nc.data(), sizeof(int) * B_, hipMemcpyHostToDevice, stream()));
+      hipLaunchKernelGGL(k_combine, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), V_.as<double2>(), ldb_,
+                         n_, coef_.as<double>(), ldc, (long long)(ncvmax_ + 1) * ldc, nk_.as<int>(),
+                         nc_.as<int>(), X, ldx);
+    }
     // FP64 K(k) per problem for residuals and refinement
-    std::vector<const double2*> kptr(B_);
-    std::vector<DevBuf> kb(B_);
-    for (int b = 0; b < B_; ++b) {
-      HostProblem* hp = probs_[b];
-      if (nc[b] == 0) { kptr[b] = mk_[b]; continue; }
-      kb[b].alloc(sizeof(double2) * 9 * op_.nblk);
-      hipLaunchKernelGGL(k_build_phase_blocks, dim3((S_.nf + 127) / 128), dim3(128), 0, stream(), S_.nf,
-                         M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), M_.d_tslot.as<int>(),
-                         M_.d_pinned.as<unsigned char>(), op_.Kc.as<double>(), op_.Kp.as<double>(),
-                         op_.dKc.as<double>(), op_.dKp.as<double>(), hp->s, hp->pre, hp->pim,
-                         kb[b].as<double2>());
-      kptr[b] = kb[b].as<double2>();
-    }
-    DevBuf kp = upload(kptr, stream());
+    build_stiffness_blocks();
+    const DevBuf& kp = kptr_;
     residuals(X, KX, MX, ldx, nemax, nc, kp);
     // Refinement of Ritz vectors whose residual ||K x - lambda M x|| / ||K x|| is
     // above kRefineTol: one block inverse iteration Y = (K - sigma M)^-1 M X
     // followed by an FP64 Rayleigh-Ritz step on span(Y).  Inverse iteration damps
     // the roundoff-level high-mode content of x by lambda_i / lambda_high.
-    DevBuf Y;
     for (int pass = 0; pass < 2; ++pass) {
       std::vector<int> nr(B_, 0);
       bool any = false;
@@ -1048,19 +1169,22 @@ class BatchLanczos {
         if (nc[b] == 0) continue;
         double worst = 0;
         for (int c = 0; c < nc[b]; ++c) worst = std::max(worst, hp->residual[c]);
-        if (worst > kRefineTol || !(worst == worst)) {
+        // block results are already Rayleigh-Ritz optimal on a large basis: refine
+        // only those left above the bar
+        if (worst > (block_ ? kBlockStall : kRefineTol) || !(worst == worst)) {
           nr[b] = nc[b];
           any = true;
         }
       }
       if (!any) break;
-      if (!Y.p) Y.alloc(sizeof(double2) * ldx * B_);
-      refine(X, Y, KX, MX, ldx, nemax, nr, kp);
+      if (!Yo.p) Yo.alloc(pool(), sizeof(double2) * ldx * B_);
+      refine(X, Yo.as<double2>(), KX, MX, ldx, nemax, nr, kp);
       residuals(X, KX, MX, ldx, nemax, nc, kp);
       for (int b = 0; b < B_; ++b)
         if (nr[b]) probs_[b]->refinements += 1;
     }
     // results: degenerate duplication + expansion (src/cyclic_solver.cpp:1153-1211)
+    HostTimer hd(ctx_, "deliver");
     for (int b = 0; b < B_; ++b) {
       HostProblem* hp = probs_[b];
       tm_result* out = hp->out;
@@ -1128,19 +1252,551 @@ class BatchLanczos {
       if (out->whirl) std::copy(whirl.begin(), whirl.end(), out->whirl);
       if (out->modal_mass) std::copy(mass.begin(), mass.end(), out->modal_mass);
       if (out->mode_shapes && nm > 0) {
-        DevBuf dsrc = upload(src, stream()), dconj = upload(conjq, stream());
-        DevBuf dout(sizeof(double2) * 3 * (size_t)S_.nn * nm);
+        DevBuf dsrc = upload(pool(), src, stream()), dconj = upload(pool(), conjq, stream());
+        DevBuf dout(pool(), sizeof(double2) * 3 * (size_t)S_.nn * nm);
         const double alpha = 2.0 * M_PI / S_.N;
         const long long tot = (long long)S_.nn * nm;
         hipLaunchKernelGGL(k_expand_modes, dim3((tot + 255) / 256), dim3(256), 0, stream(), S_.nn,
                            M_.d_free_of_full.as<int>(), M_.d_node_class.as<int>(), std::cos(alpha),
-                           std::sin(alpha), S_.c1, S_.c2, S_.cax, X.as<double2>() + b * ldx, n_,
+                           std::sin(alpha), S_.c1, S_.c2, S_.cax, X + b * ldx, n_,
                            dsrc.as<int>(), dconj.as<int>(), nm, hp->pre, hp->pim, dout.as<double2>());
-        TM_HIP(hipMemcpyAsync(out->mode_shapes, dout.p, dout.bytes, hipMemcpyDeviceToHost, stream()));
+        TM_HIP(hipMemcpyAsync(out->mode_shapes, dout.p, sizeof(double2) * 3 * (size_t)S_.nn * nm,
+                              hipMemcpyDeviceToHost, stream()));
         TM_HIP(hipStreamSynchronize(stream()));
       }
     }
     TM_HIP(hipStreamSynchronize(stream()));
+  }
+
+  // -------------------------------------------------------------------------
+  // Block shift-invert Krylov with explicit FP64 Rayleigh-Ritz.
+  //
+  // The basis V (M-orthonormal, up to mmax columns per problem) grows by
+  // blocks of pb vectors: W = (K - sigma M)^-1 M Y for the block Y of V last
+  // added (one multi-RHS triangular sweep pair per block); W is
+  // M-orthogonalised against V twice (classical Gram-Schmidt) and within
+  // itself (Cholesky QR).  K V and M V are kept beside V, and the projected
+  // matrices H = V^H K V, G = V^H M V are extended by one column block per
+  // step.  Near a full basis (and after every restart) a Rayleigh-Ritz step
+  // with the FP64 K(k), M(k) checks the nev lowest Ritz pairs: accepted when
+  // ||K x - lambda M x|| / ||K x|| <= max(kBlockTol, 4 x the FP64 floor of
+  // that residual), or when it is below 1e-8 and stopped improving (the
+  // rounding of x = V s sets a level a few times the floor).  A full basis
+  // restarts from its nkeep lowest Ritz vectors and expands from the lowest
+  // unconverged ones.
+  // -------------------------------------------------------------------------
+  static constexpr double kBlockTol = 5e-9;
+  static constexpr double kBlockStall = 1e-8;
+  static constexpr double kBlockStallMax = 1e-6;
+  static constexpr int kBlockMaxRestarts = 40;
+  // TMHIP_TRACE=1 prints the convergence history of the block method to stderr
+  const bool trace_ = std::getenv("TMHIP_TRACE") != nullptr;
+  bool block_ = false;
+  int pb_ = 0, mmax_ = 0, nkeep_ = 0, nts_ = 0;
+  long long ldv_ = 0, ldw_ = 0, ldk_ = 0;
+  DevBuf Vb_, MVb_, KVb_, Wb_, MWb_, KWb_, Xb_, KXb_, MXb_, tsp_, tsc_;
+  std::vector<DevBuf> kblk_;
+  DevBuf kptr_;
+  std::vector<int> nv_, y0_;
+  std::vector<std::vector<std::complex<double>>> Hm_, Gm_;  // mmax x mmax per problem
+  std::vector<std::vector<double>> prev_res_;
+
+  void block_setup() {
+    pb_ = nevmax_ <= 6 ? 8 : 16;
+    mmax_ = pb_ == 8 ? 48 : 64;
+    nkeep_ = mmax_ / 2;
+    static_assert(kTsMaxRows >= 64, "tsmm coefficient rows");
+    ldv_ = (long long)mmax_ * n_;
+    ldw_ = (long long)pb_ * n_;
+    ldk_ = (long long)nkeep_ * n_;
+    nts_ = static_cast<int>((n_ + kTsChunk - 1) / kTsChunk);
+    for (DevBuf* b : {&Vb_, &MVb_, &KVb_}) b->alloc(pool(), sizeof(double2) * ldv_ * B_);
+    for (DevBuf* b : {&Wb_, &MWb_, &KWb_}) b->alloc(pool(), sizeof(double2) * ldw_ * B_);
+    for (DevBuf* b : {&Xb_, &KXb_, &MXb_}) b->alloc(pool(), sizeof(double2) * ldk_ * B_);
+    tsp_.alloc(pool(), sizeof(double2) * (size_t)B_ * mmax_ * 16 * nts_);
+    tsc_.alloc(pool(), sizeof(double2) * (size_t)B_ * mmax_ * 16);
+    Hm_.assign(B_, std::vector<std::complex<double>>((size_t)mmax_ * mmax_));
+    Gm_.assign(B_, std::vector<std::complex<double>>((size_t)mmax_ * mmax_));
+    prev_res_.assign(B_, std::vector<double>());
+    build_stiffness_blocks();
+  }
+
+  // FP64 K(k) blocks per problem (residuals, Rayleigh-Ritz)
+  void build_stiffness_blocks() {
+    if (kptr_.p) return;
+    std::vector<const double2*> kptr(B_);
+    kblk_.resize(B_);
+    for (int b = 0; b < B_; ++b) {
+      HostProblem* hp = probs_[b];
+      kblk_[b].alloc(pool(), sizeof(double2) * 9 * op_.nblk);
+      hipLaunchKernelGGL(k_build_phase_blocks, dim3((S_.nf + 127) / 128), dim3(128), 0, stream(), S_.nf,
+                         M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), M_.d_tslot.as<int>(),
+                         M_.d_pinned.as<unsigned char>(), op_.Kc.as<double>(), op_.Kp.as<double>(),
+                         op_.dKc.as<double>(), op_.dKp.as<double>(), hp->s, hp->pre, hp->pim,
+                         kblk_[b].as<double2>());
+      kptr[b] = kblk_[b].as<double2>();
+    }
+    TM_HIP(hipGetLastError());
+    kptr_ = upload(pool(), kptr, stream());
+  }
+
+  DevBuf ints(const std::vector<int>& v) { return upload(pool(), v, stream()); }
+
+  // C_b = A_b^H B_b (ka_b x kb, kb <= 16), returned on the host as [b][a][c] (rows of 16)
+  std::vector<double2> tsgemm(const double2* A, long long lda, const DevBuf& ka, int kamax, const double2* Bm,
+                              long long ldb, int kb, const DevBuf& act) {
+    KTimer t(ctx_, "gram", 0, 8.0 * n_ * kamax * kb * B_);
+    hipLaunchKernelGGL(k_tsgemm_partial, dim3(nts_, B_, (kamax + 15) / 16), dim3(256), 0, stream(), A, lda, Bm, ldb,
+                       n_, ka.as<int>(), kb, act.as<int>(), mmax_, tsp_.as<double2>(), nts_);
+    hipLaunchKernelGGL(k_tsgemm_reduce, dim3((kamax * 16 + 255) / 256, B_), dim3(256), 0, stream(),
+                       tsp_.as<double2>(), ka.as<int>(), kb, act.as<int>(), mmax_, nts_, tsc_.as<double2>());
+    std::vector<double2> h((size_t)B_ * mmax_ * 16);
+    TM_HIP(hipMemcpyAsync(h.data(), tsc_.p, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, stream()));
+    TM_HIP(hipStreamSynchronize(stream()));
+    return h;
+  }
+
+  // Y_b[:, yoff_b + c] = beta Y + alpha sum_a A_b[:, a] C_b[a][c], c < kc (C on the host, [b][a][16])
+  void tsmm(const double2* A, long long lda, const DevBuf& ka, const std::vector<double2>& C, int kc, double alpha,
+            double beta, double2* Y, long long ldy, const DevBuf* yoff, const DevBuf& act) {
+    DevBuf dc = upload(pool(), C, stream());
+    KTimer t(ctx_, "vec", 0, 0);
+    hipLaunchKernelGGL(k_tsmm, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), A, lda, n_, dc.as<double2>(),
+                       ka.as<int>(), kc, act.as<int>(), mmax_, alpha, beta, Y, ldy,
+                       yoff ? yoff->as<int>() : (const int*)nullptr);
+  }
+
+  // Y_b[:, c] = A_b X_b[:, c], c < ncol_b (<= cmax); same column stride ld for X and Y
+  void spmv_cols(const DevBuf& blocks, const double2* X, long long ld, const DevBuf& ncol, int cmax, double2* Y) {
+    KTimer t(ctx_, "spmv_m", 0, 0);
+    for (int c0 = 0; c0 < cmax; c0 += 16) {
+      const long long off = (long long)c0 * n_;
+      std::vector<int> nc;
+      const DevBuf* dn = &ncol;
+      DevBuf shifted;
+      if (c0 > 0) {  // columns c0.. of each problem
+        std::vector<int> h(B_);
+        TM_HIP(hipMemcpyAsync(h.data(), ncol.p, sizeof(int) * B_, hipMemcpyDeviceToHost, stream()));
+        TM_HIP(hipStreamSynchronize(stream()));
+        for (int& x : h) x = std::max(0, x - c0);
+        shifted = ints(h);
+        dn = &shifted;
+      }
+      if (cmax - c0 <= 8)
+        hipLaunchKernelGGL(k_spmv_cols<8>, dim3((3 * S_.nf + 255) / 256, B_), dim3(256), 0, stream(), S_.nf,
+                           M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), blocks.as<const double2*>(), X + off, ld,
+                           n_, dn->as<int>(), Y + off, ld);
+      else
+        hipLaunchKernelGGL(k_spmv_cols<16>, dim3((3 * S_.nf + 255) / 256, B_), dim3(256), 0, stream(), S_.nf,
+                           M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), blocks.as<const double2*>(), X + off, ld,
+                           n_, dn->as<int>(), Y + off, ld);
+    }
+  }
+
+  // M-orthogonalise W against V[:, 0:nv_b] (twice), orthonormalise it (Cholesky
+  // QR), append it to V, MV, KV at column nv_b and extend H, G.  Returns the
+  // problems whose block was rank deficient (nothing appended for them).
+  std::vector<int> append_block(const std::vector<int>& act) {
+    DevBuf dact = ints(act);
+    std::vector<int> nvc(B_, 0), pbv(B_, 0);
+    int nvmax = 0;
+    for (int b = 0; b < B_; ++b)
+      if (act[b]) {
+        nvc[b] = nv_[b];
+        pbv[b] = pb_;
+        nvmax = std::max(nvmax, nv_[b]);
+      }
+    DevBuf dnv = ints(nvc), dpb = ints(pbv);
+    if (nvmax > 0)
+      for (int pass = 0; pass < 2; ++pass) {
+        std::vector<double2> C = tsgemm(MVb_.as<double2>(), ldv_, dnv, nvmax, Wb_.as<double2>(), ldw_, pb_, dact);
+        tsmm(Vb_.as<double2>(), ldv_, dnv, C, pb_, -1.0, 1.0, Wb_.as<double2>(), ldw_, nullptr, dact);
+      }
+    spmv_cols(mk_ptrs_, Wb_.as<double2>(), ldw_, dpb, pb_, MWb_.as<double2>());
+    spmv_cols(kptr_, Wb_.as<double2>(), ldw_, dpb, pb_, KWb_.as<double2>());
+    std::vector<double2> G = tsgemm(Wb_.as<double2>(), ldw_, dpb, pb_, MWb_.as<double2>(), ldw_, pb_, dact);
+    std::vector<double2> Rinv((size_t)B_ * mmax_ * 16, make_double2(0.0, 0.0));
+    std::vector<int> bad, ok(act);
+    for (int b = 0; b < B_; ++b) {
+      if (!act[b]) continue;
+      if (!chol_inverse(&G[(size_t)b * mmax_ * 16], &Rinv[(size_t)b * mmax_ * 16])) {
+        bad.push_back(b);
+        ok[b] = 0;
+      }
+    }
+    DevBuf dok = ints(ok), doff = ints(nvc);
+    tsmm(Wb_.as<double2>(), ldw_, dpb, Rinv, pb_, 1.0, 0.0, Vb_.as<double2>(), ldv_, &doff, dok);
+    tsmm(MWb_.as<double2>(), ldw_, dpb, Rinv, pb_, 1.0, 0.0, MVb_.as<double2>(), ldv_, &doff, dok);
+    tsmm(KWb_.as<double2>(), ldw_, dpb, Rinv, pb_, 1.0, 0.0, KVb_.as<double2>(), ldv_, &doff, dok);
+    // new column block of H and G: V[:, 0:nv+pb]^H (K W, M W) Rinv
+    std::vector<int> nvn(B_, 0);
+    int nvnmax = 0;
+    for (int b = 0; b < B_; ++b)
+      if (ok[b]) {
+        nvn[b] = nv_[b] + pb_;
+        nvnmax = std::max(nvnmax, nvn[b]);
+      }
+    if (nvnmax > 0) {
+      DevBuf dnvn = ints(nvn);
+      std::vector<double2> hk = tsgemm(Vb_.as<double2>(), ldv_, dnvn, nvnmax, KWb_.as<double2>(), ldw_, pb_, dok);
+      std::vector<double2> hm = tsgemm(Vb_.as<double2>(), ldv_, dnvn, nvnmax, MWb_.as<double2>(), ldw_, pb_, dok);
+      using cd = std::complex<double>;
+      for (int b = 0; b < B_; ++b) {
+        if (!ok[b]) continue;
+        const int n0 = nv_[b], n1 = nvn[b];
+        const double2* R = &Rinv[(size_t)b * mmax_ * 16];
+        for (int a = 0; a < n1; ++a)
+          for (int c = 0; c < pb_; ++c) {
+            cd sk(0, 0), sm(0, 0);
+            for (int q = 0; q <= c; ++q) {
+              const cd r(R[q * 16 + c].x, R[q * 16 + c].y);
+              const double2 xk = hk[((size_t)b * mmax_ + a) * 16 + q], xm = hm[((size_t)b * mmax_ + a) * 16 + q];
+              sk += cd(xk.x, xk.y) * r;
+              sm += cd(xm.x, xm.y) * r;
+            }
+            Hm_[b][(size_t)a * mmax_ + n0 + c] = sk;
+            Gm_[b][(size_t)a * mmax_ + n0 + c] = sm;
+          }
+        for (int a = n0; a < n1; ++a)  // Hermitian completion (the new diagonal block is averaged)
+          for (int c = 0; c < n1; ++c) {
+            if (c >= n0 && c < a) continue;
+            const cd hk2 = std::conj(Hm_[b][(size_t)c * mmax_ + a]), gm2 = std::conj(Gm_[b][(size_t)c * mmax_ + a]);
+            if (c >= n0) {
+              const cd h1 = 0.5 * (Hm_[b][(size_t)a * mmax_ + c] + hk2);
+              const cd g1 = 0.5 * (Gm_[b][(size_t)a * mmax_ + c] + gm2);
+              Hm_[b][(size_t)a * mmax_ + c] = h1;
+              Hm_[b][(size_t)c * mmax_ + a] = std::conj(h1);
+              Gm_[b][(size_t)a * mmax_ + c] = g1;
+              Gm_[b][(size_t)c * mmax_ + a] = std::conj(g1);
+            } else {
+              Hm_[b][(size_t)a * mmax_ + c] = hk2;
+              Gm_[b][(size_t)a * mmax_ + c] = gm2;
+            }
+          }
+        nv_[b] = n1;
+      }
+    }
+    return bad;
+  }
+
+  // G (pb x pb, rows of 16) = R^H R; Rinv = R^-1 (upper).  False if G is not
+  // numerically positive definite.
+  bool chol_inverse(const double2* G, double2* Rinv) const {
+    using cd = std::complex<double>;
+    const int p = pb_;
+    std::vector<cd> R((size_t)p * p, cd(0, 0));
+    double gmax = 0;
+    for (int i = 0; i < p; ++i) gmax = std::max(gmax, std::fabs(G[i * 16 + i].x));
+    for (int j = 0; j < p; ++j) {
+      double d = G[j * 16 + j].x;
+      for (int k = 0; k < j; ++k) d -= std::norm(R[(size_t)k * p + j]);
+      if (!(d > 1e-20 * gmax) || !(gmax > 0)) return false;
+      const double rjj = std::sqrt(d);
+      R[(size_t)j * p + j] = rjj;
+      for (int i = j + 1; i < p; ++i) {
+        cd s(G[j * 16 + i].x, G[j * 16 + i].y);  // (W^H M W)[j][i]
+        for (int k = 0; k < j; ++k) s -= std::conj(R[(size_t)k * p + j]) * R[(size_t)k * p + i];
+        R[(size_t)j * p + i] = s / rjj;
+      }
+    }
+    for (int c = 0; c < p; ++c)  // R X = I, column by column
+      for (int i = c; i >= 0; --i) {
+        cd s = i == c ? cd(1, 0) : cd(0, 0);
+        for (int k = i + 1; k <= c; ++k) s -= R[(size_t)i * p + k] * cd(Rinv[k * 16 + c].x, Rinv[k * 16 + c].y);
+        const cd x = s / R[(size_t)i * p + i];
+        Rinv[i * 16 + c] = make_double2(x.real(), x.imag());
+      }
+    return true;
+  }
+
+  void random_block(const std::vector<int>& act, int salt) {
+    for (int c = 0; c < pb_; ++c) {
+      std::vector<double> v = random_vector(7919u * (uint32_t)(c + 1) + 104729u * (uint32_t)salt);
+      int first = -1;
+      for (int b = 0; b < B_; ++b) {
+        if (!act[b]) continue;
+        double2* dst = Wb_.as<double2>() + b * ldw_ + (long long)c * n_;
+        if (first < 0) {
+          TM_HIP(hipMemcpyAsync(dst, v.data(), sizeof(double2) * n_, hipMemcpyHostToDevice, stream()));
+          first = b;
+        } else {
+          TM_HIP(hipMemcpyAsync(dst, Wb_.as<double2>() + first * ldw_ + (long long)c * n_, sizeof(double2) * n_,
+                                hipMemcpyDeviceToDevice, stream()));
+        }
+      }
+      TM_HIP(hipStreamSynchronize(stream()));
+    }
+  }
+
+  void block_iterate() {
+    nv_.assign(B_, 0);
+    y0_.assign(B_, 0);
+    std::vector<int> act(B_, 0);
+    for (int b = 0; b < B_; ++b) act[b] = probs_[b]->failed ? 0 : 1;
+    random_block(act, 0);
+    append_or_retry(act, 0);
+    BatchBufs bb = bm_.bufs(P_, B_);
+    for (int it = 0;; ++it) {
+      bool any = false;
+      for (int b = 0; b < B_; ++b) any = any || act[b];
+      if (!any) break;
+      // W = (K - sigma M)^-1 M Y, Y = V[:, y0 : y0 + pb]
+      DevBuf dact = ints(act), dy0 = ints(y0_);
+      {
+        KTimer t(ctx_, "vec", 0, 0);
+        hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), MVb_.as<double2>(), ldv_,
+                           dy0.as<int>(), Wb_.as<double2>(), ldw_, (const int*)nullptr, pb_, n_, dact.as<int>());
+      }
+      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>(), ldw_, dact.as<int>(), pb_);
+      append_or_retry(act, it + 1);
+      for (int b = 0; b < B_; ++b)
+        if (act[b]) y0_[b] = nv_[b] - pb_;
+      rayleigh_ritz(act);
+      ctx_->flush_timing();
+    }
+  }
+
+  // Appends W; a problem whose block lost rank (an invariant subspace was
+  // reached) continues from fresh random directions, up to three times.
+  void append_or_retry(std::vector<int>& act, int salt) {
+    std::vector<int> bad = append_block(act);
+    for (int attempt = 1; attempt <= 3 && !bad.empty(); ++attempt) {
+      std::vector<int> sub(B_, 0);
+      for (int b : bad) sub[b] = 1;
+      random_block(sub, 1000 * salt + attempt);
+      bad = append_block(sub);
+    }
+    for (int b : bad) fail_block(b, act);
+  }
+
+  void fail_block(int b, std::vector<int>& act) {
+    probs_[b]->failed = true;
+    probs_[b]->message = "block Krylov basis lost rank";
+    act[b] = 0;
+  }
+
+  // X[:, c0:c1] = V S[:, c0:c1] for the problems in sel; K X and M X by fresh
+  // products (no drift of K V, M V across restarts)
+  void ritz_vectors(const std::vector<int>& sel, const std::vector<std::vector<double>>& Sv, int c0, int c1) {
+    std::vector<int> nvc(B_, 0);
+    for (int b = 0; b < B_; ++b)
+      if (sel[b]) nvc[b] = nv_[b];
+    DevBuf dsel = ints(sel), dnv = ints(nvc);
+    for (int q0 = c0; q0 < c1; q0 += 16) {
+      const int kc = std::min(16, c1 - q0);
+      std::vector<double2> C((size_t)B_ * mmax_ * 16, make_double2(0.0, 0.0));
+      for (int b = 0; b < B_; ++b)
+        if (sel[b])
+          for (int a = 0; a < nv_[b]; ++a)
+            for (int c = 0; c < kc && q0 + c < nv_[b]; ++c) {
+              const size_t s = (size_t)a * nv_[b] + q0 + c;
+              C[((size_t)b * mmax_ + a) * 16 + c] = make_double2(Sv[b][2 * s], Sv[b][2 * s + 1]);
+            }
+      std::vector<int> off(B_, q0);
+      DevBuf doff = ints(off);
+      tsmm(Vb_.as<double2>(), ldv_, dnv, C, kc, 1.0, 0.0, Xb_.as<double2>(), ldk_, &doff, dsel);
+    }
+    std::vector<int> ncol(B_, 0);
+    for (int b = 0; b < B_; ++b)
+      if (sel[b]) ncol[b] = c1 - c0;
+    DevBuf dn = ints(ncol);
+    const long long o = (long long)c0 * n_;
+    spmv_cols(kptr_, Xb_.as<double2>() + o, ldk_, dn, c1 - c0, KXb_.as<double2>() + o);
+    spmv_cols(mk_ptrs_, Xb_.as<double2>() + o, ldk_, dn, c1 - c0, MXb_.as<double2>() + o);
+  }
+
+  // Rayleigh-Ritz on span(V) for the problems near a full basis or restarted;
+  // convergence, restart and the next expansion block.
+  void rayleigh_ritz(std::vector<int>& act) {
+    std::vector<int> rr(B_, 0);
+    bool anyrr = false;
+    for (int b = 0; b < B_; ++b) {
+      if (!act[b]) continue;
+      HostProblem* hp = probs_[b];
+      const bool due = nv_[b] >= mmax_ - pb_ || hp->restarts > 0 || nv_[b] + pb_ > mmax_;
+      if (due && nv_[b] >= std::min(hp->nev + pb_, mmax_)) {
+        rr[b] = 1;
+        anyrr = true;
+      }
+    }
+    if (!anyrr) return;
+    std::vector<std::vector<double>> lam(B_), Sv(B_);
+    std::vector<int> ok(B_, 1);
+    {
+      HostTimer h(ctx_, "rr_eig");
+      parallel_for(B_, [&](int b) {
+        if (!rr[b]) return;
+        const int m = nv_[b];
+        std::vector<double> H(2 * (size_t)m * m), G(2 * (size_t)m * m);
+        for (int i = 0; i < m; ++i)
+          for (int j = 0; j < m; ++j) {
+            const std::complex<double> x = Hm_[b][(size_t)i * mmax_ + j], y = Gm_[b][(size_t)i * mmax_ + j];
+            H[2 * ((size_t)i * m + j)] = x.real();
+            H[2 * ((size_t)i * m + j) + 1] = x.imag();
+            G[2 * ((size_t)i * m + j)] = y.real();
+            G[2 * ((size_t)i * m + j) + 1] = y.imag();
+          }
+        ok[b] = gen_eig_chol(m, H, G, lam[b], Sv[b]) ? 1 : 0;
+      });
+    }
+    for (int b = 0; b < B_; ++b)
+      if (rr[b] && !ok[b]) {
+        fail_block(b, act);
+        rr[b] = 0;
+      }
+    // Ritz vectors of the nev lowest, their residuals
+    std::vector<int> ncheck(B_, 0);
+    int ncmax = 0;
+    for (int b = 0; b < B_; ++b)
+      if (rr[b]) {
+        ncheck[b] = std::min(probs_[b]->nev, nv_[b]);
+        ncmax = std::max(ncmax, ncheck[b]);
+      }
+    if (ncmax == 0) return;
+    ritz_vectors(rr, Sv, 0, ncmax);
+    std::vector<double> lamv((size_t)B_ * nkeep_, 0.0);
+    for (int b = 0; b < B_; ++b)
+      for (int c = 0; c < ncheck[b]; ++c) lamv[(size_t)b * nkeep_ + c] = lam[b][c];
+    DevBuf dl = upload(pool(), lamv, stream()), dnc = ints(ncheck);
+    DevBuf part(pool(), sizeof(double) * 3 * (size_t)B_ * nkeep_ * nchunk_);
+    hipLaunchKernelGGL(k_residual_partial, dim3(nchunk_, B_, ncmax), dim3(256), 0, stream(), Xb_.as<double2>(),
+                       KXb_.as<double2>(), MXb_.as<double2>(), ldk_, n_, dl.as<double>(), nkeep_, dnc.as<int>(),
+                       part.as<double>(), nchunk_);
+    std::vector<double> hpart(3 * (size_t)B_ * nkeep_ * nchunk_);
+    TM_HIP(hipMemcpyAsync(hpart.data(), part.p, sizeof(double) * hpart.size(), hipMemcpyDeviceToHost, stream()));
+    TM_HIP(hipStreamSynchronize(stream()));
+    std::vector<std::vector<double>> res(B_), den(B_);
+    std::vector<int> need_floor(B_, 0);
+    int nfmax = 0;
+    for (int b = 0; b < B_; ++b) {
+      if (!rr[b]) continue;
+      double worst = 0;
+      for (int c = 0; c < ncheck[b]; ++c) {
+        double num = 0, dd = 0;
+        for (int q = 0; q < nchunk_; ++q) {
+          const double* pq = &hpart[(((size_t)b * nkeep_ + c) * nchunk_ + q) * 3];
+          num += pq[0];
+          dd += pq[1];
+        }
+        res[b].push_back(dd > 0 ? std::sqrt(num / dd) : std::sqrt(num));
+        den[b].push_back(dd);
+        worst = std::max(worst, res[b].back());
+      }
+      // the floor only matters once the residuals are near the tolerance
+      if (worst > kBlockTol && worst < 1e-6) {
+        need_floor[b] = ncheck[b];
+        nfmax = std::max(nfmax, ncheck[b]);
+      }
+    }
+    std::vector<std::vector<double>> floor(B_);
+    if (nfmax > 0) {
+      DevBuf fpart(pool(), sizeof(double) * (size_t)B_ * nkeep_ * nchunk_);
+      for (int c0 = 0; c0 < nfmax; c0 += pb_) {
+        std::vector<int> nc2(B_, 0);
+        for (int b = 0; b < B_; ++b) nc2[b] = std::max(0, std::min(pb_, need_floor[b] - c0));
+        DevBuf dnc2 = ints(nc2);
+        const int cm = std::min(pb_, nfmax - c0);
+        // |K||x| and |M||x| into the (consumed) W, MW buffers, pb columns at a time
+        hipLaunchKernelGGL(k_spmv_abs, dim3((3 * S_.nf + 255) / 256, B_, cm), dim3(256), 0, stream(), S_.nf,
+                           M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), kptr_.as<const double2*>(),
+                           Xb_.as<double2>() + (long long)c0 * n_, ldk_, n_, dnc2.as<int>(), Wb_.as<double2>(), ldw_);
+        hipLaunchKernelGGL(k_spmv_abs, dim3((3 * S_.nf + 255) / 256, B_, cm), dim3(256), 0, stream(), S_.nf,
+                           M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), mk_ptrs_.as<const double2*>(),
+                           Xb_.as<double2>() + (long long)c0 * n_, ldk_, n_, dnc2.as<int>(), MWb_.as<double2>(),
+                           ldw_);
+        hipLaunchKernelGGL(k_floor_partial, dim3(nchunk_, B_, cm), dim3(256), 0, stream(), Wb_.as<double2>(),
+                           MWb_.as<double2>(), ldw_, n_, dl.as<double>() + c0, nkeep_, dnc2.as<int>(),
+                           fpart.as<double>() + (size_t)c0 * nchunk_, nchunk_);
+      }
+      std::vector<double> fl((size_t)B_ * nkeep_ * nchunk_);
+      TM_HIP(hipMemcpyAsync(fl.data(), fpart.p, sizeof(double) * fl.size(), hipMemcpyDeviceToHost, stream()));
+      TM_HIP(hipStreamSynchronize(stream()));
+      for (int b = 0; b < B_; ++b)
+        for (int c = 0; c < need_floor[b]; ++c) {
+          double fs = 0;
+          for (int q = 0; q < nchunk_; ++q) fs += fl[((size_t)b * nkeep_ + c) * nchunk_ + q];
+          floor[b].push_back(den[b][c] > 0 ? 2.220446049250313e-16 * std::sqrt(fs / den[b][c]) : 0.0);
+        }
+    }
+    // decisions
+    std::vector<int> restart(B_, 0);
+    for (int b = 0; b < B_; ++b) {
+      if (!rr[b]) continue;
+      HostProblem* hp = probs_[b];
+      int first_bad = -1;
+      for (int c = 0; c < ncheck[b]; ++c) {
+        const double r = res[b][c];
+        const double fl = c < (int)floor[b].size() ? floor[b][c] : 0.0;
+        bool good = r <= std::max(kBlockTol, 4.0 * fl);
+        // stagnated (no 2x gain since the previous check) at the rounding level of
+        // the basis: accept; the extraction refines what is left above 1e-8 with
+        // fresh triangular solves
+        if (!good && r <= kBlockStallMax && c < (int)prev_res_[b].size() && r > 0.5 * prev_res_[b][c])
+          good = true;
+        if (!good && first_bad < 0) first_bad = c;
+        if (trace_)
+          std::fprintf(stderr, "[tmhip] k=%d rpm=%.0f nv=%d rst=%d c=%d lam=%.10e res=%.2e floor=%.2e\n", hp->k,
+                       hp->in.rpm, nv_[b], hp->restarts, c, lam[b][c], r, fl);
+      }
+      prev_res_[b] = res[b];
+      const bool converged = first_bad < 0 && ncheck[b] >= hp->nev;
+      if (converged || hp->restarts >= std::min(hp->max_restarts, kBlockMaxRestarts)) {
+        hp->converged = converged;
+        hp->done = true;
+        hp->n_extract = ncheck[b];
+        hp->theta_sorted_lambda.assign(lam[b].begin(), lam[b].begin() + ncheck[b]);
+        act[b] = 0;
+        continue;
+      }
+      if (nv_[b] + pb_ > mmax_) {
+        restart[b] = 1;
+        hp->restarts += 1;
+        const int nx = std::min(nv_[b], nkeep_);
+        y0_[b] = std::max(0, std::min(first_bad < 0 ? 0 : first_bad, nx - pb_));
+      }
+    }
+    bool anyr = false;
+    int ncm = 0;
+    for (int b = 0; b < B_; ++b)
+      if (restart[b]) {
+        anyr = true;
+        ncm = std::max(ncm, ncheck[b]);
+      }
+    if (!anyr) return;
+    // restart: V <- the nkeep lowest Ritz vectors (columns ncheck.. formed now)
+    ritz_vectors(restart, Sv, ncm, nkeep_);
+    {
+      DevBuf dr = ints(restart);
+      KTimer t(ctx_, "vec", 0, 0);
+      hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), Xb_.as<double2>(), ldk_,
+                         (const int*)nullptr, Vb_.as<double2>(), ldv_, (const int*)nullptr, nkeep_, n_, dr.as<int>());
+      hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), KXb_.as<double2>(), ldk_,
+                         (const int*)nullptr, KVb_.as<double2>(), ldv_, (const int*)nullptr, nkeep_, n_, dr.as<int>());
+      hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), MXb_.as<double2>(), ldk_,
+                         (const int*)nullptr, MVb_.as<double2>(), ldv_, (const int*)nullptr, nkeep_, n_, dr.as<int>());
+    }
+    // projected matrices of the kept block, recomputed: X^H K X and X^H M X
+    std::vector<int> nk(B_, 0);
+    for (int b = 0; b < B_; ++b)
+      if (restart[b]) nk[b] = std::min(nv_[b], nkeep_);
+    DevBuf dr = ints(restart), dnk = ints(nk);
+    for (int c0 = 0; c0 < nkeep_; c0 += 16) {
+      const int kb = std::min(16, nkeep_ - c0);
+      std::vector<double2> hk = tsgemm(Vb_.as<double2>(), ldv_, dnk, nkeep_, KVb_.as<double2>() + (long long)c0 * n_,
+                                       ldv_, kb, dr);
+      std::vector<double2> hm = tsgemm(Vb_.as<double2>(), ldv_, dnk, nkeep_, MVb_.as<double2>() + (long long)c0 * n_,
+                                       ldv_, kb, dr);
+      for (int b = 0; b < B_; ++b)
+        if (restart[b])
+          for (int a = 0; a < nk[b]; ++a)
+            for (int c = 0; c < kb && c0 + c < nk[b]; ++c) {
+              const double2 x = hk[((size_t)b * mmax_ + a) * 16 + c], y = hm[((size_t)b * mmax_ + a) * 16 + c];
+              Hm_[b][(size_t)a * mmax_ + c0 + c] = std::complex<double>(x.x, x.y);
+              Gm_[b][(size_t)a * mmax_ + c0 + c] = std::complex<double>(y.x, y.y);
+            }
+    }
+    for (int b = 0; b < B_; ++b)
+      if (restart[b]) nv_[b] = nk[b];
   }
 
   void finish_failed() {
@@ -1152,56 +1808,92 @@ class BatchLanczos {
   }
 };
 
-int64_t problem_footprint(const Operator& op, int num_modes, int harmonic) {
+// The block method serves problems with at least this many free DOFs (its basis
+// holds up to 64 vectors); smaller ones, and the reference convergence policy,
+// run the single-vector Lanczos.
+static constexpr int kBlockMinFree = 256;
+
+static bool use_block(const Symbolic& S, const tm_problem& p) {
+  if (p.method != TM_METHOD_BLOCK || p.conv_policy == TM_CONV_REFERENCE || S.n_free_dofs < kBlockMinFree)
+    return false;
+  HostProblem hp;
+  hp.in = p;
+  fill_policy(S, hp);
+  return hp.nev <= 24;  // the kept Ritz block (32 vectors) must cover nev with room to spare
+}
+
+int64_t problem_footprint(const Operator& op, int num_modes, int harmonic, int method) {
   const Symbolic& S = op.mesh->S;
   const Plan& P = op.mesh->plan;
   HostProblem hp;
+  hp.in = tm_problem{};
   hp.in.harmonic = harmonic;
   hp.in.num_modes = num_modes;
-  hp.in.rpm = 0;
-  hp.in.min_frequency = 0.0;
-  hp.in.max_restarts = 0;
+  hp.in.method = method;
   fill_policy(S, hp);
   const int64_t n = 3LL * S.nf;
-  int64_t bytes = 16 * (P.panel_entries + P.upool_entries + P.tvec_entries + P.inv_scratch_entries);
-  bytes += 2 * 16 * (int64_t)(hp.ncv + 1) * n;       // V, MV
-  bytes += 2 * 16 * n;                               // W, MW
-  bytes += 3 * 16 * (int64_t)(hp.nev + 1) * n;       // restart / Ritz / residual vectors
-  bytes += 16 * 9 * (int64_t)op.nblk;                // K(k) for the residual check
+  const bool block = use_block(S, hp.in);
+  const int pb = hp.nev <= 6 ? 8 : 16;
+  const int pcols = std::min(16, std::max(block ? pb : 1, hp.nev));
+  int64_t bytes = 16 * (P.panel_entries + P.upool_entries + pcols * P.tvec_entries + P.inv_scratch_entries);
+  if (block) {
+    const int mmax = pb == 8 ? 48 : 64;
+    bytes += 3 * 16 * (int64_t)mmax * n;             // V, MV, KV
+    bytes += 3 * 16 * (int64_t)pb * n;               // W, MW, KW
+    bytes += 4 * 16 * (int64_t)(mmax / 2) * n;       // Ritz X, KX, MX, refinement
+  } else {
+    bytes += 2 * 16 * (int64_t)(hp.ncv + 1) * n;     // V, MV
+    bytes += 2 * 16 * n;                             // W, MW
+    bytes += 4 * 16 * (int64_t)(hp.nev + 1) * n;     // restart / Ritz / residual vectors
+  }
+  bytes += 2 * 16 * 9 * (int64_t)op.nblk;            // K(k), M(k)
   return bytes;
 }
 
 void solve_problems(Context* ctx, const tm_problem* problems, int n, tm_result* results) {
   std::vector<HostProblem> hps(n);
-  std::map<Operator*, std::vector<HostProblem*>> groups;
-  std::vector<Operator*> order;
+  // batches: problems of one operator and one method, ordered by nev so a
+  // batch shares its block size and basis length
+  std::map<std::pair<Operator*, int>, std::vector<HostProblem*>> groups;
+  std::vector<std::pair<Operator*, int>> order;
   for (int i = 0; i < n; ++i) {
     hps[i].in = problems[i];
     hps[i].out = results ? &results[i] : nullptr;
     if (!problems[i].op) throw ArgError("problem without operator");
     if (problems[i].num_modes <= 0) throw ArgError("num_modes must be positive");
+    if (problems[i].method != TM_METHOD_LANCZOS && problems[i].method != TM_METHOD_BLOCK)
+      throw ArgError("unknown method");
     Operator* op = static_cast<Operator*>(problems[i].op);
     const int max_k = op->mesh->S.N / 2;
     if (problems[i].harmonic < 0 || problems[i].harmonic > max_k)
       throw ArgError("harmonic index out of range 0..N/2");
-    if (!groups.count(op)) order.push_back(op);
-    groups[op].push_back(&hps[i]);
+    const auto key = std::make_pair(op, use_block(op->mesh->S, problems[i]) ? 1 : 0);
+    if (!groups.count(key)) order.push_back(key);
+    groups[key].push_back(&hps[i]);
   }
-  for (Operator* op : order) {
-    auto& list = groups[op];
+  for (const auto& key : order) {
+    Operator* op = key.first;
+    auto& list = groups[key];
     int64_t per = 0;
     for (auto* hp : list) {
       fill_policy(op->mesh->S, *hp);
-      per = std::max<int64_t>(per, problem_footprint(*op, hp->in.num_modes, hp->k));
+      per = std::max<int64_t>(per, problem_footprint(*op, hp->in.num_modes, hp->k, hp->in.method));
     }
+    std::stable_sort(list.begin(), list.end(), [](HostProblem* a, HostProblem* b) { return a->nev < b->nev; });
     size_t free_b = 0, total_b = 0;
     TM_HIP(hipMemGetInfo(&free_b, &total_b));
-    int64_t budget = (int64_t)(0.85 * (double)free_b);
+    // blocks cached by the context's pool are reusable by this call
+    int64_t budget = (int64_t)(0.85 * (double)(free_b + ctx->pool.cached()));
     size_t max_batch = (size_t)std::max<int64_t>(1, std::min<int64_t>(64, budget / std::max<int64_t>(per, 1)));
-    for (size_t i = 0; i < list.size(); i += max_batch) {
-      std::vector<HostProblem*> chunk(list.begin() + i, list.begin() + std::min(list.size(), i + max_batch));
-      BatchLanczos bl(ctx, *op, chunk);
+    for (size_t i = 0; i < list.size();) {
+      // a batch never mixes block sizes
+      size_t j = i;
+      const int pb_i = list[i]->nev <= 6 ? 8 : 16;
+      while (j < list.size() && j - i < max_batch && (key.second == 0 || (list[j]->nev <= 6 ? 8 : 16) == pb_i)) ++j;
+      std::vector<HostProblem*> chunk(list.begin() + i, list.begin() + j);
+      BatchLanczos bl(ctx, *op, chunk, key.second == 1);
       bl.run();
+      i = j;
     }
   }
 }

@@ -31,46 +31,122 @@ struct NumericError : std::runtime_error {
       throw ::tme::HipError(std::string(#call) + ": " + hipGetErrorString(e_));    \
   } while (0)
 
-// Device buffer owned by the engine.
+// Cache of device blocks owned by one context.  Every user of a context's
+// blocks launches on the context's single stream, so a block released by one
+// batch and handed to the next is only touched by work ordered after every
+// earlier use: no synchronisation is needed on reuse, and the steady state of
+// a long sweep makes no hipMalloc / hipFree calls.
+class DevPool {
+ public:
+  DevPool() = default;
+  DevPool(const DevPool&) = delete;
+  DevPool& operator=(const DevPool&) = delete;
+  ~DevPool() { trim(); }
+  static size_t round(size_t n) {
+    const size_t g = n <= (size_t(1) << 20) ? 4096 : (size_t(2) << 20);
+    return (n + g - 1) / g * g;
+  }
+  // a cached block of at least n bytes (n is rounded up), or nullptr
+  void* take(size_t& n) {
+    n = round(n);
+    auto it = free_.lower_bound(n);
+    if (it == free_.end() || it->first > 2 * n + (size_t(4) << 20)) return nullptr;
+    void* p = it->second;
+    n = it->first;
+    cached_ -= n;
+    free_.erase(it);
+    return p;
+  }
+  void give(void* p, size_t n) {
+    free_.emplace(n, p);
+    cached_ += n;
+  }
+  void trim() {
+    for (auto& f : free_) (void)hipFree(f.second);
+    free_.clear();
+    cached_ = 0;
+  }
+  size_t cached() const { return cached_; }
+
+ private:
+  std::multimap<size_t, void*> free_;
+  size_t cached_ = 0;
+};
+
+// Device buffer owned by the engine (from a context's pool when one is given).
+// `bytes` is the size asked for (what copies may move); `cap` the block held.
 struct DevBuf {
   void* p = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0, cap = 0;
+  DevPool* pool = nullptr;
   DevBuf() = default;
   explicit DevBuf(size_t n) { alloc(n); }
+  DevBuf(DevPool* pl, size_t n) { alloc(pl, n); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), cap(o.cap), pool(o.pool) {
+    o.p = nullptr;
+    o.bytes = o.cap = 0;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    if (this != &o) {
+      release();
+      p = o.p; bytes = o.bytes; cap = o.cap; pool = o.pool;
+      o.p = nullptr; o.bytes = o.cap = 0;
+    }
     return *this;
   }
   ~DevBuf() { release(); }
-  void alloc(size_t n) {
+  void alloc(size_t n) { alloc(nullptr, n); }
+  void alloc(DevPool* pl, size_t n) {
     release();
+    pool = pl;
     if (n == 0) return;
-    hipError_t e = hipMalloc(&p, n);
+    size_t m = n;
+    if (pool) {
+      if ((p = pool->take(m))) {  // m: the cached block's size
+        bytes = n;
+        cap = m;
+        return;
+      }
+    }
+    hipError_t e = hipMalloc(&p, m);
+    if (e != hipSuccess && pool && pool->cached()) {
+      (void)hipGetLastError();
+      pool->trim();
+      e = hipMalloc(&p, m);
+    }
     if (e != hipSuccess) {
       p = nullptr;
-      throw HipError("hipMalloc(" + std::to_string(n) + " B): " + hipGetErrorString(e));
+      throw HipError("hipMalloc(" + std::to_string(m) + " B): " + hipGetErrorString(e));
     }
     bytes = n;
+    cap = m;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (pool) pool->give(p, cap);
+      else (void)hipFree(p);
+    }
     p = nullptr;
-    bytes = 0;
+    bytes = cap = 0;
   }
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
 template <class T>
-inline DevBuf upload(const std::vector<T>& v, hipStream_t s) {
-  DevBuf b(v.size() * sizeof(T));
+inline DevBuf upload(DevPool* pool, const std::vector<T>& v, hipStream_t s) {
+  DevBuf b(pool, v.size() * sizeof(T));
   if (!v.empty()) {
     TM_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
     TM_HIP(hipStreamSynchronize(s));  // the host vector may be a temporary
   }
   return b;
+}
+
+template <class T>
+inline DevBuf upload(const std::vector<T>& v, hipStream_t s) {
+  return upload<T>(nullptr, v, s);
 }
 
 // Per-kernel event timing (bench roofline) -----------------------------------
@@ -82,6 +158,7 @@ struct KernelStat {
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
+  DevPool pool;  // workspaces of the solve batches
   std::string last_error;
   bool profiling = false;
   std::map<std::string, KernelStat> stats;
@@ -91,6 +168,15 @@ struct Context {
   std::vector<hipEvent_t> event_pool;
   hipEvent_t get_event();
   void flush_timing();  // synchronises and folds pending events into stats
+};
+
+// RAII scope that adds its host wall time to stats["host:" + name] (profiling only).
+struct HostTimer {
+  Context* ctx;
+  const char* name;
+  double t0 = 0;
+  HostTimer(Context* c, const char* n);
+  ~HostTimer();
 };
 
 // RAII scope that times everything launched on ctx->stream inside it.

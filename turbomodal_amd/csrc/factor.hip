@@ -177,27 +177,40 @@ __global__ __launch_bounds__(256) void k_panel_trsm(FactorArgs fa, const RowTask
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int jb = min(NBMAX, w - kb);
   double2* Pn = fa.L + b * fa.PE + s.poff;
+  // Li = inverse of the unit lower diagonal block (zero above the diagonal and
+  // outside jb x jb), so that x = b L11^-H is a product with static indices
   __shared__ double2 Lk[NBMAX][NBMAX + 1];
+  __shared__ double2 Li[NBMAX][NBMAX + 1];
   __shared__ double dk[NBMAX];
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < jb * jb; idx += 256) {
-    int i = idx % jb, j = idx / jb;
-    if (i > j) Lk[i][j] = Pn[(kb + i) + (long long)(kb + j) * m];
-    if (i == j) dk[i] = Pn[(kb + i) + (long long)(kb + i) * m].x;
+  for (int idx = tid; idx < NBMAX * NBMAX; idx += 256) {
+    const int i = idx % NBMAX, j = idx / NBMAX;
+    Lk[i][j] = (i > j && i < jb) ? Pn[(kb + i) + (long long)(kb + j) * m] : make_double2(0.0, 0.0);
+    Li[i][j] = make_double2(0.0, 0.0);
+    if (i == j && i < jb) dk[i] = Pn[(kb + i) + (long long)(kb + i) * m].x;
+  }
+  __syncthreads();
+  if (tid < jb) {  // column c of the inverse by forward substitution
+    const int c = tid;
+    Li[c][c] = make_double2(1.0, 0.0);
+    for (int i = c + 1; i < jb; ++i) {
+      double2 v = make_double2(0.0, 0.0);
+      for (int k = c; k < i; ++k) v = csub(v, cmul(Lk[i][k], Li[k][c]));
+      Li[i][c] = v;
+    }
   }
   __syncthreads();
   const int i = t.r0 + tid;
   if (i >= t.r1) return;
   double2 x[NBMAX];
-  for (int j = 0; j < jb; ++j) x[j] = Pn[i + (long long)(kb + j) * m];
+#pragma unroll
+  for (int k = 0; k < NBMAX; ++k) x[k] = k < jb ? Pn[i + (long long)(kb + k) * m] : make_double2(0.0, 0.0);
   for (int j = 0; j < jb; ++j) {
-    double2 v = x[j];
-    for (int k = 0; k < j; ++k) v = csub(v, cmulc(x[k], Lk[j][k]));
-    x[j] = v;
-  }
-  for (int j = 0; j < jb; ++j) {
+    double2 v = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < NBMAX; ++k) v = cadd(v, cmulc(x[k], Li[j][k]));
     const double inv = 1.0 / dk[j];
-    Pn[i + (long long)(kb + j) * m] = make_double2(x[j].x * inv, x[j].y * inv);
+    Pn[i + (long long)(kb + j) * m] = make_double2(v.x * inv, v.y * inv);
   }
 }
 
@@ -672,7 +685,10 @@ void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs&
   sa.active = active;
   sa.p = p;
   const int B = bb.B;
-  KTimer t(ctx, "trisolve", P.trisolve_bytes_c16 * B, 8.0 * p * B * P.trisolve_bytes_c16 / 20.0);
+  // algorithmic bytes: the factor once (L21 + inverted pivot triangles) and ~5 n-vectors per right-hand side
+  const double vec_bytes = 5.0 * 3 * M.S.nf * 16.0;
+  const double l_bytes = P.trisolve_bytes_c16 - vec_bytes;
+  KTimer t(ctx, "trisolve", (l_bytes + p * vec_bytes) * B, 8.0 * p * B * l_bytes / 20.0);
   if (p == 1) trisolve_levels<1>(ctx, P, sa, B);
   else if (p <= 4) trisolve_levels<4>(ctx, P, sa, B);
   else if (p <= 8) trisolve_levels<8>(ctx, P, sa, B);

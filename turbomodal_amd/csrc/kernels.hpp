@@ -143,7 +143,7 @@ OpArrays op_arrays(const Operator& op);
 const double2* harmonic_mass(Context* ctx, Operator& op, int k);
 // solves every problem; results written through each problem's tm_result
 void solve_problems(Context* ctx, const tm_problem* problems, int n, tm_result* results);
-int64_t problem_footprint(const Operator& op, int num_modes, int harmonic);
+int64_t problem_footprint(const Operator& op, int num_modes, int harmonic, int method);
 
 }  // namespace tme
 

@@ -477,29 +477,129 @@ __global__ void k_tsgemm_reduce(const double2* __restrict__ part, const int* __r
 // (C_b row-major kamax x 16); one thread per row
 __global__ __launch_bounds__(256) void k_tsmm(const double2* __restrict__ A, long long lda_b, long long n,
                                               const double2* __restrict__ C, const int* __restrict__ ka_b,
-                                              int kc, const int* __restrict__ act, int kamax, double beta,
-                                              double2* __restrict__ Y, long long ldy_b) {
+                                              int kc, const int* __restrict__ act, int kamax, double alpha,
+                                              double beta, double2* __restrict__ Y, long long ldy_b,
+                                              const int* __restrict__ yoff) {
   const int b = blockIdx.y;
   if (act && !act[b]) return;
   const int ka = ka_b[b];
-  __shared__ double2 Cs[2304];
+  __shared__ double2 Cs[kTsMaxRows * 16];
   for (int idx = threadIdx.x; idx < ka * 16; idx += 256) Cs[idx] = C[(long long)b * kamax * 16 + idx];
   __syncthreads();
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double2* Ab = A + b * lda_b;
   double2 acc[16];
+#pragma unroll
   for (int c = 0; c < 16; ++c) acc[c] = make_double2(0.0, 0.0);
   for (int a = 0; a < ka; ++a) {
     const double2 x = Ab[(long long)a * n + i];
-    for (int c = 0; c < 16; ++c)
-      if (c < kc) acc[c] = cadd(acc[c], cmul(x, Cs[a * 16 + c]));
+#pragma unroll
+    for (int c = 0; c < 16; ++c) acc[c] = cadd(acc[c], cmul(x, Cs[a * 16 + c]));
   }
-  double2* Yb = Y + b * ldy_b;
-  for (int c = 0; c < kc; ++c) {
-    double2 y = beta != 0.0 ? Yb[(long long)c * n + i] : make_double2(0.0, 0.0);
-    Yb[(long long)c * n + i] = make_double2(beta * y.x + acc[c].x, beta * y.y + acc[c].y);
+  double2* Yb = Y + b * ldy_b + (yoff ? (long long)yoff[b] * n : 0LL);
+#pragma unroll
+  for (int c = 0; c < 16; ++c)
+    if (c < kc) {
+      const double2 y = beta != 0.0 ? Yb[(long long)c * n + i] : make_double2(0.0, 0.0);
+      Yb[(long long)c * n + i] = make_double2(beta * y.x + alpha * acc[c].x, beta * y.y + alpha * acc[c].y);
+    }
+}
+
+// Y_b[:, c] = A_b X_b[:, c] for c < ncol_b <= PB: one thread per scalar row reads
+// its block row once and applies it to every column (the columns stay in L2)
+template <int PB>
+__global__ __launch_bounds__(256) void k_spmv_cols(int nf, const int* __restrict__ rowptr,
+                                                   const int* __restrict__ colidx,
+                                                   const double2* const* __restrict__ blocks,
+                                                   const double2* __restrict__ X, long long ldx, long long n,
+                                                   const int* __restrict__ ncol, double2* __restrict__ Y,
+                                                   long long ldy) {
+  const int b = blockIdx.y;
+  const int nc = ncol[b];
+  if (nc <= 0) return;
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= 3 * nf) return;
+  const int I = row / 3, cc = row % 3;
+  const double2* Mb = blocks[b];
+  const double2* x = X + b * ldx;
+  double2 acc[PB];
+#pragma unroll
+  for (int c = 0; c < PB; ++c) acc[c] = make_double2(0.0, 0.0);
+  for (int s = rowptr[I]; s < rowptr[I + 1]; ++s) {
+    const long long J3 = 3LL * colidx[s];
+    const double2* blk = Mb + (long long)s * 9 + 3 * cc;
+    const double2 a0 = blk[0], a1 = blk[1], a2 = blk[2];
+#pragma unroll
+    for (int c = 0; c < PB; ++c)
+      if (c < nc) {
+        const double2* xc = x + (long long)c * n + J3;
+        acc[c] = cadd(acc[c], cadd(cmul(a0, xc[0]), cadd(cmul(a1, xc[1]), cmul(a2, xc[2]))));
+      }
   }
+  double2* y = Y + b * ldy + row;
+#pragma unroll
+  for (int c = 0; c < PB; ++c)
+    if (c < nc) y[(long long)c * n] = acc[c];
+}
+template __global__ void k_spmv_cols<8>(int, const int*, const int*, const double2* const*, const double2*,
+                                        long long, long long, const int*, double2*, long long);
+template __global__ void k_spmv_cols<16>(int, const int*, const int*, const double2* const*, const double2*,
+                                         long long, long long, const int*, double2*, long long);
+
+// dst_b[:, d0_b + c] = src_b[:, s0_b + c] for c < ncol (problems with act_b); the
+// per-problem column offsets s0v / d0v are optional (else 0)
+__global__ void k_copy_cols(const double2* __restrict__ src, long long lds, const int* __restrict__ s0v,
+                            double2* __restrict__ dst, long long ldd, const int* __restrict__ d0v, int ncol,
+                            long long n, const int* __restrict__ act) {
+  const int b = blockIdx.y;
+  if (act && !act[b]) return;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s0 = s0v ? s0v[b] : 0, d0 = d0v ? d0v[b] : 0;
+  for (int c = 0; c < ncol; ++c) dst[b * ldd + (long long)(d0 + c) * n + i] = src[b * lds + (long long)(s0 + c) * n + i];
+}
+
+// Y_b[:, c] = |A_b| |X_b[:, c]| (entrywise moduli; real result in .x) for c < ncol_b
+__global__ void k_spmv_abs(int nf, const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                           const double2* const* __restrict__ blocks, const double2* __restrict__ X,
+                           long long ldx, long long n, const int* __restrict__ ncol, double2* __restrict__ Y,
+                           long long ldy) {
+  const int b = blockIdx.y;
+  const int c = blockIdx.z;
+  if (c >= ncol[b]) return;
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= 3 * nf) return;
+  const int I = row / 3, cc = row % 3;
+  const double2* Mb = blocks[b];
+  const double2* x = X + b * ldx + (long long)c * n;
+  double acc = 0.0;
+  for (int s = rowptr[I]; s < rowptr[I + 1]; ++s) {
+    const int J = colidx[s];
+    const double2* blk = Mb + (long long)s * 9 + 3 * cc;
+    for (int q = 0; q < 3; ++q) acc += hypot(blk[q].x, blk[q].y) * hypot(x[3LL * J + q].x, x[3LL * J + q].y);
+  }
+  Y[b * ldy + (long long)c * n + row] = make_double2(acc, 0.0);
+}
+
+// per column: sum_i (AK_i + |lam| AM_i)^2 (real parts), chunk partials
+__global__ __launch_bounds__(256) void k_floor_partial(const double2* __restrict__ AK, const double2* __restrict__ AM,
+                                                       long long ldb, long long n, const double* __restrict__ lam,
+                                                       int lam_ld, const int* __restrict__ ncol,
+                                                       double* __restrict__ part, int nchunk) {
+  const int b = blockIdx.y, c = blockIdx.z;
+  if (c >= ncol[b]) return;
+  __shared__ double sh[8];
+  const long long i0 = (long long)blockIdx.x * kDotChunk;
+  const double l = fabs(lam[(long long)b * lam_ld + c]);
+  double s = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i0 + kDotChunk && i < n; i += 256) {
+    const long long o = b * ldb + (long long)c * n + i;
+    const double t = AK[o].x + l * AM[o].x;
+    s += t * t;
+  }
+  const double ts = block_sum(s, sh);
+  if (threadIdx.x == 0) part[((long long)b * lam_ld + c) * nchunk + blockIdx.x] = ts;
 }
 
 __global__ void k_real_to_complex(const double* __restrict__ x, double2* __restrict__ y, long long n) {

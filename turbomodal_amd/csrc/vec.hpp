@@ -7,6 +7,7 @@ namespace tme {
 
 constexpr int kDotChunk = 2048;  // rows per partial-dot block (8 per thread)
 constexpr int kTsChunk = 4096;   // rows per tall-skinny A^H B block
+constexpr int kTsMaxRows = 64;   // coefficient rows of k_tsmm (basis vectors per problem)
 
 // Device-side state of a batch of Lanczos runs.
 struct LzState {
@@ -70,7 +71,22 @@ __global__ void k_tsgemm_partial(const double2* A, long long lda_b, const double
 __global__ void k_tsgemm_reduce(const double2* part, const int* ka_b, int kb, const int* act, int kamax, int nchunk,
                                 double2* C);
 __global__ void k_tsmm(const double2* A, long long lda_b, long long n, const double2* C, const int* ka_b, int kc,
-                       const int* act, int kamax, double beta, double2* Y, long long ldy_b);
+                       const int* act, int kamax, double alpha, double beta, double2* Y, long long ldy_b,
+                       const int* yoff);
+__global__ void k_copy_cols(const double2* src, long long lds, const int* s0v, double2* dst, long long ldd,
+                            const int* d0v, int ncol, long long n, const int* act);
+template <int PB>
+__global__ void k_spmv_cols(int nf, const int* rowptr, const int* colidx, const double2* const* blocks,
+                            const double2* X, long long ldx, long long n, const int* ncol, double2* Y, long long ldy);
+__global__ void k_spmv_abs(int nf, const int* rowptr, const int* colidx, const double2* const* blocks,
+                           const double2* X, long long ldx, long long n, const int* ncol, double2* Y,
+                           long long ldy);
+__global__ void k_floor_partial(const double2* AK, const double2* AM, long long ldb, long long n, const double* lam,
+                                int lam_ld, const int* ncol, double* part, int nchunk);
+// Host: H x = lambda G x, G Hermitian positive definite (Cholesky reduction + herm_eig);
+// false when G is not numerically positive definite.  X[k*m+q] (re, im) = component k of vector q.
+bool gen_eig_chol(int m, const std::vector<double>& H, const std::vector<double>& G, std::vector<double>& lam,
+                  std::vector<double>& X);
 
 // Host: Hermitian eigenproblem (Householder + QL), theta ascending, S[k*m+q] (re, im)
 void herm_eig(int m, const std::vector<double>& A, std::vector<double>& theta, std::vector<double>& S);
