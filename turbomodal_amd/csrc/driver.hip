@@ -1369,7 +1369,7 @@ class BatchLanczos {
   // Y_b[:, c] = A_b X_b[:, c], c < ncol_b (<= cmax); same column stride ld for X and Y
   void spmv_cols(const DevBuf& blocks, const double2* X, long long ld, const DevBuf& ncol, int cmax, double2* Y) {
     KTimer t(ctx_, "spmv_m", 0, 0);
-    for (int c0 = 0; c0 < cmax; c0 += 16) {
+    for (int c0 = 0; c0 < cmax; c0 += 8) {
       const long long off = (long long)c0 * n_;
       std::vector<int> nc;
       const DevBuf* dn = &ncol;
@@ -1382,14 +1382,9 @@ class BatchLanczos {
         shifted = ints(h);
         dn = &shifted;
       }
-      if (cmax - c0 <= 8)
-        hipLaunchKernelGGL(k_spmv_cols<8>, dim3((3 * S_.nf + 255) / 256, B_), dim3(256), 0, stream(), S_.nf,
-                           M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), blocks.as<const double2*>(), X + off, ld,
-                           n_, dn->as<int>(), Y + off, ld);
-      else
-        hipLaunchKernelGGL(k_spmv_cols<16>, dim3((3 * S_.nf + 255) / 256, B_), dim3(256), 0, stream(), S_.nf,
-                           M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), blocks.as<const double2*>(), X + off, ld,
-                           n_, dn->as<int>(), Y + off, ld);
+      hipLaunchKernelGGL(k_spmv_cols<8>, dim3((S_.nf + 255) / 256, B_), dim3(256), 0, stream(), S_.nf,
+                         M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), blocks.as<const double2*>(), X + off, ld, n_,
+                         dn->as<int>(), Y + off, ld);
     }
   }
 

@@ -214,41 +214,63 @@ __global__ __launch_bounds__(256) void k_panel_trsm(FactorArgs fa, const RowTask
   }
 }
 
+// F[i][j] -= sum_{k in [tk, tk+kd)} L[i][k] d_k conj(L[j][k]) on a 64x64 tile,
+// source columns staged through LDS 32 at a time (one read-modify-write of the
+// tile per super-panel instead of one per 32 columns)
 __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTask* __restrict__ tasks,
                                                     int t0, int kb) {
+  (void)kb;
   const TileTask t = tasks[t0 + blockIdx.x];
   const int b = blockIdx.y;
   const DevSn s = fa.sn[t.sn];
   const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r;
-  const int jb = min(NBMAX, w - kb);
+  const int jlim = t.jend < 0 ? m : min(t.jend, w);
   double2* Pn = fa.L + b * fa.PE + s.poff;
   double2* Un = fa.U + b * fa.UE + s.uoff;
   __shared__ double2 Li[TILE][NBMAX + 1];
   __shared__ double2 Lj[TILE][NBMAX + 1];
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < TILE * jb; idx += 256) {
-    const int ii = idx % TILE, k = idx / TILE;
-    const double d = Pn[(kb + k) + (long long)(kb + k) * m].x;
-    const int gi = t.i0 + ii, gj = t.j0 + ii;
-    double2 a = gi < m ? Pn[gi + (long long)(kb + k) * m] : make_double2(0.0, 0.0);
-    Li[ii][k] = make_double2(a.x * d, a.y * d);
-    Lj[ii][k] = gj < m ? Pn[gj + (long long)(kb + k) * m] : make_double2(0.0, 0.0);
-  }
-  __syncthreads();
   const int tx = tid % 16, ty = tid / 16;
   double2 acc[4][4];
+#pragma unroll
   for (int p = 0; p < 4; ++p)
+#pragma unroll
     for (int q = 0; q < 4; ++q) acc[p][q] = make_double2(0.0, 0.0);
-  for (int k = 0; k < jb; ++k) {
-    double2 a[4], bq[4];
-    for (int p = 0; p < 4; ++p) a[p] = Li[tx + 16 * p][k];
-    for (int q = 0; q < 4; ++q) bq[q] = Lj[ty + 16 * q][k];
-    for (int p = 0; p < 4; ++p)
-      for (int q = 0; q < 4; ++q) acc[p][q] = cadd(acc[p][q], cmulc(a[p], bq[q]));
+  for (int c0 = t.tk; c0 < t.tk + t.kd; c0 += NBMAX) {
+    const int jb = min(NBMAX, t.tk + t.kd - c0);
+    __syncthreads();
+    for (int idx = tid; idx < TILE * NBMAX; idx += 256) {
+      const int ii = idx % TILE, k = idx / TILE;
+      const int gi = t.i0 + ii, gj = t.j0 + ii;
+      if (k < jb) {
+        const double d = Pn[(c0 + k) + (long long)(c0 + k) * m].x;
+        const double2 a = gi < m ? Pn[gi + (long long)(c0 + k) * m] : make_double2(0.0, 0.0);
+        Li[ii][k] = make_double2(a.x * d, a.y * d);
+        Lj[ii][k] = gj < m ? Pn[gj + (long long)(c0 + k) * m] : make_double2(0.0, 0.0);
+      } else {
+        Li[ii][k] = make_double2(0.0, 0.0);
+        Lj[ii][k] = make_double2(0.0, 0.0);
+      }
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < NBMAX; ++k) {
+      double2 a[4], bq[4];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) a[p] = Li[tx + 16 * p][k];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bq[q] = Lj[ty + 16 * q][k];
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[p][q] = cadd(acc[p][q], cmulc(a[p], bq[q]));
+    }
   }
+#pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int gj = t.j0 + ty + 16 * q;
-    if (gj >= m) continue;
+    if (gj >= jlim) continue;
+#pragma unroll
     for (int p = 0; p < 4; ++p) {
       const int gi = t.i0 + tx + 16 * p;
       if (gi >= m || gi < gj) continue;
@@ -524,66 +546,107 @@ __global__ __launch_bounds__(256) void k_fwd_upd(SolveArgs sa, const RowTask* __
   }
 }
 
-// stage 0: z_j = y_j / d_j - sum_q conj(L[w+q, j]) x[struct(q)]      -> pivot temp
-// stage 1: x_j = z_j + sum_{i>j} conj(Linv[i, j]) z_i                -> x
-// one wave per pivot column j, lanes over the sum index
+// backward: the solved ancestor values x[struct(q)] of every front of the level,
+// gathered once into the front's update-vector slots (free after the forward
+// sweep), so the column dot products below read them contiguously
+__global__ __launch_bounds__(256) void k_bwd_gather(SolveArgs sa, const int* __restrict__ level_sn, int s0) {
+  const int b = blockIdx.y;
+  if (sa.active && !sa.active[b]) return;
+  const DevSn s = sa.sn[level_sn[s0 + blockIdx.x]];
+  const int r = 3 * s.nstr, p = sa.p;
+  const double2* xb = sa.X + b * sa.ldx;
+  for (int idx = threadIdx.x; idx < r * p; idx += 256) {
+    const int q = idx % r, c = idx / r;
+    const long long g = 3LL * sa.struct_nodes[s.str0 + q / 3] + q % 3;
+    *tcol(sa, b, c, s.toff + q) = xb[(long long)c * sa.n + g];
+  }
+}
+
+// Backward sweep of one front, 64 pivot columns [j0, j1) per block:
+//   stage 0: z_j = y_j / d_j - sum_q conj(L[w+q, j]) x[struct(q)]   -> pivot temp
+//   stage 1: x_j = z_j + sum_{i>j} conj(Linv[i, j]) z_i             -> x
+// 32-row tiles of the column-major panel are staged in LDS with coalesced
+// loads; thread (j, part) owns column j and right-hand sides part, part + NP,
+// ... (PM >= 4), or rows part, part + 4, ... of every tile with a reduction of
+// the four partial sums at the end (PM = 1).  Four waves per block.
 template <int PM>
-__global__ __launch_bounds__(256) void k_bwd_col(SolveArgs sa, const RowTask* __restrict__ tasks, int t0,
-                                                 int stage) {
+__global__ __launch_bounds__(256) void k_bwd_tile(SolveArgs sa, const RowTask* __restrict__ tasks, int t0,
+                                                  int stage) {
+  constexpr int NP = 4;
+  constexpr int CPT = PM >= 4 ? PM / NP : 1;  // right-hand sides per thread
+  constexpr bool ROWSPLIT = PM < 4;
+  constexpr int TR = 32;        // tile rows
   const int b = blockIdx.y;
   if (sa.active && !sa.active[b]) return;
   const RowTask t = tasks[t0 + blockIdx.x];
   const DevSn s = sa.sn[t.sn];
   const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r, p = sa.p;
+  const int j0 = t.r0, nj = t.r1 - t.r0;
   const double2* Pn = sa.L + b * sa.PE + s.poff;
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int j = t.r0 + wave;
-  if (j >= t.r1) return;
-  const double2* col = Pn + (long long)j * m;
-  double re[PM], im[PM];
-  for (int c = 0; c < PM; ++c) re[c] = im[c] = 0.0;
-  if (stage == 0) {
-    const double2* xb = sa.X + b * sa.ldx;
-    for (int q = lane; q < r; q += 64) {
-      const double2 l = col[w + q];
-      const long long g = 3LL * sa.struct_nodes[s.str0 + q / 3] + q % 3;
-      for (int c = 0; c < PM; ++c)
-        if (c < p) {
-          const double2 u = cconjmul(l, xb[(long long)c * sa.n + g]);
-          re[c] += u.x;
-          im[c] += u.y;
-        }
+  __shared__ double2 At[TR][64 + 1];
+  __shared__ double2 vs[TR][PM];
+  const int tid = threadIdx.x, jj = tid % 64, part = tid / 64;
+  const int j = j0 + jj;
+  double2 acc[CPT];
+#pragma unroll
+  for (int u = 0; u < CPT; ++u) acc[u] = make_double2(0.0, 0.0);
+  const int ib = stage == 0 ? w : j0, ie = stage == 0 ? m : w;
+  for (int i0 = ib; i0 < ie; i0 += TR) {
+    const int ni = min(TR, ie - i0);
+    __syncthreads();
+    for (int idx = tid; idx < TR * 64; idx += 64 * NP) {
+      const int ii = idx % TR, jc = idx / TR;
+      At[ii][jc] = (ii < ni && jc < nj) ? Pn[(i0 + ii) + (long long)(j0 + jc) * m] : make_double2(0.0, 0.0);
     }
-  } else {
-    for (int i = j + 1 + lane; i < w; i += 64) {
-      const double2 l = col[i];
-      for (int c = 0; c < PM; ++c)
-        if (c < p) {
-          const double2 u = cconjmul(l, *tcol(sa, b, c, s.toff + r + i));
-          re[c] += u.x;
-          im[c] += u.y;
-        }
+    for (int idx = tid; idx < TR * PM; idx += 64 * NP) {
+      const int ii = idx % TR, c = idx / TR;
+      const long long off = stage == 0 ? s.toff + (i0 + ii - w) : s.toff + r + (i0 + ii);
+      vs[ii][c] = (ii < ni && c < p) ? *tcol(sa, b, c, off) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    if (ROWSPLIT) {
+      for (int k = part; k < ni; k += NP) {
+        double2 a = At[k][jj];
+        if (stage == 1 && i0 + k <= j) a = make_double2(0.0, 0.0);  // strictly below the diagonal
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) acc[u] = cadd(acc[u], cconjmul(a, vs[k][u]));
+      }
+    } else {
+      for (int k = 0; k < ni; ++k) {
+        double2 a = At[k][jj];
+        if (stage == 1 && i0 + k <= j) a = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) acc[u] = cadd(acc[u], cconjmul(a, vs[k][part + NP * u]));
+      }
     }
   }
-  for (int c = 0; c < PM; ++c)
-    if (c < p) {
-      re[c] = wave_sum(re[c]);
-      im[c] = wave_sum(im[c]);
+  if (ROWSPLIT) {  // fixed-order sum of the four row partials (PM < 4: CPT = 1, columns 0..p-1 ... p = 1)
+    __syncthreads();
+    At[part][jj] = acc[0];
+    __syncthreads();
+    if (part != 0) return;
+    acc[0] = cadd(cadd(At[0][jj], At[1][jj]), cadd(At[2][jj], At[3][jj]));
+  }
+  if (jj >= nj) return;
+  const double2* x = sa.X + b * sa.ldx + 3LL * s.col0;
+  if (stage == 0) {
+    const double d = Pn[j + (long long)j * m].x;
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+      const int c = part + NP * u;
+      if (c < p) {
+        const double2 y = x[(long long)c * sa.n + j];
+        *tcol(sa, b, c, s.toff + r + j) = make_double2(y.x / d - acc[u].x, y.y / d - acc[u].y);
+      }
     }
-  if (lane == 0) {
-    if (stage == 0) {
-      const double d = col[j].x;
-      for (int c = 0; c < PM; ++c)
-        if (c < p) {
-          const double2 y = sa.X[b * sa.ldx + (long long)c * sa.n + 3LL * s.col0 + j];
-          *tcol(sa, b, c, s.toff + r + j) = make_double2(y.x / d - re[c], y.y / d - im[c]);
-        }
-    } else {
-      for (int c = 0; c < PM; ++c)
-        if (c < p) {
-          const double2 z = *tcol(sa, b, c, s.toff + r + j);
-          sa.X[b * sa.ldx + (long long)c * sa.n + 3LL * s.col0 + j] = make_double2(z.x + re[c], z.y + im[c]);
-        }
+  } else {
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+      const int c = part + NP * u;
+      if (c < p) {
+        const double2 z = *tcol(sa, b, c, s.toff + r + j);
+        sa.X[b * sa.ldx + (long long)c * sa.n + 3LL * s.col0 + j] = make_double2(z.x + acc[u].x, z.y + acc[u].y);
+      }
     }
   }
 }
@@ -661,9 +724,10 @@ static void trisolve_levels(Context* ctx, const Plan& P, const SolveArgs& sa, in
   for (int h = static_cast<int>(P.levels.size()) - 1; h >= 0; --h) {
     const LevelPlan& L = P.levels[h];
     if (!L.nbcol) continue;
-    hipLaunchKernelGGL(k_bwd_col<PM>, dim3(L.nbcol, B), dim3(256), 0, ctx->stream, sa, P.d_bcol.as<RowTask>(),
+    hipLaunchKernelGGL(k_bwd_gather, dim3(L.nsn, B), dim3(256), 0, ctx->stream, sa, lsn, L.sn0);
+    hipLaunchKernelGGL(k_bwd_tile<PM>, dim3(L.nbcol, B), dim3(256), 0, ctx->stream, sa, P.d_bcol.as<RowTask>(),
                        L.bcol0, 0);
-    hipLaunchKernelGGL(k_bwd_col<PM>, dim3(L.nbcol, B), dim3(256), 0, ctx->stream, sa, P.d_bcol.as<RowTask>(),
+    hipLaunchKernelGGL(k_bwd_tile<PM>, dim3(L.nbcol, B), dim3(256), 0, ctx->stream, sa, P.d_bcol.as<RowTask>(),
                        L.bcol0, 1);
   }
 }

@@ -39,6 +39,8 @@ struct RowTask {    // rows [r0, r1) (DOF) of snode sn
 };
 struct TileTask {   // 64x64 lower tile of the trailing front
   int sn, i0, j0;
+  int tk, kd;        // Schur update source columns [tk, tk + kd) (factor tiles)
+  int jend;          // update columns < jend only (< 0: no limit)
 };
 
 // Inversion of the unit lower L11 of every front, 64-column block J (from the last).
@@ -49,6 +51,10 @@ struct InvStep {
 };
 
 // Dense partial factorization plan step at one (level, kb).
+// One sub-panel step of the blocked LDL^H: diagonal block and panel solve of
+// columns [kb, kb + 32), then the Schur updates of its tile tasks (each names
+// its source columns and column limit: the rest of the current super-panel, or
+// the whole trailing matrix once a front's super-panel is complete).
 struct FactorStep {
   int kb;
   int diag0, ndiag;    // range in diag task list (snode ids)
@@ -63,12 +69,13 @@ struct LevelPlan {
   int sn0, nsn;            // snode ids of this level (range in level_sn)
   int finv0, nfinv;        // forward: rows of L11^-1 y (64-row chunks of the pivot block)
   int fup0, nfup;          // forward: rows [0, m) (pivot write-back + update vector)
-  int bcol0, nbcol;        // backward: pivot columns (4 per task, one wave each)
+  int bcol0, nbcol;        // backward: pivot columns (64 per task)
 };
 
 struct Plan {
   int nb = 32;             // diagonal block width
   int tile = 64;           // Schur tile
+  int kw = 128;            // super-panel width: depth of the trailing Schur update
   std::vector<LevelPlan> levels;
   std::vector<AsmTask> asm_tasks;
   std::vector<int> diag_tasks;

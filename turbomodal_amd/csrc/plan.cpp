@@ -120,7 +120,7 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
   constexpr int kAsmChunk = 16;   // node columns per assembly task
   constexpr int kRowChunk = 256;  // rows per panel-solve / forward-update task
   constexpr int kInvChunk = 64;   // rows per L11^-1 task
-  constexpr int kColChunk = 4;    // columns per backward task (one wave each)
+  constexpr int kColChunk = 64;   // pivot columns per backward task
   double tri_bytes = 0;
   for (const auto& lvl : S.levels) {
     LevelPlan L;
@@ -177,28 +177,43 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
     L.nfinv = static_cast<int>(P.finv_tasks.size()) - L.finv0;
     L.nfup = static_cast<int>(P.fup_tasks.size()) - L.fup0;
     L.nbcol = static_cast<int>(P.bcol_tasks.size()) - L.bcol0;
-    for (int kb = 0; kb < maxw; kb += NB) {
-      FactorStep st{};
-      st.kb = kb;
-      st.diag0 = static_cast<int>(P.diag_tasks.size());
-      st.trsm0 = static_cast<int>(P.row_tasks.size());
-      st.tile0 = static_cast<int>(P.tile_tasks.size());
-      for (int id : lvl) {
-        const Snode& s = S.sn[id];
-        const int w = s.w(), m = s.m();
-        if (w <= kb) continue;
-        const int jb = std::min(NB, w - kb);
-        P.diag_tasks.push_back(id);
-        const int t0 = kb + jb;
-        for (int r0 = t0; r0 < m; r0 += kRowChunk)
-          P.row_tasks.push_back({id, r0, std::min(m, r0 + kRowChunk)});
-        for (int i0 = t0; i0 < m; i0 += TL)
-          for (int j0 = t0; j0 <= i0; j0 += TL) P.tile_tasks.push_back({id, i0, j0});
+    // super-panels of KW columns made of NB-wide sub-panels: each sub-panel
+    // updates the rest of its super-panel (narrow update, depth NB); the last
+    // one updates the trailing matrix with the whole super-panel (depth <= KW)
+    const int KW = P.kw;
+    for (int ks = 0; ks < maxw; ks += KW) {
+      for (int kb = ks; kb < std::min(maxw, ks + KW); kb += NB) {
+        const bool last = kb + NB >= ks + KW;
+        FactorStep st{};
+        st.kb = kb;
+        st.diag0 = static_cast<int>(P.diag_tasks.size());
+        st.trsm0 = static_cast<int>(P.row_tasks.size());
+        st.tile0 = static_cast<int>(P.tile_tasks.size());
+        for (int id : lvl) {
+          const Snode& s = S.sn[id];
+          const int w = s.w(), m = s.m();
+          if (w <= kb) continue;
+          const int jb = std::min(NB, w - kb);
+          P.diag_tasks.push_back(id);
+          const int t0 = kb + jb;
+          for (int r0 = t0; r0 < m; r0 += kRowChunk)
+            P.row_tasks.push_back({id, r0, std::min(m, r0 + kRowChunk)});
+          const int se = std::min(w, ks + KW);           // end of this front's super-panel
+          if (last || kb + NB >= w) {
+            // super-panel complete: trailing matrix, depth se - ks
+            for (int j0 = se; j0 < m; j0 += TL)
+              for (int i0 = j0; i0 < m; i0 += TL) P.tile_tasks.push_back({id, i0, j0, ks, se - ks, -1});
+          } else {
+            // the rest of the super-panel, columns [t0, se), depth jb
+            for (int j0 = t0; j0 < se; j0 += TL)
+              for (int i0 = j0; i0 < m; i0 += TL) P.tile_tasks.push_back({id, i0, j0, kb, jb, se});
+          }
+        }
+        st.ndiag = static_cast<int>(P.diag_tasks.size()) - st.diag0;
+        st.ntrsm = static_cast<int>(P.row_tasks.size()) - st.trsm0;
+        st.ntile = static_cast<int>(P.tile_tasks.size()) - st.tile0;
+        L.steps.push_back(st);
       }
-      st.ndiag = static_cast<int>(P.diag_tasks.size()) - st.diag0;
-      st.ntrsm = static_cast<int>(P.row_tasks.size()) - st.trsm0;
-      st.ntile = static_cast<int>(P.tile_tasks.size()) - st.tile0;
-      L.steps.push_back(st);
     }
     const int maxblk = (maxw + 63) / 64;
     for (int J = maxblk - 1; J >= 0; --J) {
@@ -210,7 +225,7 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
         const int nblk = (S.sn[id].w() + 63) / 64;
         if (nblk <= J) continue;
         P.inv_diag.push_back(id);
-        for (int I = J + 1; I < nblk; ++I) P.inv_tiles.push_back({id, I, J});
+        for (int I = J + 1; I < nblk; ++I) P.inv_tiles.push_back({id, I, J, 0, 0, -1});
       }
       iv.ndiag = static_cast<int>(P.inv_diag.size()) - iv.diag0;
       iv.ntile = static_cast<int>(P.inv_tiles.size()) - iv.tile0;

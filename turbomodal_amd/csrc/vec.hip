@@ -506,8 +506,9 @@ __global__ __launch_bounds__(256) void k_tsmm(const double2* __restrict__ A, lon
     }
 }
 
-// Y_b[:, c] = A_b X_b[:, c] for c < ncol_b <= PB: one thread per scalar row reads
-// its block row once and applies it to every column (the columns stay in L2)
+// Y_b[:, c] = A_b X_b[:, c] for c < ncol_b <= PB: one thread per node row reads
+// its 3x3 blocks once and applies them to every column; each gathered x triple
+// serves the node's three rows
 template <int PB>
 __global__ __launch_bounds__(256) void k_spmv_cols(int nf, const int* __restrict__ rowptr,
                                                    const int* __restrict__ colidx,
@@ -518,34 +519,41 @@ __global__ __launch_bounds__(256) void k_spmv_cols(int nf, const int* __restrict
   const int b = blockIdx.y;
   const int nc = ncol[b];
   if (nc <= 0) return;
-  const int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= 3 * nf) return;
-  const int I = row / 3, cc = row % 3;
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nf) return;
   const double2* Mb = blocks[b];
   const double2* x = X + b * ldx;
-  double2 acc[PB];
+  double2 acc[3][PB];
 #pragma unroll
-  for (int c = 0; c < PB; ++c) acc[c] = make_double2(0.0, 0.0);
+  for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+    for (int c = 0; c < PB; ++c) acc[rr][c] = make_double2(0.0, 0.0);
   for (int s = rowptr[I]; s < rowptr[I + 1]; ++s) {
     const long long J3 = 3LL * colidx[s];
-    const double2* blk = Mb + (long long)s * 9 + 3 * cc;
-    const double2 a0 = blk[0], a1 = blk[1], a2 = blk[2];
+    const double2* blk = Mb + (long long)s * 9;
+    double2 a[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) a[e] = blk[e];
 #pragma unroll
     for (int c = 0; c < PB; ++c)
       if (c < nc) {
         const double2* xc = x + (long long)c * n + J3;
-        acc[c] = cadd(acc[c], cadd(cmul(a0, xc[0]), cadd(cmul(a1, xc[1]), cmul(a2, xc[2]))));
+        const double2 x0 = xc[0], x1 = xc[1], x2 = xc[2];
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr)
+          acc[rr][c] = cadd(acc[rr][c],
+                            cadd(cmul(a[3 * rr], x0), cadd(cmul(a[3 * rr + 1], x1), cmul(a[3 * rr + 2], x2))));
       }
   }
-  double2* y = Y + b * ldy + row;
+  double2* y = Y + b * ldy + 3LL * I;
 #pragma unroll
   for (int c = 0; c < PB; ++c)
-    if (c < nc) y[(long long)c * n] = acc[c];
+    if (c < nc)
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr) y[(long long)c * n + rr] = acc[rr][c];
 }
 template __global__ void k_spmv_cols<8>(int, const int*, const int*, const double2* const*, const double2*,
                                         long long, long long, const int*, double2*, long long);
-template __global__ void k_spmv_cols<16>(int, const int*, const int*, const double2* const*, const double2*,
-                                         long long, long long, const int*, double2*, long long);
 
 // dst_b[:, d0_b + c] = src_b[:, s0_b + c] for c < ncol (problems with act_b); the
 // per-problem column offsets s0v / d0v are optional (else 0)
