@@ -71,12 +71,116 @@ def cpu_baseline(sm, rpm, num_modes, ks=(0, 1)):
     return rate, t_ops, t_solve, sols, m, ops
 
 
+def run_c5(args):
+    """Serving: seeded Poisson stream of uneven requests through the continuous-batching
+    server; meshes, symbolic analyses and operators resident before the timed region."""
+    import threading
+    import turbomodal_amd as tm
+    from turbomodal_amd import capi, meshgen
+    core = tm._core
+    rng = np.random.default_rng(args.seed)
+    names = ["flat_disk_c1", "bladed_c2"] + (["bladed_c3"] if args.c5_c3 else [])
+    mats = {"steel": (200e9, 0.3, 7800.0), "titanium": (114e9, 0.34, 4430.0)}
+    ctx = capi.Context()
+    ops, keep = [], []
+    t0 = time.perf_counter()
+    for name in names:
+        for N in (24, 36, 72):
+            sm = meshgen.config_mesh(name, N)
+            cm = meshgen.to_core_mesh(sm, core)
+            solver = core.CyclicSymmetrySolver(cm, core.Material(*mats["steel"]))
+            em = capi.engine_from_core(ctx, cm, solver.build_constrained_dofs())
+            keep += [cm, solver, em]
+            for mname, mat in mats.items():
+                ops.append((name, N, mname, capi.EngineOperator(em, *mat, True)))
+    print(f"[bench] c5: {len(ops)} resident operators in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    def draw(n):
+        out = []
+        for _ in range(n):
+            name, N, mname, op = ops[int(rng.integers(len(ops)))]
+            out.append(dict(op=op, rpm=float(rng.uniform(0, 15000)), harmonic=int(rng.integers(0, N // 2 + 1)),
+                            num_modes=int(rng.choice([5, 10, 20])), mesh=name))
+        return out
+
+    # saturation: the whole stream at once
+    reqs = draw(args.c5_requests)
+    with capi.Server(ctx, max_slots=64) as srv:
+        ts = time.perf_counter()
+        tickets = [srv.submit(p, want_shapes=True) for p in reqs]
+        for t in tickets:
+            srv.wait(t)
+        sat = len(reqs) / (time.perf_counter() - ts)
+    print(f"[bench] c5 saturation {sat:.2f} solves/s", file=sys.stderr, flush=True)
+    # Poisson arrivals at 80 % of saturation
+    rate = 0.8 * sat
+    reqs = draw(args.c5_requests)
+    gaps = rng.exponential(1.0 / rate, len(reqs))
+    arrival, done_at = {}, {}
+    lock = threading.Lock()
+    with capi.Server(ctx, max_slots=64) as srv:
+        from concurrent.futures import ThreadPoolExecutor
+        pool = ThreadPoolExecutor(max_workers=16)
+
+        def waiter(t):
+            out = srv.wait(t)
+            with lock:
+                done_at[t] = time.perf_counter()
+            return out
+
+        futs = []
+        tstart = time.perf_counter()
+        nxt = tstart
+        for p, g in zip(reqs, gaps):
+            nxt += g
+            dt = nxt - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            t = srv.submit(p)
+            arrival[t] = time.perf_counter()
+            futs.append(pool.submit(waiter, t))
+        results = [f.result() for f in futs]
+        tend = time.perf_counter()
+        pool.shutdown()
+    lat = np.array([done_at[t] - arrival[t] for t in arrival])
+    served = len(reqs) / (tend - tstart)
+    ok = sum(1 for r in results if r.status == 0)
+    # the plain FP64 path: one request at a time, single-vector Lanczos, on a sample
+    sample = reqs[: args.c5_plain]
+    tp = time.perf_counter()
+    for p in sample:
+        capi.solve_batch(ctx, [dict(p, method=capi.TM_METHOD_LANCZOS)])
+    plain = len(sample) / (time.perf_counter() - tp)
+    out = {
+        "metric": "ND modal solves/sec + p99 latency (serving)", "value": round(served, 3), "unit": "ND solves/s",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(1e3 * (tend - tstart), 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128 (complex f64)",
+        "data": "synthetic (seeded Kuhn TET10 lattice meshes; Poisson arrivals)",
+        "config": {"workload": "c5 serving stream", "requests": len(reqs), "meshes": names, "sectors": [24, 36, 72],
+                   "materials": list(mats), "arrival_rate_per_s": round(rate, 3), "seed": args.seed},
+        "saturation_solves_per_s": round(sat, 3),
+        "p50_latency_ms": round(1e3 * float(np.percentile(lat, 50)), 1),
+        "p99_latency_ms": round(1e3 * float(np.percentile(lat, 99)), 1),
+        "solved": ok,
+        "plain_fp64_one_at_a_time": {"value": round(plain, 3), "unit": "ND solves/s", "sample": len(sample),
+                                     "eigensolver": "lanczos"},
+        "speedup_vs_plain": round(sat / plain, 2),
+    }
+    for o in ops:
+        o[3].close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c4", choices=["c4", "c2", "c1"])
+    ap.add_argument("--workload", default="c4", choices=["c4", "c2", "c1", "c5"])
+    ap.add_argument("--c5-requests", type=int, default=240)
+    ap.add_argument("--c5-plain", type=int, default=24)
+    ap.add_argument("--c5-c3", action="store_true", help="include the ~330k-DOF C3 generator in the stream")
+    ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--precision", default="fp64", choices=["fp64"])
     ap.add_argument("--method", default="block", choices=["block", "lanczos"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -86,6 +190,9 @@ def main():
               "(the (rpm, harmonic) problems would shard across GPUs with no collectives)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c5":
+        run_c5(args)
+        return
 
     tm, sm, mesh, mat, solver = build_solver(args.workload, args.precision)
     solver.engine_method = args.method

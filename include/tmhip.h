@@ -175,6 +175,29 @@ typedef struct tm_result {
 int tm_solve_batch(tm_context* ctx, const tm_problem* problems, int32_t n,
                    tm_result* results);
 
+/* Continuous-batching server (serving requests of uneven size as they arrive;
+ * no counterpart in the reference, whose solve_at_rpm is a blocking call per
+ * rpm, src/cyclic_solver.cpp:725-1357).  A scheduler thread owns the context:
+ * submitted problems are admitted into slot groups (one per operator and block
+ * size, capacity from tm_problem_footprint against hbm_fraction of the free
+ * device memory) as slots and memory free up; their factorizations run
+ * batched, masked to the new slots, and their block-Krylov steps share the
+ * launches of the problems already in flight.  Results land in the caller's
+ * tm_result buffers, which must stay valid until tm_server_wait reports done.
+ * A problem's result does not depend on what it shares the device with.
+ * While a server exists, tm_solve_batch on its context fails. */
+typedef struct tm_server tm_server;
+int tm_server_create(tm_context* ctx, int32_t max_slots /* per group, <= 64 */,
+                     double hbm_fraction /* of free device memory, e.g. 0.85 */,
+                     tm_server** out);
+int tm_server_submit(tm_server* s, const tm_problem* problem, tm_result* result,
+                     uint64_t* ticket);
+/* Blocks until the problem is done (*done = 1, the ticket is released) or
+ * timeout_s passes (*done = 0; timeout_s < 0 waits without limit). */
+int tm_server_wait(tm_server* s, uint64_t ticket, double timeout_s, int32_t* done);
+/* Finishes every submitted problem, then stops the scheduler. */
+void tm_server_destroy(tm_server* s);
+
 /* Device bytes one problem holds while in flight (factor + Krylov basis);
  * the serving loop admits requests against this. */
 int64_t tm_problem_footprint(const tm_operator* op, int32_t num_modes,

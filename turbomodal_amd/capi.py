@@ -33,7 +33,8 @@ EXPORTS = (
     "tm_kernel_stats", "tm_reset_kernel_stats", "tm_mesh_create", "tm_mesh_destroy",
     "tm_mesh_get_info", "tm_operator_create", "tm_operator_destroy", "tm_solve_batch",
     "tm_problem_footprint", "tm_device_free_bytes", "tm_operator_export",
-    "tm_operator_static_u",
+    "tm_operator_static_u", "tm_server_create", "tm_server_submit", "tm_server_wait",
+    "tm_server_destroy",
 )
 
 
@@ -101,6 +102,11 @@ def lib():
     L.tm_device_free_bytes.restype = i64
     L.tm_operator_export.argtypes = [vp, dbl, i32, i32, P(i64), P(i32), P(i32), P(dbl)]
     L.tm_operator_static_u.argtypes = [vp, P(dbl)]
+    L.tm_server_create.argtypes = [vp, i32, dbl, P(vp)]
+    L.tm_server_submit.argtypes = [vp, P(Problem), P(Result), P(C.c_uint64)]
+    L.tm_server_wait.argtypes = [vp, C.c_uint64, dbl, P(i32)]
+    L.tm_server_destroy.argtypes = [vp]
+    L.tm_server_destroy.restype = None
     _lib = L
     return L
 
@@ -245,57 +251,98 @@ class SolveOutput:
     modal_mass: np.ndarray = None
 
 
+def _prepare(p, want_shapes, prob, res):
+    """Fill a Problem and a Result (caller-owned buffers) from a dict."""
+    op = p["op"]
+    prob.op = op.h
+    prob.rpm = float(p.get("rpm", 0.0))
+    prob.harmonic = int(p["harmonic"])
+    prob.num_modes = int(p["num_modes"])
+    prob.precision = int(p.get("precision", TM_PREC_FP64))
+    prob.conv_policy = int(p.get("conv_policy", TM_CONV_TIGHT))
+    prob.shift = float(p.get("shift", 1.0))
+    prob.tol_rel = float(p.get("tol_rel", 1e-12))
+    prob.max_restarts = int(p.get("max_restarts", 1000))
+    prob.method = int(p.get("method", TM_METHOD_BLOCK))
+    prob.min_frequency = float(p.get("min_frequency", 0.0))
+    nm = prob.num_modes
+    nn = op.mesh.n_nodes
+    f = np.zeros(nm)
+    w = np.zeros(nm, np.int32)
+    ev = np.zeros(nm)
+    rs = np.zeros(nm)
+    ms = np.zeros(2 * 3 * nn * nm) if want_shapes else None
+    mm = np.zeros(nm)
+    res.modal_mass = _ptr(mm, C.c_double)
+    res.frequencies = _ptr(f, C.c_double)
+    res.whirl = _ptr(w, C.c_int32)
+    res.eigenvalues = _ptr(ev, C.c_double)
+    res.residuals = _ptr(rs, C.c_double)
+    res.mode_shapes = _ptr(ms, C.c_double) if want_shapes else None
+    return (f, w, ev, rs, ms, mm)
+
+
+def _output(p, r, keep):
+    f, w, ev, rs, ms, mm = keep
+    nm = r.n_modes
+    nn = p["op"].mesh.n_nodes
+    shapes = None
+    if ms is not None:
+        z = ms[: 2 * 3 * nn * nm]
+        shapes = (z[0::2] + 1j * z[1::2]).reshape(3 * nn, nm)
+    return SolveOutput(int(p["harmonic"]), float(p.get("rpm", 0.0)), f[:nm].copy(), shapes,
+                       w[:nm].copy(), ev[: r.n_unique].copy(), rs[: r.n_unique].copy(),
+                       bool(r.converged), int(r.restarts), int(r.status), float(r.seconds),
+                       mm[:nm].copy())
+
+
 def solve_batch(ctx: Context, problems, want_shapes=True):
     """problems: list of dicts {op, rpm, harmonic, num_modes, [precision, conv_policy, shift,
     tol_rel, max_restarts, method, min_frequency]} -> list[SolveOutput]."""
     n = len(problems)
     parr = (Problem * n)()
     rarr = (Result * n)()
-    keep = []
-    for i, p in enumerate(problems):
-        op = p["op"]
-        parr[i].op = op.h
-        parr[i].rpm = float(p.get("rpm", 0.0))
-        parr[i].harmonic = int(p["harmonic"])
-        parr[i].num_modes = int(p["num_modes"])
-        parr[i].precision = int(p.get("precision", TM_PREC_FP64))
-        parr[i].conv_policy = int(p.get("conv_policy", TM_CONV_TIGHT))
-        parr[i].shift = float(p.get("shift", 1.0))
-        parr[i].tol_rel = float(p.get("tol_rel", 1e-12))
-        parr[i].max_restarts = int(p.get("max_restarts", 1000))
-        parr[i].method = int(p.get("method", TM_METHOD_BLOCK))
-        parr[i].min_frequency = float(p.get("min_frequency", 0.0))
-        nm = parr[i].num_modes
-        nn = op.mesh.n_nodes
-        f = np.zeros(nm)
-        w = np.zeros(nm, np.int32)
-        ev = np.zeros(nm)
-        rs = np.zeros(nm)
-        ms = np.zeros(2 * 3 * nn * nm) if want_shapes else None
-        mm = np.zeros(nm)
-        keep.append((f, w, ev, rs, ms, mm))
-        rarr[i].modal_mass = _ptr(mm, C.c_double)
-        rarr[i].frequencies = _ptr(f, C.c_double)
-        rarr[i].whirl = _ptr(w, C.c_int32)
-        rarr[i].eigenvalues = _ptr(ev, C.c_double)
-        rarr[i].residuals = _ptr(rs, C.c_double)
-        rarr[i].mode_shapes = _ptr(ms, C.c_double) if want_shapes else None
+    keep = [_prepare(p, want_shapes, parr[i], rarr[i]) for i, p in enumerate(problems)]
     ctx.check(ctx.L.tm_solve_batch(ctx.h, parr, n, rarr))
-    out = []
-    for i, p in enumerate(problems):
-        f, w, ev, rs, ms, mm = keep[i]
-        r = rarr[i]
-        nm = r.n_modes
-        nn = p["op"].mesh.n_nodes
-        shapes = None
-        if ms is not None:
-            z = ms[: 2 * 3 * nn * nm]
-            shapes = (z[0::2] + 1j * z[1::2]).reshape(3 * nn, nm)
-        out.append(SolveOutput(int(p["harmonic"]), float(p.get("rpm", 0.0)), f[:nm].copy(), shapes,
-                               w[:nm].copy(), ev[: r.n_unique].copy(), rs[: r.n_unique].copy(),
-                               bool(r.converged), int(r.restarts), int(r.status), float(r.seconds),
-                               mm[:nm].copy()))
-    return out
+    return [_output(p, rarr[i], keep[i]) for i, p in enumerate(problems)]
+
+
+class Server:
+    """Continuous-batching server on a context (tm_server_*): submit() returns a
+    ticket at once, wait() blocks until that problem's result is in."""
+
+    def __init__(self, ctx: Context, max_slots: int = 64, hbm_fraction: float = 0.85):
+        h = C.c_void_p()
+        ctx.check(ctx.L.tm_server_create(ctx.h, int(max_slots), float(hbm_fraction), C.byref(h)))
+        self.ctx, self.h, self._live = ctx, h, {}
+
+    def submit(self, p, want_shapes=True) -> int:
+        prob, res = Problem(), Result()
+        keep = _prepare(p, want_shapes, prob, res)
+        t = C.c_uint64()
+        self.ctx.check(self.ctx.L.tm_server_submit(self.h, C.byref(prob), C.byref(res), C.byref(t)))
+        self._live[t.value] = (p, res, keep)
+        return t.value
+
+    def wait(self, ticket: int, timeout=None):
+        done = C.c_int32()
+        self.ctx.check(self.ctx.L.tm_server_wait(self.h, ticket, -1.0 if timeout is None else float(timeout),
+                                                 C.byref(done)))
+        if not done.value:
+            return None
+        p, res, keep = self._live.pop(ticket)
+        return _output(p, res, keep)
+
+    def close(self):
+        if self.h:
+            self.ctx.L.tm_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def mesh_arrays_from_model(nodes, elements, left, right, pairs, constrained_dofs):
@@ -310,3 +357,12 @@ def mesh_arrays_from_model(nodes, elements, left, right, pairs, constrained_dofs
     cm = np.zeros(3 * nn, np.uint8)
     cm[list(constrained_dofs)] = 1
     return cls, partner, cm
+
+
+def engine_from_core(ctx: Context, core_mesh, constrained_dofs):
+    """EngineMesh of a ``_core.Mesh`` whose cyclic boundaries are matched (the same
+    inputs the drop-in solver hands the engine)."""
+    cls, partner, cm = mesh_arrays_from_model(core_mesh.nodes, core_mesh.elements, core_mesh.left_boundary,
+                                              core_mesh.right_boundary, core_mesh.matched_pairs, constrained_dofs)
+    return EngineMesh(ctx, core_mesh.nodes, core_mesh.elements, cls, partner, cm, core_mesh.num_sectors,
+                      int(getattr(core_mesh, "rotation_axis", 2)))

@@ -236,6 +236,7 @@ int tm_solve_batch(tm_context* ctx, const tm_problem* problems, int32_t n, tm_re
   if (!ctx || (n > 0 && !problems)) return TM_ERR_ARG;
   if (n <= 0) return TM_OK;
   return guarded(ctx, [&] {
+    if (ctx->server) throw std::runtime_error("the context is owned by a server; submit to it instead");
     TM_HIP(hipSetDevice(ctx->device));
     for (int i = 0; i < n; ++i)
       if (results) {
@@ -248,6 +249,37 @@ int tm_solve_batch(tm_context* ctx, const tm_problem* problems, int32_t n, tm_re
       }
     solve_problems(ctx, problems, n, results);
   });
+}
+
+int tm_server_create(tm_context* ctx, int32_t max_slots, double hbm_fraction, tm_server** out) {
+  if (!ctx || !out) return TM_ERR_ARG;
+  *out = nullptr;
+  return guarded(ctx, [&] {
+    if (ctx->server) throw std::runtime_error("the context already has a server");
+    TM_HIP(hipSetDevice(ctx->device));
+    ctx->flush_timing();
+    auto* s = new tm_server{ctx, server_create(ctx, max_slots, hbm_fraction)};
+    ctx->server = s->s;
+    *out = s;
+  });
+}
+
+int tm_server_submit(tm_server* s, const tm_problem* p, tm_result* r, uint64_t* ticket) {
+  if (!s || !p || !ticket) return TM_ERR_ARG;
+  return guarded(s->ctx, [&] { *ticket = server_submit(s->s, *p, r); });
+}
+
+int tm_server_wait(tm_server* s, uint64_t ticket, double timeout_s, int32_t* done) {
+  if (!s || !done) return TM_ERR_ARG;
+  *done = 0;
+  return guarded(s->ctx, [&] { *done = server_wait(s->s, ticket, timeout_s) ? 1 : 0; });
+}
+
+void tm_server_destroy(tm_server* s) {
+  if (!s) return;
+  server_destroy(s->s);
+  s->ctx->server = nullptr;
+  delete s;
 }
 
 int64_t tm_problem_footprint(const tm_operator* op, int32_t num_modes, int32_t precision,

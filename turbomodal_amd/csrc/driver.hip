@@ -18,6 +18,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
 #include <random>
 #include <thread>
 
@@ -505,6 +508,9 @@ void create_operator(Context* ctx, Mesh& M, Operator& op) {
 }
 
 const double2* harmonic_mass(Context* ctx, Operator& op, int k) {
+  // operators are shared by the server's slot groups, each on its own thread and stream
+  static std::mutex mu;
+  std::lock_guard<std::mutex> g(mu);
   auto it = op.Mk_cache.find(k);
   if (it != op.Mk_cache.end()) return it->second.as<double2>();
   Mesh& M = *op.mesh;
@@ -517,6 +523,7 @@ const double2* harmonic_mass(Context* ctx, Operator& op, int k) {
                      (const double*)nullptr, (const double*)nullptr, 0.0, std::cos(k * alpha),
                      std::sin(k * alpha), b.as<double2>());
   TM_HIP(hipGetLastError());
+  TM_HIP(hipStreamSynchronize(ctx->stream));  // complete before other streams read it
   const double2* p = b.as<double2>();
   op.Mk_cache.emplace(k, std::move(b));
   return p;
@@ -540,6 +547,8 @@ struct HostProblem {
   std::vector<double> residual, xmx;        // per extracted pair
   int refinements = 0;
   int ncv_final = 0;
+  int steps = 0;                             // block steps taken (seeds its restarts)
+  std::chrono::steady_clock::time_point t_admit;
   std::string message;
 };
 
@@ -564,7 +573,7 @@ static void fill_policy(const Symbolic& S, HostProblem& hp) {
 
 class BatchLanczos {
  public:
-  BatchLanczos(Context* ctx, Operator& op, std::vector<HostProblem*>& probs, bool block)
+  BatchLanczos(Context* ctx, Operator& op, const std::vector<HostProblem*>& probs, bool block)
       : ctx_(ctx), op_(op), M_(*op.mesh), S_(M_.S), P_(M_.plan), probs_(probs), block_(block) {}
 
   void run() {
@@ -579,16 +588,12 @@ class BatchLanczos {
     }
     {
       HostTimer h(ctx_, "iterate");
-      if (block_) {
-        block_iterate();
-      } else {
-        start_vectors();
-        iterate();
-      }
+      start_vectors();
+      iterate();
     }
     {
       HostTimer h(ctx_, "extract");
-      extract();
+      extract(std::vector<int>(B_, 1));
     }
     double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (auto* hp : probs_)
@@ -601,7 +606,7 @@ class BatchLanczos {
   Mesh& M_;
   const Symbolic& S_;
   const Plan& P_;
-  std::vector<HostProblem*>& probs_;
+  std::vector<HostProblem*> probs_;  // per slot (nullptr: free)
   int B_ = 0, ncvmax_ = 0, kmax_ = 0, nevmax_ = 0, nchunk_ = 0;
   long long n_ = 0, ldb_ = 0;
   BatchMem bm_;
@@ -1036,6 +1041,7 @@ class BatchLanczos {
     TM_HIP(hipStreamSynchronize(stream()));
     for (int b = 0; b < B_; ++b) {
       HostProblem* hp = probs_[b];
+      if (!nc[b]) continue;
       hp->residual.assign(nc[b], 0.0);
       hp->xmx.assign(nc[b], 0.0);
       for (int c = 0; c < nc[b]; ++c) {
@@ -1103,18 +1109,21 @@ class BatchLanczos {
                        dz.as<double2>(), nemax, (long long)nemax * nemax, dnr.as<int>(), X, ldx);
   }
 
-  void extract() {
+  // Ritz vectors, residuals, refinement and results of the selected problems
+  void extract(const std::vector<int>& sel) {
     int nemax = 0;
-    for (auto* hp : probs_)
-      if (hp->done && !hp->failed) nemax = std::max(nemax, hp->n_extract);
-    if (nemax == 0) {
-      finish_failed();
-      return;
-    }
     std::vector<int> nc(B_, 0);
     for (int b = 0; b < B_; ++b) {
       HostProblem* hp = probs_[b];
-      if (hp->done && !hp->failed) nc[b] = hp->n_extract;
+      if (!sel[b] || !hp) continue;
+      if (hp->done && !hp->failed) {
+        nc[b] = hp->n_extract;
+        nemax = std::max(nemax, hp->n_extract);
+      }
+    }
+    if (nemax == 0) {
+      finish_failed(sel);
+      return;
     }
     DevBuf Xo, KXo, MXo, Yo;
     double2 *X, *KX, *MX;
@@ -1153,8 +1162,16 @@ class BatchLanczos {
                          n_, coef_.as<double>(), ldc, (long long)(ncvmax_ + 1) * ldc, nk_.as<int>(),
                          nc_.as<int>(), X, ldx);
     }
-    // FP64 K(k) per problem for residuals and refinement
-    build_stiffness_blocks();
+    // FP64 K(k) per problem for residuals and refinement (the block method built
+    // them on admission)
+    if (!block_) {
+      kblk_.resize(B_);
+      kptr_host_.assign(B_, nullptr);
+      std::vector<int> all;
+      for (int b = 0; b < B_; ++b)
+        if (nc[b]) all.push_back(b);
+      build_stiffness_blocks(all);
+    }
     const DevBuf& kp = kptr_;
     residuals(X, KX, MX, ldx, nemax, nc, kp);
     // Refinement of Ritz vectors whose residual ||K x - lambda M x|| / ||K x|| is
@@ -1187,6 +1204,7 @@ class BatchLanczos {
     HostTimer hd(ctx_, "deliver");
     for (int b = 0; b < B_; ++b) {
       HostProblem* hp = probs_[b];
+      if (!sel[b] || !hp) continue;
       tm_result* out = hp->out;
       if (!out) continue;
       if (hp->failed || !hp->done || hp->n_extract == 0) {
@@ -1296,7 +1314,9 @@ class BatchLanczos {
   long long ldv_ = 0, ldw_ = 0, ldk_ = 0;
   DevBuf Vb_, MVb_, KVb_, Wb_, MWb_, KWb_, Xb_, KXb_, MXb_, tsp_, tsc_;
   std::vector<DevBuf> kblk_;
+  std::vector<const double2*> kptr_host_;
   DevBuf kptr_;
+  std::vector<int> iter_;  // slot is iterating
   std::vector<int> nv_, y0_;
   std::vector<std::vector<std::complex<double>>> Hm_, Gm_;  // mmax x mmax per problem
   std::vector<std::vector<double>> prev_res_;
@@ -1318,26 +1338,25 @@ class BatchLanczos {
     Hm_.assign(B_, std::vector<std::complex<double>>((size_t)mmax_ * mmax_));
     Gm_.assign(B_, std::vector<std::complex<double>>((size_t)mmax_ * mmax_));
     prev_res_.assign(B_, std::vector<double>());
-    build_stiffness_blocks();
+    kblk_.resize(B_);
+    kptr_host_.assign(B_, nullptr);
   }
 
-  // FP64 K(k) blocks per problem (residuals, Rayleigh-Ritz)
-  void build_stiffness_blocks() {
-    if (kptr_.p) return;
-    std::vector<const double2*> kptr(B_);
-    kblk_.resize(B_);
-    for (int b = 0; b < B_; ++b) {
+  // FP64 K(k) blocks of the problems in the listed slots (residuals, Rayleigh-Ritz)
+  void build_stiffness_blocks(const std::vector<int>& slots) {
+    for (int b : slots) {
       HostProblem* hp = probs_[b];
-      kblk_[b].alloc(pool(), sizeof(double2) * 9 * op_.nblk);
+      const Operator& sop = *static_cast<const Operator*>(hp->in.op);
+      if (!kblk_[b].p) kblk_[b].alloc(pool(), sizeof(double2) * 9 * op_.nblk);
       hipLaunchKernelGGL(k_build_phase_blocks, dim3((S_.nf + 127) / 128), dim3(128), 0, stream(), S_.nf,
                          M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), M_.d_tslot.as<int>(),
-                         M_.d_pinned.as<unsigned char>(), op_.Kc.as<double>(), op_.Kp.as<double>(),
-                         op_.dKc.as<double>(), op_.dKp.as<double>(), hp->s, hp->pre, hp->pim,
+                         M_.d_pinned.as<unsigned char>(), sop.Kc.as<double>(), sop.Kp.as<double>(),
+                         sop.dKc.as<double>(), sop.dKp.as<double>(), hp->s, hp->pre, hp->pim,
                          kblk_[b].as<double2>());
-      kptr[b] = kblk_[b].as<double2>();
+      kptr_host_[b] = kblk_[b].as<double2>();
     }
     TM_HIP(hipGetLastError());
-    kptr_ = upload(pool(), kptr, stream());
+    kptr_ = upload(pool(), kptr_host_, stream());
   }
 
   DevBuf ints(const std::vector<int>& v) { return upload(pool(), v, stream()); }
@@ -1504,61 +1523,206 @@ class BatchLanczos {
     return true;
   }
 
-  void random_block(const std::vector<int>& act, int salt) {
-    for (int c = 0; c < pb_; ++c) {
-      std::vector<double> v = random_vector(7919u * (uint32_t)(c + 1) + 104729u * (uint32_t)salt);
-      int first = -1;
-      for (int b = 0; b < B_; ++b) {
-        if (!act[b]) continue;
-        double2* dst = Wb_.as<double2>() + b * ldw_ + (long long)c * n_;
-        if (first < 0) {
-          TM_HIP(hipMemcpyAsync(dst, v.data(), sizeof(double2) * n_, hipMemcpyHostToDevice, stream()));
-          first = b;
-        } else {
-          TM_HIP(hipMemcpyAsync(dst, Wb_.as<double2>() + first * ldw_ + (long long)c * n_, sizeof(double2) * n_,
+  // W[:, 0:pb] of the selected slots <- seeded normal vectors (zero on pinned
+  // DOFs); the seed depends on the column and the slot's salt only, so a
+  // problem's start and restart vectors do not depend on its batch
+  void random_block(const std::vector<int>& sel, const std::vector<int>& salt) {
+    std::map<int, std::vector<int>> by_salt;
+    for (int b = 0; b < B_; ++b)
+      if (sel[b]) by_salt[salt[b]].push_back(b);
+    for (const auto& g : by_salt)
+      for (int c = 0; c < pb_; ++c) {
+        std::vector<double> v = random_vector(7919u * (uint32_t)(c + 1) + 104729u * (uint32_t)g.first);
+        const int first = g.second[0];
+        TM_HIP(hipMemcpyAsync(Wb_.as<double2>() + first * ldw_ + (long long)c * n_, v.data(), sizeof(double2) * n_,
+                              hipMemcpyHostToDevice, stream()));
+        for (size_t i = 1; i < g.second.size(); ++i)
+          TM_HIP(hipMemcpyAsync(Wb_.as<double2>() + g.second[i] * ldw_ + (long long)c * n_,
+                                Wb_.as<double2>() + first * ldw_ + (long long)c * n_, sizeof(double2) * n_,
                                 hipMemcpyDeviceToDevice, stream()));
-        }
+        TM_HIP(hipStreamSynchronize(stream()));
       }
+  }
+
+  // ---- slots: continuous batching ----------------------------------------
+ public:
+  // Allocates the workspaces of `capacity` slots for problems of at most nevmax
+  // wanted pairs (one block size).
+  void slot_setup(int capacity, int nevmax) {
+    HostTimer h(ctx_, "setup");
+    B_ = capacity;
+    probs_.assign(B_, nullptr);
+    n_ = 3LL * S_.nf;
+    nevmax_ = nevmax;
+    ncvmax_ = 1;
+    nchunk_ = static_cast<int>((n_ + kDotChunk - 1) / kDotChunk);
+    const int pcols = std::min(16, std::max(nevmax_ <= 6 ? 8 : 16, nevmax_));
+    bm_.alloc(pool(), P_, B_, pcols);
+    act_.alloc(pool(), sizeof(int) * B_);
+    h_params_.assign(B_, ProbParam{1.0, 0.0, 0.0, 1.0});
+    h_ops_.assign(B_, op_arrays(op_));
+    mk_.assign(B_, nullptr);
+    h_ncv_.assign(B_, 0);
+    iter_.assign(B_, 0);
+    nv_.assign(B_, 0);
+    y0_.assign(B_, 0);
+    block_setup();
+  }
+
+  int free_slots() const {
+    int n = 0;
+    for (auto* hp : probs_) n += hp ? 0 : 1;
+    return n;
+  }
+  bool busy() const { return free_slots() < B_; }
+  int capacity() const { return B_; }
+
+  // Puts problems into free slots, factors them (only them), and starts their
+  // block Krylov bases; they then step with the problems already in flight.
+  void admit(std::vector<HostProblem*> adm) {
+    std::vector<int> slots, mask(B_, 0);
+    const auto now = std::chrono::steady_clock::now();
+    for (int b = 0; b < B_ && !adm.empty(); ++b)
+      if (!probs_[b]) {
+        HostProblem* hp = adm.front();
+        adm.erase(adm.begin());
+        probs_[b] = hp;
+        hp->t_admit = now;
+        h_params_[b] = {hp->pre, hp->pim, hp->s, hp->in.shift};
+        Operator& sop = *static_cast<Operator*>(hp->in.op);
+        if (sop.mesh != &M_) throw ArgError("admit: problem of another mesh");
+        mk_[b] = harmonic_mass(ctx_, sop, hp->k);
+        h_ops_[b] = op_arrays(sop);
+        nv_[b] = 0;
+        y0_[b] = 0;
+        prev_res_[b].clear();
+        slots.push_back(b);
+        mask[b] = 1;
+      }
+    if (!adm.empty()) throw ArgError("admit: more problems than free slots");
+    if (slots.empty()) return;
+    TM_HIP(hipMemcpyAsync(bm_.params.p, h_params_.data(), sizeof(ProbParam) * B_, hipMemcpyHostToDevice, stream()));
+    std::vector<int> zero(B_, 0);
+    TM_HIP(hipMemcpyAsync(bm_.status.p, zero.data(), sizeof(int) * B_, hipMemcpyHostToDevice, stream()));
+    mk_ptrs_ = upload(pool(), mk_, stream());
+    build_stiffness_blocks(slots);
+    DevBuf dmask = ints(mask);
+    {
+      HostTimer h(ctx_, "factor");
+      BatchBufs bb = bm_.bufs(P_, B_);
+      DevBuf dops = upload(pool(), h_ops_, stream());
+      factor_batch(ctx_, M_, P_, op_arrays(op_), bb, dmask.as<int>(), static_cast<int>(slots.size()),
+                   dops.as<OpArrays>());
+      std::vector<int> status(B_);
+      TM_HIP(hipMemcpyAsync(status.data(), bm_.status.p, sizeof(int) * B_, hipMemcpyDeviceToHost, stream()));
       TM_HIP(hipStreamSynchronize(stream()));
+      for (int b : slots)
+        if (status[b]) {
+          probs_[b]->failed = true;
+          probs_[b]->message = "zero pivot in the shift-invert factorization";
+          mask[b] = 0;
+        }
+    }
+    HostTimer h(ctx_, "iterate");
+    random_block(mask, std::vector<int>(B_, 0));
+    append_or_retry(mask);
+    for (int b : slots)
+      if (mask[b]) {
+        iter_[b] = 1;
+        y0_[b] = nv_[b] - pb_;
+      }
+  }
+
+  // One block step of every iterating slot: expansion, orthogonalisation,
+  // Rayleigh-Ritz and convergence (finished slots stop iterating).
+  void block_step() {
+    HostTimer h(ctx_, "iterate");
+    bool any = false;
+    for (int b = 0; b < B_; ++b) any = any || iter_[b];
+    if (!any) return;
+    BatchBufs bb = bm_.bufs(P_, B_);
+    // W = (K - sigma M)^-1 M Y, Y = V[:, y0 : y0 + pb]
+    DevBuf dact = ints(iter_), dy0 = ints(y0_);
+    {
+      KTimer t(ctx_, "vec", 0, 0);
+      hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), MVb_.as<double2>(), ldv_,
+                         dy0.as<int>(), Wb_.as<double2>(), ldw_, (const int*)nullptr, pb_, n_, dact.as<int>());
+    }
+    trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>(), ldw_, dact.as<int>(), pb_);
+    for (int b = 0; b < B_; ++b)
+      if (iter_[b]) probs_[b]->steps += 1;
+    append_or_retry(iter_);
+    for (int b = 0; b < B_; ++b)
+      if (iter_[b]) y0_[b] = nv_[b] - pb_;
+    rayleigh_ritz(iter_);
+    ctx_->flush_timing();
+  }
+
+  // Extracts the results of every finished slot and frees it; returns the
+  // problems that finished.
+  std::vector<HostProblem*> retire() {
+    std::vector<int> sel(B_, 0);
+    std::vector<HostProblem*> out;
+    for (int b = 0; b < B_; ++b)
+      if (probs_[b] && !iter_[b]) {
+        sel[b] = 1;
+        out.push_back(probs_[b]);
+      }
+    if (out.empty()) return out;
+    {
+      HostTimer h(ctx_, "extract");
+      extract(sel);
+    }
+    const auto now = std::chrono::steady_clock::now();
+    for (int b = 0; b < B_; ++b)
+      if (sel[b]) {
+        HostProblem* hp = probs_[b];
+        if (hp->out) hp->out->seconds = std::chrono::duration<double>(now - hp->t_admit).count();
+        probs_[b] = nullptr;
+      }
+    return out;
+  }
+
+  // Streams the queue through the slots: refills once `refill` slots (or all
+  // the rest of the queue) can be admitted together.  refill = capacity drains
+  // the slots between admissions (best when the whole queue is known: the
+  // factorizations stay fully batched); smaller values let problems that
+  // converge early hand their slots on.
+  void run_slots(std::vector<HostProblem*> queue, int capacity, int refill) {
+    int nevmax = 1;
+    for (auto* hp : queue) nevmax = std::max(nevmax, hp->nev);
+    slot_setup(capacity, nevmax);
+    refill = std::max(1, std::min(refill, capacity));
+    size_t next = 0;
+    while (true) {
+      const int nfree = free_slots();
+      const size_t left = queue.size() - next;
+      if (left > 0 && nfree > 0 && (nfree >= refill || (size_t)nfree >= left)) {
+        const size_t take = std::min(left, (size_t)nfree);
+        admit(std::vector<HostProblem*>(queue.begin() + next, queue.begin() + next + take));
+        next += take;
+      }
+      if (!busy()) break;
+      block_step();
+      retire();
     }
   }
 
-  void block_iterate() {
-    nv_.assign(B_, 0);
-    y0_.assign(B_, 0);
-    std::vector<int> act(B_, 0);
-    for (int b = 0; b < B_; ++b) act[b] = probs_[b]->failed ? 0 : 1;
-    random_block(act, 0);
-    append_or_retry(act, 0);
-    BatchBufs bb = bm_.bufs(P_, B_);
-    for (int it = 0;; ++it) {
-      bool any = false;
-      for (int b = 0; b < B_; ++b) any = any || act[b];
-      if (!any) break;
-      // W = (K - sigma M)^-1 M Y, Y = V[:, y0 : y0 + pb]
-      DevBuf dact = ints(act), dy0 = ints(y0_);
-      {
-        KTimer t(ctx_, "vec", 0, 0);
-        hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), MVb_.as<double2>(), ldv_,
-                           dy0.as<int>(), Wb_.as<double2>(), ldw_, (const int*)nullptr, pb_, n_, dact.as<int>());
-      }
-      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>(), ldw_, dact.as<int>(), pb_);
-      append_or_retry(act, it + 1);
-      for (int b = 0; b < B_; ++b)
-        if (act[b]) y0_[b] = nv_[b] - pb_;
-      rayleigh_ritz(act);
-      ctx_->flush_timing();
-    }
-  }
+ private:
+  std::vector<ProbParam> h_params_;
+  std::vector<OpArrays> h_ops_;  // per slot: the problem's operator (material) on the shared mesh
 
   // Appends W; a problem whose block lost rank (an invariant subspace was
   // reached) continues from fresh random directions, up to three times.
-  void append_or_retry(std::vector<int>& act, int salt) {
+  void append_or_retry(std::vector<int>& act) {
     std::vector<int> bad = append_block(act);
     for (int attempt = 1; attempt <= 3 && !bad.empty(); ++attempt) {
-      std::vector<int> sub(B_, 0);
-      for (int b : bad) sub[b] = 1;
-      random_block(sub, 1000 * salt + attempt);
+      std::vector<int> sub(B_, 0), salt(B_, 0);
+      for (int b : bad) {
+        sub[b] = 1;
+        salt[b] = 1000 * probs_[b]->steps + attempt;
+      }
+      random_block(sub, salt);
       bad = append_block(sub);
     }
     for (int b : bad) fail_block(b, act);
@@ -1568,6 +1732,7 @@ class BatchLanczos {
     probs_[b]->failed = true;
     probs_[b]->message = "block Krylov basis lost rank";
     act[b] = 0;
+    if (b < (int)iter_.size()) iter_[b] = 0;
   }
 
   // X[:, c0:c1] = V S[:, c0:c1] for the problems in sel; K X and M X by fresh
@@ -1794,11 +1959,11 @@ class BatchLanczos {
       if (restart[b]) nv_[b] = nk[b];
   }
 
-  void finish_failed() {
-    for (auto* hp : probs_)
-      if (hp->out) {
-        hp->out->status = TM_ERR_NUMERIC;
-        hp->out->n_modes = 0;
+  void finish_failed(const std::vector<int>& sel) {
+    for (int b = 0; b < B_; ++b)
+      if (sel[b] && probs_[b] && probs_[b]->out) {
+        probs_[b]->out->status = TM_ERR_NUMERIC;
+        probs_[b]->out->n_modes = 0;
       }
   }
 };
@@ -1849,8 +2014,11 @@ void solve_problems(Context* ctx, const tm_problem* problems, int n, tm_result* 
   std::vector<HostProblem> hps(n);
   // batches: problems of one operator and one method, ordered by nev so a
   // batch shares its block size and basis length
-  std::map<std::pair<Operator*, int>, std::vector<HostProblem*>> groups;
-  std::vector<std::pair<Operator*, int>> order;
+  // key: (mesh, 1) for the block method, whose slots may mix materials of one mesh;
+  // (operator, 0) for Lanczos batches
+  std::map<std::pair<const void*, int>, std::vector<HostProblem*>> groups;
+  std::map<std::pair<const void*, int>, Operator*> rep;
+  std::vector<std::pair<const void*, int>> order;
   for (int i = 0; i < n; ++i) {
     hps[i].in = problems[i];
     hps[i].out = results ? &results[i] : nullptr;
@@ -1862,12 +2030,16 @@ void solve_problems(Context* ctx, const tm_problem* problems, int n, tm_result* 
     const int max_k = op->mesh->S.N / 2;
     if (problems[i].harmonic < 0 || problems[i].harmonic > max_k)
       throw ArgError("harmonic index out of range 0..N/2");
-    const auto key = std::make_pair(op, use_block(op->mesh->S, problems[i]) ? 1 : 0);
-    if (!groups.count(key)) order.push_back(key);
+    const bool blk = use_block(op->mesh->S, problems[i]);
+    const auto key = std::make_pair(blk ? (const void*)op->mesh : (const void*)op, blk ? 1 : 0);
+    if (!groups.count(key)) {
+      order.push_back(key);
+      rep[key] = op;
+    }
     groups[key].push_back(&hps[i]);
   }
   for (const auto& key : order) {
-    Operator* op = key.first;
+    Operator* op = rep[key];
     auto& list = groups[key];
     int64_t per = 0;
     for (auto* hp : list) {
@@ -1881,16 +2053,303 @@ void solve_problems(Context* ctx, const tm_problem* problems, int n, tm_result* 
     int64_t budget = (int64_t)(0.85 * (double)(free_b + ctx->pool.cached()));
     size_t max_batch = (size_t)std::max<int64_t>(1, std::min<int64_t>(64, budget / std::max<int64_t>(per, 1)));
     for (size_t i = 0; i < list.size();) {
-      // a batch never mixes block sizes
+      // one block size per slot group (Lanczos: lockstep batches)
       size_t j = i;
       const int pb_i = list[i]->nev <= 6 ? 8 : 16;
-      while (j < list.size() && j - i < max_batch && (key.second == 0 || (list[j]->nev <= 6 ? 8 : 16) == pb_i)) ++j;
-      std::vector<HostProblem*> chunk(list.begin() + i, list.begin() + j);
-      BatchLanczos bl(ctx, *op, chunk, key.second == 1);
-      bl.run();
+      if (key.second == 1) {
+        while (j < list.size() && (list[j]->nev <= 6 ? 8 : 16) == pb_i) ++j;
+        std::vector<HostProblem*> queue(list.begin() + i, list.begin() + j);
+        BatchLanczos bl(ctx, *op, {}, true);
+        const int cap = static_cast<int>(std::min(max_batch, queue.size()));
+        bl.run_slots(queue, cap, cap);
+      } else {
+        while (j < list.size() && j - i < max_batch) ++j;
+        std::vector<HostProblem*> chunk(list.begin() + i, list.begin() + j);
+        BatchLanczos bl(ctx, *op, chunk, false);
+        bl.run();
+      }
       i = j;
     }
   }
 }
+
+}  // namespace tme
+
+// ---------------------------------------------------------------------------
+// Continuous-batching server.  A dispatcher thread takes submitted problems and
+// hands them to slot groups, one per (mesh, block size): a group may mix
+// materials and rpm of one mesh.  Each group runs on its own host thread and
+// HIP stream with its own device pool, so the groups' kernels overlap on the
+// device; its slot capacity comes from the problems' device footprint within
+// a server-wide HBM budget (a group holds at most a quarter of it).  A group
+// admits queued problems as slots free up (factorizations batched, masked to
+// the new slots), steps all its slots together, retires finished problems into
+// the caller's result buffers, and returns its memory when it runs dry.
+// Problems too small for the block method run as Lanczos batches on the
+// dispatcher thread.
+// ---------------------------------------------------------------------------
+namespace tme {
+
+// a context of its own (stream, pool) on the parent's device, for one slot group
+struct SubContext {
+  Context c;
+  explicit SubContext(int device) {
+    c.device = device;
+    TM_HIP(hipSetDevice(device));
+    TM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  }
+  ~SubContext() {
+    (void)hipStreamSynchronize(c.stream);
+    c.pool.trim();
+    for (auto& p : c.pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto ev : c.event_pool) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(c.stream);
+  }
+};
+
+struct Server {
+  struct Req {
+    HostProblem hp;
+    bool done = false;
+  };
+  struct Group {
+    Server* srv = nullptr;
+    const Mesh* mesh = nullptr;
+    Operator* rep = nullptr;  // any operator of the mesh (plan, pattern)
+    int pbclass = 8;
+    std::unique_ptr<SubContext> sub;
+    std::unique_ptr<BatchLanczos> bl;
+    std::deque<HostProblem*> queue;  // guarded by srv->mu
+    double reserved = 0;             // guarded by srv->mu
+    std::thread th;
+    bool quit = false;               // guarded by srv->mu
+  };
+
+  Context* ctx;
+  int max_slots;
+  double frac, budget = 0, reserved = 0;
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv_in, cv_done, cv_group;
+  bool stop = false;
+  std::string fatal;
+  std::deque<uint64_t> incoming;
+  std::map<uint64_t, std::unique_ptr<Req>> reqs;
+  std::map<HostProblem*, uint64_t> ticket_of;
+  uint64_t next_ticket = 1;
+  std::map<std::pair<const Mesh*, int>, std::unique_ptr<Group>> groups;
+
+  Server(Context* c, int slots, double f) : ctx(c), max_slots(std::max(1, std::min(64, slots))), frac(f) {
+    if (!(frac > 0.0 && frac <= 0.95)) frac = 0.85;
+    size_t free_b = 0, total_b = 0;
+    TM_HIP(hipMemGetInfo(&free_b, &total_b));
+    budget = frac * (double)(free_b + ctx->pool.cached());
+    th = std::thread([this] { dispatch(); });
+  }
+  ~Server() {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      stop = true;
+    }
+    cv_in.notify_all();
+    if (th.joinable()) th.join();
+  }
+
+  uint64_t submit(const tm_problem& p, tm_result* out) {
+    if (!p.op) throw ArgError("problem without operator");
+    if (p.num_modes <= 0) throw ArgError("num_modes must be positive");
+    if (p.method != TM_METHOD_LANCZOS && p.method != TM_METHOD_BLOCK) throw ArgError("unknown method");
+    Operator* op = static_cast<Operator*>(p.op);
+    if (p.harmonic < 0 || p.harmonic > op->mesh->S.N / 2) throw ArgError("harmonic index out of range 0..N/2");
+    auto r = std::make_unique<Req>();
+    r->hp.in = p;
+    r->hp.out = out;
+    if (out) {
+      out->n_modes = 0;
+      out->n_unique = 0;
+      out->converged = 0;
+      out->restarts = 0;
+      out->status = TM_ERR_RUNTIME;
+      out->seconds = 0;
+    }
+    std::lock_guard<std::mutex> g(mu);
+    if (!fatal.empty()) throw HipError("server stopped: " + fatal);
+    const uint64_t t = next_ticket++;
+    ticket_of[&r->hp] = t;
+    reqs[t] = std::move(r);
+    incoming.push_back(t);
+    cv_in.notify_all();
+    return t;
+  }
+
+  bool wait(uint64_t t, double timeout_s) {
+    std::unique_lock<std::mutex> lk(mu);
+    if (!reqs.count(t)) throw ArgError("unknown ticket");
+    auto pred = [&] { return reqs[t]->done || !fatal.empty(); };
+    if (timeout_s < 0) cv_done.wait(lk, pred);
+    else if (!cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred)) return false;
+    if (!reqs[t]->done) throw HipError("server stopped: " + fatal);
+    ticket_of.erase(&reqs[t]->hp);
+    reqs.erase(t);
+    return true;
+  }
+
+  void finish(const std::vector<HostProblem*>& fin) {
+    std::lock_guard<std::mutex> g(mu);
+    for (auto* hp : fin) {
+      auto it = ticket_of.find(hp);
+      if (it != ticket_of.end()) reqs[it->second]->done = true;
+    }
+    cv_done.notify_all();
+  }
+
+  void fail(const std::string& msg) {
+    std::lock_guard<std::mutex> g(mu);
+    if (fatal.empty()) fatal = msg;
+    for (auto& kv : reqs)
+      if (!kv.second->done && kv.second->hp.out) kv.second->hp.out->status = TM_ERR_HIP;
+    cv_done.notify_all();
+    cv_group.notify_all();
+  }
+
+  // one group's thread: admit, step, retire until the server stops
+  void run_group(Group* g) {
+    try {
+      TM_HIP(hipSetDevice(ctx->device));
+      while (true) {
+        std::vector<HostProblem*> adm;
+        bool need_setup = false;
+        int64_t per = 0;
+        {
+          std::unique_lock<std::mutex> lk(mu);
+          const bool busy = g->bl && g->bl->busy();
+          if (!busy && g->queue.empty()) {
+            if (g->bl) {  // ran dry: give the memory back
+              lk.unlock();
+              g->bl.reset();
+              g->sub->c.pool.trim();
+              lk.lock();
+              reserved -= g->reserved;
+              g->reserved = 0;
+              cv_group.notify_all();
+            }
+            cv_group.wait(lk, [&] { return g->quit || !g->queue.empty() || !fatal.empty(); });
+            if (!fatal.empty()) return;
+            if (g->quit && g->queue.empty()) return;
+            continue;
+          }
+          if (!g->bl) {
+            for (const HostProblem* h : g->queue)
+              per = std::max<int64_t>(per, problem_footprint(*g->rep, h->in.num_modes, h->k, TM_METHOD_BLOCK));
+            per = std::max<int64_t>(per, 1) * 21 / 20;
+            const double avail = std::min(budget - reserved, 0.25 * budget);
+            const int cap = (int)std::min<double>(max_slots, std::floor(avail / (double)per));
+            if (cap < 1) {  // no memory now: wait for another group to return some
+              cv_group.wait_for(lk, std::chrono::milliseconds(20));
+              continue;
+            }
+            g->reserved = (double)cap * per;
+            reserved += g->reserved;
+            need_setup = true;
+            per = cap;  // capacity
+          }
+        }
+        if (need_setup) {
+          g->bl = std::make_unique<BatchLanczos>(&g->sub->c, *g->rep, std::vector<HostProblem*>{}, true);
+          g->bl->slot_setup((int)per, g->pbclass == 8 ? 6 : 24);
+        }
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          const int nfree = g->bl->free_slots();
+          const int refill = std::max(1, g->bl->capacity() / 8);
+          if (!g->queue.empty() && nfree > 0 &&
+              (nfree >= refill || !g->bl->busy() || g->queue.size() <= (size_t)nfree))
+            while (!g->queue.empty() && (int)adm.size() < nfree) {
+              adm.push_back(g->queue.front());
+              g->queue.pop_front();
+            }
+        }
+        if (!adm.empty()) g->bl->admit(adm);
+        if (g->bl->busy()) {
+          g->bl->block_step();
+          finish(g->bl->retire());
+        }
+      }
+    } catch (const std::exception& e) {
+      fail(e.what());
+    }
+  }
+
+  void dispatch() {
+    try {
+      TM_HIP(hipSetDevice(ctx->device));
+      while (true) {
+        std::vector<uint64_t> fresh;
+        {
+          std::unique_lock<std::mutex> lk(mu);
+          cv_in.wait(lk, [&] { return stop || !incoming.empty(); });
+          if (stop && incoming.empty()) break;
+          fresh.assign(incoming.begin(), incoming.end());
+          incoming.clear();
+        }
+        std::map<Operator*, std::vector<HostProblem*>> small;
+        for (uint64_t t : fresh) {
+          HostProblem* hp;
+          {
+            std::lock_guard<std::mutex> g(mu);
+            hp = &reqs[t]->hp;
+          }
+          Operator* op = static_cast<Operator*>(hp->in.op);
+          fill_policy(op->mesh->S, *hp);
+          if (!use_block(op->mesh->S, hp->in)) {
+            small[op].push_back(hp);
+            continue;
+          }
+          const int pbc = hp->nev <= 6 ? 8 : 16;
+          std::lock_guard<std::mutex> g(mu);
+          auto& gp = groups[{op->mesh, pbc}];
+          if (!gp) {
+            gp = std::make_unique<Group>();
+            gp->srv = this;
+            gp->mesh = op->mesh;
+            gp->rep = op;
+            gp->pbclass = pbc;
+            gp->sub = std::make_unique<SubContext>(ctx->device);
+            Group* raw = gp.get();
+            gp->th = std::thread([this, raw] { run_group(raw); });
+          }
+          gp->queue.push_back(hp);
+          cv_group.notify_all();
+        }
+        for (auto& kv : small) {
+          BatchLanczos bl(ctx, *kv.first, kv.second, false);
+          bl.run();
+          finish(kv.second);
+        }
+      }
+    } catch (const std::exception& e) {
+      fail(e.what());
+    }
+    // drain: every group finishes its queue, then stops
+    {
+      std::lock_guard<std::mutex> g(mu);
+      for (auto& kv : groups) kv.second->quit = true;
+      cv_group.notify_all();
+    }
+    for (auto& kv : groups)
+      if (kv.second->th.joinable()) kv.second->th.join();
+    groups.clear();
+  }
+};
+
+Server* server_create(Context* ctx, int max_slots, double hbm_fraction) {
+  return new Server(ctx, max_slots, hbm_fraction);
+}
+void server_destroy(Server* s) { delete s; }
+uint64_t server_submit(Server* s, const tm_problem& p, tm_result* out) { return s->submit(p, out); }
+bool server_wait(Server* s, uint64_t ticket, double timeout_s) { return s->wait(ticket, timeout_s); }
 
 }  // namespace tme

@@ -70,12 +70,15 @@ struct FactorArgs {
   long long PE, UE;
   int* status;
   const ProbParam* params;
+  const int* active;  // per problem: factor it (null: all)
+  const OpArrays* ops_b;  // per problem operator arrays (null: op for all)
 };
 
 __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const AsmTask* __restrict__ tasks,
                                                         int t0) {
   const AsmTask t = tasks[t0 + blockIdx.x];
   const int b = blockIdx.y;
+  if (fa.active && !fa.active[b]) return;
   const DevSn s = fa.sn[t.sn];
   const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r;
   double2* Pn = fa.L + b * fa.PE + s.poff;
@@ -103,9 +106,10 @@ __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const Asm
       const long long tt = fa.tslot[sl];
       const long long e1 = sl * 9 + 3 * ci + cj;
       const long long e2 = tt * 9 + 3 * cj + ci;
-      double c = fa.op.Kc[e1] + pp.s * fa.op.dKc[e1] - pp.sigma * fa.op.Mc[e1];
-      double p = fa.op.Kp[e1] + pp.s * fa.op.dKp[e1] - pp.sigma * fa.op.Mp[e1];
-      double qv = fa.op.Kp[e2] + pp.s * fa.op.dKp[e2] - pp.sigma * fa.op.Mp[e2];
+      const OpArrays& op = fa.ops_b ? fa.ops_b[b] : fa.op;
+      double c = op.Kc[e1] + pp.s * op.dKc[e1] - pp.sigma * op.Mc[e1];
+      double p = op.Kp[e1] + pp.s * op.dKp[e1] - pp.sigma * op.Mp[e1];
+      double qv = op.Kp[e2] + pp.s * op.dKp[e2] - pp.sigma * op.Mp[e2];
       v = make_double2(c + pp.pre * (p + qv), pp.pim * (p - qv));
     }
     *front_at(Pn, Un, m, w, r, gi, gj) = v;
@@ -134,6 +138,7 @@ __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __re
                                                   int kb) {
   const int id = tasks[t0 + blockIdx.x];
   const int b = blockIdx.y;
+  if (fa.active && !fa.active[b]) return;
   const DevSn s = fa.sn[id];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int jb = min(NBMAX, w - kb);
@@ -173,6 +178,7 @@ __global__ __launch_bounds__(256) void k_panel_trsm(FactorArgs fa, const RowTask
                                                     int t0, int kb) {
   const RowTask t = tasks[t0 + blockIdx.x];
   const int b = blockIdx.y;
+  if (fa.active && !fa.active[b]) return;
   const DevSn s = fa.sn[t.sn];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int jb = min(NBMAX, w - kb);
@@ -222,6 +228,7 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
   (void)kb;
   const TileTask t = tasks[t0 + blockIdx.x];
   const int b = blockIdx.y;
+  if (fa.active && !fa.active[b]) return;
   const DevSn s = fa.sn[t.sn];
   const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r;
   const int jlim = t.jend < 0 ? m : min(t.jend, w);
@@ -290,11 +297,13 @@ struct InvArgs {
   double2* L;
   double2* S;
   long long PE, SE;
+  const int* active;  // per problem (null: all)
 };
 
 __global__ __launch_bounds__(64) void k_inv_diag(InvArgs ia, const int* __restrict__ tasks, int t0, int J) {
   const int id = tasks[t0 + blockIdx.x];
   const int b = blockIdx.y;
+  if (ia.active && !ia.active[b]) return;
   const DevSn s = ia.sn[id];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int j0 = 64 * J, nb = min(64, w - j0);
@@ -350,6 +359,7 @@ __device__ __forceinline__ void tile_gemm(double2 (&acc)[4][4], int k0, int k1, 
 __global__ __launch_bounds__(256) void k_inv_z(InvArgs ia, const TileTask* __restrict__ tasks, int t0) {
   const TileTask tk = tasks[t0 + blockIdx.x];
   const int b = blockIdx.y;
+  if (ia.active && !ia.active[b]) return;
   const DevSn s = ia.sn[tk.sn];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int i0 = 64 * tk.i0, ni = min(64, w - i0), jj0 = 64 * tk.j0, nj = min(64, w - jj0);
@@ -373,6 +383,7 @@ __global__ __launch_bounds__(256) void k_inv_z(InvArgs ia, const TileTask* __res
 __global__ __launch_bounds__(256) void k_inv_tile(InvArgs ia, const TileTask* __restrict__ tasks, int t0) {
   const TileTask tk = tasks[t0 + blockIdx.x];
   const int b = blockIdx.y;
+  if (ia.active && !ia.active[b]) return;
   const DevSn s = ia.sn[tk.sn];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int i0 = 64 * tk.i0, ni = min(64, w - i0), jj0 = 64 * tk.j0, nj = min(64, w - jj0);
@@ -401,6 +412,7 @@ __global__ __launch_bounds__(256) void k_inv_tile(InvArgs ia, const TileTask* __
 __global__ __launch_bounds__(256) void k_inv_store(InvArgs ia, const int* __restrict__ tasks, int t0, int J) {
   const int id = tasks[t0 + blockIdx.x];
   const int b = blockIdx.y;
+  if (ia.active && !ia.active[b]) return;
   const DevSn s = ia.sn[id];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int j0 = 64 * J, nj = min(64, w - j0);
@@ -669,15 +681,19 @@ static FactorArgs factor_args(const Mesh& M, const Plan& P, const OpArrays& ops,
   fa.UE = P.upool_entries;
   fa.status = bb.status;
   fa.params = bb.params;
+  fa.active = nullptr;
+  fa.ops_b = nullptr;
   return fa;
 }
 
 void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& ops,
-                  const BatchBufs& bb) {
+                  const BatchBufs& bb, const int* active, int nactive, const OpArrays* ops_b) {
   FactorArgs fa = factor_args(M, P, ops, bb);
-  InvArgs ia{P.d_sn.as<DevSn>(), bb.L, bb.S, P.panel_entries, P.inv_scratch_entries};
+  fa.active = active;
+  fa.ops_b = ops_b;
+  InvArgs ia{P.d_sn.as<DevSn>(), bb.L, bb.S, P.panel_entries, P.inv_scratch_entries, active};
   const int B = bb.B;
-  KTimer t(ctx, "factor", 0, M.S.flops * B);
+  KTimer t(ctx, "factor", 0, M.S.flops * (nactive >= 0 ? nactive : B));
   for (const LevelPlan& L : P.levels) {
     if (L.nasm)
       hipLaunchKernelGGL(k_front_assemble, dim3(L.nasm, B), dim3(256), 0, ctx->stream, fa,

@@ -138,7 +138,8 @@ struct BatchBufs {
 };
 
 void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& ops,
-                  const BatchBufs& bb);
+                  const BatchBufs& bb, const int* active = nullptr, int nactive = -1,
+                  const OpArrays* ops_b = nullptr);
 // in-place solve A X = B for each problem's block X_b = X + b*ldx: p columns of
 // length n = 3 nf, column c at X_b + c*n.  bb.T must hold p * tvec_entries per problem.
 void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs& bb, double2* X,
@@ -150,10 +151,23 @@ OpArrays op_arrays(const Operator& op);
 const double2* harmonic_mass(Context* ctx, Operator& op, int k);
 // solves every problem; results written through each problem's tm_result
 void solve_problems(Context* ctx, const tm_problem* problems, int n, tm_result* results);
+
+// continuous-batching server (driver.hip)
+struct Server;
+Server* server_create(Context* ctx, int max_slots, double hbm_fraction);
+void server_destroy(Server* s);
+uint64_t server_submit(Server* s, const tm_problem& p, tm_result* out);
+bool server_wait(Server* s, uint64_t ticket, double timeout_s);
 int64_t problem_footprint(const Operator& op, int num_modes, int harmonic, int method);
 
 }  // namespace tme
 
-struct tm_context : tme::Context {};
+struct tm_context : tme::Context {
+  tme::Server* server = nullptr;  // while set, the server's thread owns the context
+};
+struct tm_server {
+  tm_context* ctx;
+  tme::Server* s;
+};
 struct tm_mesh : tme::Mesh {};
 struct tm_operator : tme::Operator {};
