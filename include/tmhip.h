@@ -149,6 +149,12 @@ typedef struct tm_problem {
   int32_t method;            /* TM_METHOD_* */
   double min_frequency;      /* Hz; > 0: solve 6 extra modes and drop f <= min_frequency
                                 (src/cyclic_solver.cpp:1011, 1058-1084) */
+  int32_t coriolis;          /* != 0 with rpm != 0 and harmonic > 0: the gyroscopic QEP
+                                (K + w D - w^2 M) phi = 0, D = i 2 Omega G(k), through its
+                                Lancaster pencil (src/cyclic_solver.cpp:986-1005,
+                                src/modal_solver.cpp:149-312); forward and backward modes
+                                of smallest |w|, whirl = sign(w); block method, >= 64 free DOFs */
+  int32_t reserved;
 } tm_problem;
 
 /* Result of one problem, in the reference's ModalResult layout

@@ -391,6 +391,9 @@ class SectorOperators:
     K_eff_full: object = None
     M_full: object = None
     u_static: np.ndarray = None
+    Gc: object = None                # gyroscopic G (omega-free) phase split (cyclic_solver.cpp:858-899)
+    Gp: object = None
+    omega: float = 0.0
 
 
 def sector_operators(m: CyclicModel, E, nu, rho, rpm):
@@ -399,9 +402,11 @@ def sector_operators(m: CyclicModel, E, nu, rho, rpm):
     K, M = fem.assemble_KM(m.nodes, m.elements, E, nu, rho)
     K_eff = K
     u = None
+    Gc = Gp = None
     if abs(omega) > 0.0:
         ax = np.eye(3)[m.rotation_axis]
-        Kw, _G = fem.assemble_rotating(m.nodes, m.elements, rho, omega, ax)
+        Kw, G = fem.assemble_rotating(m.nodes, m.elements, rho, omega, ax)
+        Gc, Gp = project_phase_split(m, G)
         K_eff = K - Kw
         if len(m.constrained) > 0:
             F = fem.assemble_centrifugal(m.nodes, m.elements, rho, omega, ax)
@@ -410,7 +415,7 @@ def sector_operators(m: CyclicModel, E, nu, rho, rpm):
             K_eff = K_eff + Ks
     Kc, Kp = project_phase_split(m, K_eff)
     Mc, Mp = project_phase_split(m, M)
-    return SectorOperators(Kc, Kp, Mc, Mp, K_eff.tocsc(), M, u)
+    return SectorOperators(Kc, Kp, Mc, Mp, K_eff.tocsc(), M, u, Gc, Gp, omega)
 
 
 def expand_modes(m: CyclicModel, X, k):
@@ -465,6 +470,88 @@ def solve_harmonic(m: CyclicModel, ops: SectorOperators, k, num_modes, rpm,
         freqs = np.array(f2); vecs = np.array(v2).T; whirl = np.array(w2, np.int32)
     shapes = expand_modes(m, vecs, k)
     return HarmonicResult(k, rpm, freqs, shapes, whirl, res.converged, lam, X, Kk, Mk)
+
+
+# --------------------------------------------------------------------------
+# Coriolis: Lancaster-linearised QEP (src/modal_solver.cpp:149-312,
+# src/cyclic_solver.cpp:986-1005)
+# --------------------------------------------------------------------------
+def lancaster_pencil(Kk, Mk, Dk):
+    """L1 = [[D, K], [K, 0]], L2 = [[M, 0], [0, K]] (build_lancaster_L1/L2,
+    src/modal_solver.cpp:151-198): L1 z = w L2 z with z = [w phi; phi] is
+    (K + w D - w^2 M) phi = 0."""
+    L1 = sp.bmat([[Dk, Kk], [Kk, None]], format="csc")
+    L2 = sp.bmat([[Mk, None], [None, Kk]], format="csc")
+    return L1, L2
+
+
+def coriolis_matrix(ops: SectorOperators, k, N):
+    """D = i 2 omega G(k) (src/cyclic_solver.cpp:986-994) with G(k) = T(k)^H G T(k).
+    G is skew-symmetric, so the right-row coupling block of the phase split is
+    -Gp^T: G(k) = Gc + P Gp - conj(P) Gp^T, skew-Hermitian, and D Hermitian (the
+    property tests/test_cyclic.cpp:678-731 checks on the direct projection).  The
+    reference's production code adds +conj(P) Gp^H, the form of the symmetric K and
+    M, which leaves its D non-Hermitian; this follows the projection instead."""
+    alpha = 2.0 * PI / N
+    P = complex(math.cos(k * alpha), math.sin(k * alpha))
+    return (2j * ops.omega) * (ops.Gc + P * ops.Gp - np.conj(P) * ops.Gp.T).tocsc()
+
+
+def select_qep(omega, vectors, num_modes, n):
+    """Selection of solve_lancaster_qep (src/modal_solver.cpp:258-307): drop |w| < 1e-6,
+    sort by |w|, keep up to num_modes forward (w > 0) and num_modes backward (w < 0),
+    re-sort by |w|; frequency |w| / 2 pi, whirl sign(w), mode = head(n) of z."""
+    keep = [i for i in range(len(omega)) if abs(omega[i]) >= 1e-6]
+    keep.sort(key=lambda i: abs(omega[i]))
+    sel, nf, nb = [], 0, 0
+    for i in keep:
+        if omega[i] > 0 and nf < num_modes:
+            sel.append(i); nf += 1
+        elif omega[i] < 0 and nb < num_modes:
+            sel.append(i); nb += 1
+        if nf >= num_modes and nb >= num_modes:
+            break
+    sel.sort(key=lambda i: abs(omega[i]))
+    freqs = np.array([abs(omega[i]) / (2 * PI) for i in sel])
+    whirl = np.array([1 if omega[i] > 0 else -1 for i in sel], np.int32)
+    shapes = vectors[:n, sel] if len(sel) else np.zeros((n, 0), complex)
+    return freqs, whirl, shapes, np.array([omega[i] for i in sel])
+
+
+def solve_qep(Kk, Mk, Dk, num_modes):
+    """The forward/backward gyroscopic modes the reference's selection keeps: the
+    eigenvalues w of the Lancaster pencil with the smallest |w| of each sign.
+    Dense generalised eigh of (L1, L2) for 2n <= 400 (the reference's dense path
+    covers 2n <= 200, src/modal_solver.cpp:224-226, and returns every eigenvalue);
+    above that, scipy eigsh with shift-invert at 0, which finds exactly the
+    eigenvalues of smallest magnitude.  (The reference's Lanczos for large
+    systems uses sigma = sqrt(||K||_F / ||M||_F) with ascending-lambda selection,
+    whose result depends on its iteration history: parity unpinned there.)"""
+    n = Kk.shape[0]
+    L1, L2 = lancaster_pencil(Kk, Mk, Dk)
+    if 2 * n <= 400:
+        A = L1.toarray(); B = L2.toarray()
+        A = 0.5 * (A + A.conj().T); B = 0.5 * (B + B.conj().T)
+        w, Z = sla.eigh(A, B)
+    else:
+        nev = min(2 * num_modes + 6, 2 * n - 2)
+        w, Z = spla.eigsh(L1, k=nev, M=L2, sigma=0.0, which="LM")
+    return select_qep(w, Z, num_modes, n) + (L1, L2)
+
+
+def solve_harmonic_qep(m: CyclicModel, ops: SectorOperators, k, num_modes, rpm):
+    """QEP branch of solve_harmonic for k > 0 with Coriolis (src/cyclic_solver.cpp:986-1005,
+    1058-1096): no duplication; truncated to num_modes."""
+    N = m.num_sectors
+    Kk = harmonic_matrix(ops.Kc, ops.Kp, k, N)
+    Mk = harmonic_matrix(ops.Mc, ops.Mp, k, N)
+    Dk = coriolis_matrix(ops, k, N)
+    freqs, whirl, heads, om, L1, L2 = solve_qep(Kk, Mk, Dk, num_modes)
+    freqs, whirl, heads, om = freqs[:num_modes], whirl[:num_modes], heads[:, :num_modes], om[:num_modes]
+    shapes = expand_modes(m, heads, k)
+    r = HarmonicResult(k, rpm, freqs, shapes, whirl, True, om, heads, Kk, Mk)
+    r.D = Dk
+    return r
 
 
 def solve_at_rpm(m: CyclicModel, E, nu, rho, rpm, num_modes, harmonics=None,

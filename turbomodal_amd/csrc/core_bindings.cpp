@@ -617,12 +617,6 @@ class CyclicSymmetrySolver {
     else
       for (int k : hi)
         if (k >= 0 && k <= max_k) ks.push_back(k);
-    for (double rpm : rpms)
-      if (coriolis && rpm != 0.0)
-        for (int k : ks)
-          if (k > 0)
-            throw std::runtime_error(
-                "include_coriolis with rpm != 0 needs the Lancaster QEP path, which this engine does not provide");
     const int nn = mesh_.num_nodes();
     const size_t np = rpms.size() * ks.size();
     std::vector<tm_problem> probs(np);
@@ -646,6 +640,8 @@ class CyclicSymmetrySolver {
         p.tol_rel = 1e-12;
         p.max_restarts = 1000;
         p.method = method_;
+        p.coriolis = coriolis ? 1 : 0;
+        p.reserved = 0;
         p.min_frequency = min_freq;
         freq[q].assign(num_modes, 0.0);
         ev[q].assign(num_modes, 0.0);

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <numeric>
 #include <condition_variable>
 #include <deque>
@@ -521,7 +522,7 @@ const double2* harmonic_mass(Context* ctx, Operator& op, int k) {
                      M.d_rowptr.as<int>(), M.d_colidx.as<int>(), M.d_tslot.as<int>(),
                      M.d_pinned.as<unsigned char>(), op.Mc.as<double>(), op.Mp.as<double>(),
                      (const double*)nullptr, (const double*)nullptr, 0.0, std::cos(k * alpha),
-                     std::sin(k * alpha), b.as<double2>());
+                     std::sin(k * alpha), b.as<double2>(), 0, make_double2(1.0, 0.0));
   TM_HIP(hipGetLastError());
   TM_HIP(hipStreamSynchronize(ctx->stream));  // complete before other streams read it
   const double2* p = b.as<double2>();
@@ -538,6 +539,7 @@ struct HostProblem {
   int k = 0;
   double omega = 0, s = 0, pre = 1, pim = 0;
   bool degenerate = false;
+  bool qep = false;  // Coriolis: the Lancaster pencil instead of (K, M)
   int nev_target = 0, nev = 0, ncv = 0;
   int restarts = 0, max_restarts = 1000;
   bool done = false, converged = false, failed = false;
@@ -562,9 +564,20 @@ static void fill_policy(const Symbolic& S, HostProblem& hp) {
   hp.s = hp.omega * hp.omega;
   hp.degenerate = hp.k > 0 && !(N % 2 == 0 && hp.k == max_k);
   const int M = hp.in.num_modes;
-  hp.nev_target = hp.degenerate ? (M + 1) / 2 : M;
   const int n_free = S.n_free_dofs;
   const int extra = hp.in.min_frequency > 0.0 ? 6 : 0;
+  // Coriolis with rotation at k > 0: the QEP, all num_modes wanted, no duplication
+  // (src/cyclic_solver.cpp:975-1005); two extra pairs keep the |w| ordering safe
+  hp.qep = hp.in.coriolis != 0 && hp.omega != 0.0 && hp.k > 0;
+  if (hp.qep) {
+    hp.degenerate = false;
+    hp.nev_target = M;
+    hp.nev = std::min(M + 2 + extra, 2 * n_free - 1);
+    hp.ncv = 0;
+    hp.max_restarts = hp.in.max_restarts > 0 ? hp.in.max_restarts : 1000;
+    return;
+  }
+  hp.nev_target = hp.degenerate ? (M + 1) / 2 : M;
   hp.nev = std::min(hp.nev_target + extra, n_free - 1);
   if (n_free <= 200) hp.ncv = n_free;  // full Krylov space == the reference's dense solve
   else hp.ncv = std::min(std::max(4 * hp.nev + 1, 40), n_free);
@@ -629,6 +642,8 @@ class BatchLanczos {
       nevmax_ = std::max(nevmax_, hp->nev);
     }
     kmax_ = ncvmax_ + 1;
+    nvec_ = n_;
+    cs_ = n_;
     ldb_ = (long long)(ncvmax_ + 1) * n_;
     nchunk_ = static_cast<int>((n_ + kDotChunk - 1) / kDotChunk);
     // right-hand sides per triangular sweep: the Krylov block, and the Ritz
@@ -1025,17 +1040,24 @@ class BatchLanczos {
   // residual, ||K x||, x^H M x per Ritz vector (fixed-order chunk reduction)
   void residuals(double2* X, double2* KX, double2* MX, long long ldx, int nemax, const std::vector<int>& nc,
                  const DevBuf& kp) {
+    (void)kp;
     DevBuf dnc = upload(pool(), nc, stream());
-    spmv_cols(kp, X, ldx, dnc, nemax, KX);
-    spmv_cols(mk_ptrs_, X, ldx, dnc, nemax, MX);
+    apply_A(X, ldx, dnc, nemax, KX);
+    apply_B(X, ldx, dnc, nemax, MX);
     std::vector<double> lam((size_t)B_ * nemax, 0.0);
     for (int b = 0; b < B_; ++b)
-      for (int c = 0; c < nc[b]; ++c) lam[(size_t)b * nemax + c] = probs_[b]->theta_sorted_lambda[c];
+      for (int c = 0; c < nc[b]; ++c) {
+        const double t = probs_[b]->theta_sorted_lambda[c];
+        lam[(size_t)b * nemax + c] = qep_ ? (t != 0.0 ? 1.0 / t : 0.0) : t;
+      }
     DevBuf dl = upload(pool(), lam, stream());
     DevBuf part(pool(), sizeof(double) * 3 * (size_t)B_ * nemax * nchunk_);
-    hipLaunchKernelGGL(k_residual_partial, dim3(nchunk_, B_, nemax), dim3(256), 0, stream(), X,
-                       KX, MX, ldx, n_, dl.as<double>(), nemax, dnc.as<int>(),
-                       part.as<double>(), nchunk_);
+    if (qep_)  // X = [omega x; x]: ||K x + omega D x - omega^2 M x|| / ||K x|| from the b halves
+      hipLaunchKernelGGL(k_residual_partial, dim3(nchunk_, B_, nemax), dim3(256), 0, stream(), X + n_, MX + n_,
+                         KX + n_, ldx, n_, nvec_, dl.as<double>(), nemax, dnc.as<int>(), part.as<double>(), nchunk_);
+    else
+      hipLaunchKernelGGL(k_residual_partial, dim3(nchunk_, B_, nemax), dim3(256), 0, stream(), X, KX, MX, ldx,
+                         nvec_, nvec_, dl.as<double>(), nemax, dnc.as<int>(), part.as<double>(), nchunk_);
     std::vector<double> hp_part(3 * (size_t)B_ * nemax * nchunk_);
     TM_HIP(hipMemcpyAsync(hp_part.data(), part.p, sizeof(double) * hp_part.size(), hipMemcpyDeviceToHost, stream()));
     TM_HIP(hipStreamSynchronize(stream()));
@@ -1055,6 +1077,20 @@ class BatchLanczos {
         hp->residual[c] = den > 0 ? std::sqrt(num / den) : std::sqrt(num);
         hp->xmx[c] = xm;
       }
+    }
+    if (qep_) {  // Re(b^H M b) (the head of A z is M b); the modal mass is omega^2 times it
+      DevBuf hpart(pool(), sizeof(double) * (size_t)B_ * nemax * nchunk_);
+      hipLaunchKernelGGL(k_head_dot, dim3(nchunk_, B_, nemax), dim3(256), 0, stream(), X + n_, KX, ldx, nvec_, n_,
+                         dnc.as<int>(), nemax, hpart.as<double>(), nchunk_);
+      std::vector<double> hh((size_t)B_ * nemax * nchunk_);
+      TM_HIP(hipMemcpyAsync(hh.data(), hpart.p, sizeof(double) * hh.size(), hipMemcpyDeviceToHost, stream()));
+      TM_HIP(hipStreamSynchronize(stream()));
+      for (int b = 0; b < B_; ++b)
+        for (int c = 0; c < nc[b]; ++c) {
+          double s = 0;
+          for (int q = 0; q < nchunk_; ++q) s += hh[((size_t)b * nemax + c) * nchunk_ + q];
+          probs_[b]->xmx[c] = s;
+        }
     }
   }
 
@@ -1188,7 +1224,7 @@ class BatchLanczos {
         for (int c = 0; c < nc[b]; ++c) worst = std::max(worst, hp->residual[c]);
         // block results are already Rayleigh-Ritz optimal on a large basis: refine
         // only those left above the bar
-        if (worst > (block_ ? kBlockStall : kRefineTol) || !(worst == worst)) {
+        if (!qep_ && (worst > (block_ ? kBlockStall : kRefineTol) || !(worst == worst))) {
           nr[b] = nc[b];
           any = true;
         }
@@ -1213,6 +1249,10 @@ class BatchLanczos {
         continue;
       }
       const int nmodes_req = hp->in.num_modes;
+      if (qep_) {
+        deliver_qep(b, hp, X, ldx);
+        continue;
+      }
       // min_frequency filter, then truncation to nev_target (src/cyclic_solver.cpp:1058-1096)
       std::vector<int> keep;
       for (int u = 0; u < hp->n_extract; ++u) {
@@ -1276,7 +1316,7 @@ class BatchLanczos {
         const long long tot = (long long)S_.nn * nm;
         hipLaunchKernelGGL(k_expand_modes, dim3((tot + 255) / 256), dim3(256), 0, stream(), S_.nn,
                            M_.d_free_of_full.as<int>(), M_.d_node_class.as<int>(), std::cos(alpha),
-                           std::sin(alpha), S_.c1, S_.c2, S_.cax, X + b * ldx, n_,
+                           std::sin(alpha), S_.c1, S_.c2, S_.cax, X + b * ldx, nvec_,
                            dsrc.as<int>(), dconj.as<int>(), nm, hp->pre, hp->pim, dout.as<double2>());
         TM_HIP(hipMemcpyAsync(out->mode_shapes, dout.p, sizeof(double2) * 3 * (size_t)S_.nn * nm,
                               hipMemcpyDeviceToHost, stream()));
@@ -1284,6 +1324,80 @@ class BatchLanczos {
       }
     }
     TM_HIP(hipStreamSynchronize(stream()));
+  }
+
+  // Results of a Coriolis problem (src/modal_solver.cpp:258-307, src/cyclic_solver.cpp:1058-1096):
+  // drop |omega| < 1e-6, keep up to num_modes forward (omega > 0) and backward
+  // (omega < 0) pairs by |omega|, min_frequency filter, truncate to num_modes;
+  // f = |omega| / 2 pi, whirl = sign(omega), no duplication.  The mode is the
+  // a half of z, taken as omega b: the Ritz vectors are accurate in the L2
+  // norm, which weighs b with K and a only with M.
+  void deliver_qep(int b, HostProblem* hp, const double2* X, long long ldx) {
+    tm_result* out = hp->out;
+    const int M = hp->in.num_modes;
+    std::vector<double> om(hp->n_extract);
+    std::vector<int> ord;
+    for (int u = 0; u < hp->n_extract; ++u) {
+      const double mu = hp->theta_sorted_lambda[u];
+      om[u] = mu != 0.0 ? 1.0 / mu : 0.0;
+      if (std::fabs(om[u]) >= 1e-6) ord.push_back(u);
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return std::fabs(om[x]) < std::fabs(om[y]); });
+    std::vector<int> sel;
+    int nf = 0, nb = 0;
+    for (int u : ord) {
+      const double w = om[u];
+      if (w > 0 && nf < M) { sel.push_back(u); ++nf; }
+      else if (w < 0 && nb < M) { sel.push_back(u); ++nb; }
+      if (nf >= M && nb >= M) break;
+    }
+    std::vector<int> keep;
+    for (int u : sel) {
+      const double f = std::fabs(om[u]) / (2.0 * M_PI);
+      if (hp->in.min_frequency > 0.0 && !(f > hp->in.min_frequency)) continue;
+      keep.push_back(u);
+    }
+    if ((int)keep.size() > M) keep.resize(M);
+    if (keep.empty()) {
+      out->status = TM_ERR_NUMERIC;
+      out->n_modes = 0;
+      return;
+    }
+    const int nm = static_cast<int>(keep.size());
+    std::vector<int> conjq(nm, 0);
+    for (int q = 0; q < nm; ++q) {
+      const int u = keep[q];
+      const double w = om[u];
+      if (out->frequencies) out->frequencies[q] = std::fabs(w) / (2.0 * M_PI);
+      if (out->whirl) out->whirl[q] = w > 0 ? 1 : -1;
+      if (out->eigenvalues) out->eigenvalues[q] = w;
+      if (out->residuals) out->residuals[q] = hp->residual[u];
+      if (out->modal_mass) out->modal_mass[q] = static_cast<double>(S_.N) * w * w * hp->xmx[u];
+    }
+    out->n_modes = nm;
+    out->n_unique = nm;
+    out->converged = hp->converged ? 1 : 0;
+    out->restarts = hp->restarts;
+    out->status = TM_OK;
+    if (out->mode_shapes) {
+      DevBuf dsrc = upload(pool(), keep, stream()), dconj = upload(pool(), conjq, stream());
+      DevBuf dout(pool(), sizeof(double2) * 3 * (size_t)S_.nn * nm);
+      const double alpha = 2.0 * M_PI / S_.N;
+      const long long tot = (long long)S_.nn * nm;
+      hipLaunchKernelGGL(k_expand_modes, dim3((tot + 255) / 256), dim3(256), 0, stream(), S_.nn,
+                         M_.d_free_of_full.as<int>(), M_.d_node_class.as<int>(), std::cos(alpha), std::sin(alpha),
+                         S_.c1, S_.c2, S_.cax, X + b * ldx + n_, nvec_, dsrc.as<int>(), dconj.as<int>(), nm, hp->pre,
+                         hp->pim, dout.as<double2>());
+      TM_HIP(hipMemcpyAsync(out->mode_shapes, dout.p, sizeof(double2) * 3 * (size_t)S_.nn * nm,
+                            hipMemcpyDeviceToHost, stream()));
+      TM_HIP(hipStreamSynchronize(stream()));
+      double2* ms = reinterpret_cast<double2*>(out->mode_shapes);
+      for (long long r = 0; r < 3LL * S_.nn; ++r)
+        for (int q = 0; q < nm; ++q) {
+          const double w = om[keep[q]];
+          ms[r * nm + q] = make_double2(w * ms[r * nm + q].x, w * ms[r * nm + q].y);
+        }
+    }
   }
 
   // -------------------------------------------------------------------------
@@ -1306,10 +1420,18 @@ class BatchLanczos {
   static constexpr double kBlockTol = 5e-9;
   static constexpr double kBlockStall = 1e-8;
   static constexpr double kBlockStallMax = 1e-6;
+  static constexpr double kQepStallMax = 8e-9;
   static constexpr int kBlockMaxRestarts = 40;
   // TMHIP_TRACE=1 prints the convergence history of the block method to stderr
   const bool trace_ = std::getenv("TMHIP_TRACE") != nullptr;
   bool block_ = false;
+  // Coriolis pencil (L1, L2) on 2n-long vectors z = [a; b] instead of (K, M) on n
+  bool qep_ = false;
+  long long nvec_ = 0;  // basis vector length: n, or 2n for the pencil
+  long long cs_ = 0;    // column stride of the vectors handed to the SpMVs
+  std::vector<DevBuf> dblk_;
+  std::vector<const double2*> dptr_host_;
+  DevBuf dptr_;
   int pb_ = 0, mmax_ = 0, nkeep_ = 0, nts_ = 0;
   long long ldv_ = 0, ldw_ = 0, ldk_ = 0;
   DevBuf Vb_, MVb_, KVb_, Wb_, MWb_, KWb_, Xb_, KXb_, MXb_, tsp_, tsc_;
@@ -1322,14 +1444,14 @@ class BatchLanczos {
   std::vector<std::vector<double>> prev_res_;
 
   void block_setup() {
-    pb_ = nevmax_ <= 6 ? 8 : 16;
+    pb_ = (nevmax_ <= 6 && !qep_) ? 8 : 16;
     mmax_ = pb_ == 8 ? 48 : 64;
     nkeep_ = mmax_ / 2;
     static_assert(kTsMaxRows >= 64, "tsmm coefficient rows");
-    ldv_ = (long long)mmax_ * n_;
-    ldw_ = (long long)pb_ * n_;
-    ldk_ = (long long)nkeep_ * n_;
-    nts_ = static_cast<int>((n_ + kTsChunk - 1) / kTsChunk);
+    ldv_ = (long long)mmax_ * nvec_;
+    ldw_ = (long long)pb_ * nvec_;
+    ldk_ = (long long)nkeep_ * nvec_;
+    nts_ = static_cast<int>((nvec_ + kTsChunk - 1) / kTsChunk);
     for (DevBuf* b : {&Vb_, &MVb_, &KVb_}) b->alloc(pool(), sizeof(double2) * ldv_ * B_);
     for (DevBuf* b : {&Wb_, &MWb_, &KWb_}) b->alloc(pool(), sizeof(double2) * ldw_ * B_);
     for (DevBuf* b : {&Xb_, &KXb_, &MXb_}) b->alloc(pool(), sizeof(double2) * ldk_ * B_);
@@ -1340,6 +1462,8 @@ class BatchLanczos {
     prev_res_.assign(B_, std::vector<double>());
     kblk_.resize(B_);
     kptr_host_.assign(B_, nullptr);
+    dblk_.resize(B_);
+    dptr_host_.assign(B_, nullptr);
   }
 
   // FP64 K(k) blocks of the problems in the listed slots (residuals, Rayleigh-Ritz)
@@ -1352,21 +1476,33 @@ class BatchLanczos {
                          M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), M_.d_tslot.as<int>(),
                          M_.d_pinned.as<unsigned char>(), sop.Kc.as<double>(), sop.Kp.as<double>(),
                          sop.dKc.as<double>(), sop.dKp.as<double>(), hp->s, hp->pre, hp->pim,
-                         kblk_[b].as<double2>());
+                         kblk_[b].as<double2>(), 0, make_double2(1.0, 0.0));
       kptr_host_[b] = kblk_[b].as<double2>();
+      if (qep_) {  // -D(k) = -i 2 omega G(k), G skew: Gc + P Gp - conj(P) Gp^T
+        if (!dblk_[b].p) dblk_[b].alloc(pool(), sizeof(double2) * 9 * op_.nblk);
+        hipLaunchKernelGGL(k_build_phase_blocks, dim3((S_.nf + 127) / 128), dim3(128), 0, stream(), S_.nf,
+                           M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), M_.d_tslot.as<int>(),
+                           M_.d_pinned.as<unsigned char>(), sop.Gc.as<double>(), sop.Gp.as<double>(),
+                           (const double*)nullptr, (const double*)nullptr, 0.0, hp->pre, hp->pim,
+                           dblk_[b].as<double2>(), 1, make_double2(0.0, -2.0 * hp->omega));
+        dptr_host_[b] = dblk_[b].as<double2>();
+      }
     }
     TM_HIP(hipGetLastError());
     kptr_ = upload(pool(), kptr_host_, stream());
+    if (qep_) dptr_ = upload(pool(), dptr_host_, stream());
   }
 
   DevBuf ints(const std::vector<int>& v) { return upload(pool(), v, stream()); }
 
-  // C_b = A_b^H B_b (ka_b x kb, kb <= 16), returned on the host as [b][a][c] (rows of 16)
+  // C_b = A_b^H B_b (ka_b x kb, kb <= 16) over the first len rows (default: all)
+  // of columns nvec_ apart, returned on the host as [b][a][c] (rows of 16)
   std::vector<double2> tsgemm(const double2* A, long long lda, const DevBuf& ka, int kamax, const double2* Bm,
-                              long long ldb, int kb, const DevBuf& act) {
-    KTimer t(ctx_, "gram", 0, 8.0 * n_ * kamax * kb * B_);
+                              long long ldb, int kb, const DevBuf& act, long long len = -1) {
+    if (len < 0) len = nvec_;
+    KTimer t(ctx_, "gram", 0, 8.0 * len * kamax * kb * B_);
     hipLaunchKernelGGL(k_tsgemm_partial, dim3(nts_, B_, (kamax + 15) / 16), dim3(256), 0, stream(), A, lda, Bm, ldb,
-                       n_, ka.as<int>(), kb, act.as<int>(), mmax_, tsp_.as<double2>(), nts_);
+                       len, nvec_, ka.as<int>(), kb, act.as<int>(), mmax_, tsp_.as<double2>(), nts_);
     hipLaunchKernelGGL(k_tsgemm_reduce, dim3((kamax * 16 + 255) / 256, B_), dim3(256), 0, stream(),
                        tsp_.as<double2>(), ka.as<int>(), kb, act.as<int>(), mmax_, nts_, tsc_.as<double2>());
     std::vector<double2> h((size_t)B_ * mmax_ * 16);
@@ -1376,20 +1512,25 @@ class BatchLanczos {
   }
 
   // Y_b[:, yoff_b + c] = beta Y + alpha sum_a A_b[:, a] C_b[a][c], c < kc (C on the host, [b][a][16])
+  // (first len rows, default all; columns nvec_ apart)
   void tsmm(const double2* A, long long lda, const DevBuf& ka, const std::vector<double2>& C, int kc, double alpha,
-            double beta, double2* Y, long long ldy, const DevBuf* yoff, const DevBuf& act) {
+            double beta, double2* Y, long long ldy, const DevBuf* yoff, const DevBuf& act, long long len = -1) {
+    if (len < 0) len = nvec_;
     DevBuf dc = upload(pool(), C, stream());
     KTimer t(ctx_, "vec", 0, 0);
-    hipLaunchKernelGGL(k_tsmm, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), A, lda, n_, dc.as<double2>(),
+    hipLaunchKernelGGL(k_tsmm, dim3((len + 255) / 256, B_), dim3(256), 0, stream(), A, lda, len, nvec_, dc.as<double2>(),
                        ka.as<int>(), kc, act.as<int>(), mmax_, alpha, beta, Y, ldy,
                        yoff ? yoff->as<int>() : (const int*)nullptr);
   }
 
-  // Y_b[:, c] = A_b X_b[:, c], c < ncol_b (<= cmax); same column stride ld for X and Y
-  void spmv_cols(const DevBuf& blocks, const double2* X, long long ld, const DevBuf& ncol, int cmax, double2* Y) {
+  // Y_b[:, c] (+)= A_b X_b[:, c], c < ncol_b (<= cmax), n rows; problems ld apart,
+  // columns cs_ apart, for X and Y alike
+  void spmv_cols(const DevBuf& blocks, const double2* X, long long ld, const DevBuf& ncol, int cmax, double2* Y,
+                 bool accumulate = false) {
     KTimer t(ctx_, "spmv_m", 0, 0);
+    const long long cstride = cs_ > 0 ? cs_ : n_;
     for (int c0 = 0; c0 < cmax; c0 += 8) {
-      const long long off = (long long)c0 * n_;
+      const long long off = (long long)c0 * cstride;
       std::vector<int> nc;
       const DevBuf* dn = &ncol;
       DevBuf shifted;
@@ -1402,9 +1543,35 @@ class BatchLanczos {
         dn = &shifted;
       }
       hipLaunchKernelGGL(k_spmv_cols<8>, dim3((S_.nf + 255) / 256, B_), dim3(256), 0, stream(), S_.nf,
-                         M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), blocks.as<const double2*>(), X + off, ld, n_,
-                         dn->as<int>(), Y + off, ld);
+                         M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), blocks.as<const double2*>(), X + off, ld,
+                         cstride, dn->as<int>(), Y + off, ld, accumulate ? 1 : 0);
     }
+  }
+
+  // The Rayleigh-Ritz pencil (A, B) with B positive definite: (K, M) with
+  // eigenvalues lambda = omega^2, or for the Coriolis problem on z = [a; b]
+  // (a = omega x, b = x) the definite pencil (Bc, L2), Bc = [[0, M], [M, -D]],
+  // L2 = diag(M, K), with eigenvalues mu = 1 / omega.  Bc = L2 Op for the
+  // shift-invert operator Op = L1^-1 L2 of the Lancaster pencil
+  // L1 = [[D, K], [K, 0]] (L1 z = omega L2 z), so the Ritz values of this pencil
+  // are those of Op in the L2 inner product: bounded by its spectrum, with the
+  // wanted largest |mu| at its two ends (a Rayleigh-Ritz step on the indefinite
+  // L1 itself puts spurious values into the gap around omega = 0).
+  //
+  // B z: M x, or L2 z = [M a; K b]
+  void apply_B(const double2* X, long long ld, const DevBuf& ncol, int cmax, double2* Y) {
+    spmv_cols(mk_ptrs_, X, ld, ncol, cmax, Y);
+    if (qep_) spmv_cols(kptr_, X + n_, ld, ncol, cmax, Y + n_);
+  }
+  // A z: K x, or Bc z = [M b; M a - D b] (dptr_ holds -D)
+  void apply_A(const double2* X, long long ld, const DevBuf& ncol, int cmax, double2* Y) {
+    if (!qep_) {
+      spmv_cols(kptr_, X, ld, ncol, cmax, Y);
+      return;
+    }
+    spmv_cols(mk_ptrs_, X + n_, ld, ncol, cmax, Y);
+    spmv_cols(mk_ptrs_, X, ld, ncol, cmax, Y + n_);
+    spmv_cols(dptr_, X + n_, ld, ncol, cmax, Y + n_, true);
   }
 
   // M-orthogonalise W against V[:, 0:nv_b] (twice), orthonormalise it (Cholesky
@@ -1426,8 +1593,8 @@ class BatchLanczos {
         std::vector<double2> C = tsgemm(MVb_.as<double2>(), ldv_, dnv, nvmax, Wb_.as<double2>(), ldw_, pb_, dact);
         tsmm(Vb_.as<double2>(), ldv_, dnv, C, pb_, -1.0, 1.0, Wb_.as<double2>(), ldw_, nullptr, dact);
       }
-    spmv_cols(mk_ptrs_, Wb_.as<double2>(), ldw_, dpb, pb_, MWb_.as<double2>());
-    spmv_cols(kptr_, Wb_.as<double2>(), ldw_, dpb, pb_, KWb_.as<double2>());
+    apply_B(Wb_.as<double2>(), ldw_, dpb, pb_, MWb_.as<double2>());
+    if (!qep_) apply_A(Wb_.as<double2>(), ldw_, dpb, pb_, KWb_.as<double2>());
     std::vector<double2> G = tsgemm(Wb_.as<double2>(), ldw_, dpb, pb_, MWb_.as<double2>(), ldw_, pb_, dact);
     std::vector<double2> Rinv((size_t)B_ * mmax_ * 16, make_double2(0.0, 0.0));
     std::vector<int> bad, ok(act);
@@ -1440,8 +1607,29 @@ class BatchLanczos {
     }
     DevBuf dok = ints(ok), doff = ints(nvc);
     tsmm(Wb_.as<double2>(), ldw_, dpb, Rinv, pb_, 1.0, 0.0, Vb_.as<double2>(), ldv_, &doff, dok);
-    tsmm(MWb_.as<double2>(), ldw_, dpb, Rinv, pb_, 1.0, 0.0, MVb_.as<double2>(), ldv_, &doff, dok);
-    tsmm(KWb_.as<double2>(), ldw_, dpb, Rinv, pb_, 1.0, 0.0, KVb_.as<double2>(), ldv_, &doff, dok);
+    if (!qep_) {
+      tsmm(MWb_.as<double2>(), ldw_, dpb, Rinv, pb_, 1.0, 0.0, MVb_.as<double2>(), ldv_, &doff, dok);
+      tsmm(KWb_.as<double2>(), ldw_, dpb, Rinv, pb_, 1.0, 0.0, KVb_.as<double2>(), ldv_, &doff, dok);
+    } else {
+      // the Coriolis pencil's products of the new block by fresh SpMVs: scaling
+      // the remainder's products by Rinv leaves errors of eps cond(R) (large once
+      // Ritz vectors converge and the remainder is nearly singular), and the
+      // shift-invert pencil does not filter them from its Ritz vectors
+      tsmm(Wb_.as<double2>(), ldw_, dpb, Rinv, pb_, 1.0, 0.0, MWb_.as<double2>(), ldw_, nullptr, dok);
+      std::vector<int> pok(B_, 0);
+      for (int b = 0; b < B_; ++b) pok[b] = ok[b] ? pb_ : 0;
+      DevBuf dpok = ints(pok);
+      apply_B(MWb_.as<double2>(), ldw_, dpok, pb_, Wb_.as<double2>());
+      apply_A(MWb_.as<double2>(), ldw_, dpok, pb_, KWb_.as<double2>());
+      KTimer t(ctx_, "vec", 0, 0);
+      hipLaunchKernelGGL(k_copy_cols, dim3((nvec_ + 255) / 256, B_), dim3(256), 0, stream(), Wb_.as<double2>(), ldw_,
+                         (const int*)nullptr, MVb_.as<double2>(), ldv_, doff.as<int>(), pb_, nvec_, dok.as<int>());
+      hipLaunchKernelGGL(k_copy_cols, dim3((nvec_ + 255) / 256, B_), dim3(256), 0, stream(), KWb_.as<double2>(), ldw_,
+                         (const int*)nullptr, KVb_.as<double2>(), ldv_, doff.as<int>(), pb_, nvec_, dok.as<int>());
+      std::fill(Rinv.begin(), Rinv.end(), make_double2(0.0, 0.0));  // the products below are of V_new itself
+      for (int b = 0; b < B_; ++b)
+        for (int c = 0; c < pb_; ++c) Rinv[(size_t)b * mmax_ * 16 + c * 16 + c] = make_double2(1.0, 0.0);
+    }
     // new column block of H and G: V[:, 0:nv+pb]^H (K W, M W) Rinv
     std::vector<int> nvn(B_, 0);
     int nvnmax = 0;
@@ -1453,7 +1641,8 @@ class BatchLanczos {
     if (nvnmax > 0) {
       DevBuf dnvn = ints(nvn);
       std::vector<double2> hk = tsgemm(Vb_.as<double2>(), ldv_, dnvn, nvnmax, KWb_.as<double2>(), ldw_, pb_, dok);
-      std::vector<double2> hm = tsgemm(Vb_.as<double2>(), ldv_, dnvn, nvnmax, MWb_.as<double2>(), ldw_, pb_, dok);
+      std::vector<double2> hm = tsgemm(Vb_.as<double2>(), ldv_, dnvn, nvnmax, (qep_ ? Wb_ : MWb_).as<double2>(),
+                                       ldw_, pb_, dok);
       using cd = std::complex<double>;
       for (int b = 0; b < B_; ++b) {
         if (!ok[b]) continue;
@@ -1533,12 +1722,16 @@ class BatchLanczos {
     for (const auto& g : by_salt)
       for (int c = 0; c < pb_; ++c) {
         std::vector<double> v = random_vector(7919u * (uint32_t)(c + 1) + 104729u * (uint32_t)g.first);
+        if (qep_) {  // the b half from its own seed
+          std::vector<double> v2 = random_vector(7919u * (uint32_t)(c + 1) + 104729u * (uint32_t)g.first + 31u);
+          v.insert(v.end(), v2.begin(), v2.end());
+        }
         const int first = g.second[0];
-        TM_HIP(hipMemcpyAsync(Wb_.as<double2>() + first * ldw_ + (long long)c * n_, v.data(), sizeof(double2) * n_,
-                              hipMemcpyHostToDevice, stream()));
+        TM_HIP(hipMemcpyAsync(Wb_.as<double2>() + first * ldw_ + (long long)c * nvec_, v.data(),
+                              sizeof(double2) * nvec_, hipMemcpyHostToDevice, stream()));
         for (size_t i = 1; i < g.second.size(); ++i)
-          TM_HIP(hipMemcpyAsync(Wb_.as<double2>() + g.second[i] * ldw_ + (long long)c * n_,
-                                Wb_.as<double2>() + first * ldw_ + (long long)c * n_, sizeof(double2) * n_,
+          TM_HIP(hipMemcpyAsync(Wb_.as<double2>() + g.second[i] * ldw_ + (long long)c * nvec_,
+                                Wb_.as<double2>() + first * ldw_ + (long long)c * nvec_, sizeof(double2) * nvec_,
                                 hipMemcpyDeviceToDevice, stream()));
         TM_HIP(hipStreamSynchronize(stream()));
       }
@@ -1548,15 +1741,18 @@ class BatchLanczos {
  public:
   // Allocates the workspaces of `capacity` slots for problems of at most nevmax
   // wanted pairs (one block size).
-  void slot_setup(int capacity, int nevmax) {
+  void slot_setup(int capacity, int nevmax, bool qep = false) {
     HostTimer h(ctx_, "setup");
     B_ = capacity;
     probs_.assign(B_, nullptr);
     n_ = 3LL * S_.nf;
+    qep_ = qep;
+    nvec_ = qep ? 2 * n_ : n_;
+    cs_ = nvec_;
     nevmax_ = nevmax;
     ncvmax_ = 1;
-    nchunk_ = static_cast<int>((n_ + kDotChunk - 1) / kDotChunk);
-    const int pcols = std::min(16, std::max(nevmax_ <= 6 ? 8 : 16, nevmax_));
+    nchunk_ = static_cast<int>((nvec_ + kDotChunk - 1) / kDotChunk);
+    const int pcols = std::min(16, std::max((nevmax_ <= 6 && !qep_) ? 8 : 16, nevmax_));
     bm_.alloc(pool(), P_, B_, pcols);
     act_.alloc(pool(), sizeof(int) * B_);
     h_params_.assign(B_, ProbParam{1.0, 0.0, 0.0, 1.0});
@@ -1588,7 +1784,8 @@ class BatchLanczos {
         adm.erase(adm.begin());
         probs_[b] = hp;
         hp->t_admit = now;
-        h_params_[b] = {hp->pre, hp->pim, hp->s, hp->in.shift};
+        // the pencil's shift-invert operator needs K(k) itself: shift 0
+        h_params_[b] = {hp->pre, hp->pim, hp->s, qep_ ? 0.0 : hp->in.shift};
         Operator& sop = *static_cast<Operator*>(hp->in.op);
         if (sop.mesh != &M_) throw ArgError("admit: problem of another mesh");
         mk_[b] = harmonic_mass(ctx_, sop, hp->k);
@@ -1625,6 +1822,7 @@ class BatchLanczos {
     }
     HostTimer h(ctx_, "iterate");
     random_block(mask, std::vector<int>(B_, 0));
+    if (qep_) smooth_block(mask);
     append_or_retry(mask);
     for (int b : slots)
       if (mask[b]) {
@@ -1641,14 +1839,34 @@ class BatchLanczos {
     for (int b = 0; b < B_; ++b) any = any || iter_[b];
     if (!any) return;
     BatchBufs bb = bm_.bufs(P_, B_);
-    // W = (K - sigma M)^-1 M Y, Y = V[:, y0 : y0 + pb]
     DevBuf dact = ints(iter_), dy0 = ints(y0_);
-    {
-      KTimer t(ctx_, "vec", 0, 0);
-      hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), MVb_.as<double2>(), ldv_,
-                         dy0.as<int>(), Wb_.as<double2>(), ldw_, (const int*)nullptr, pb_, n_, dact.as<int>());
+    if (!qep_) {
+      // W = (K - sigma M)^-1 M Y, Y = V[:, y0 : y0 + pb]
+      {
+        KTimer t(ctx_, "vec", 0, 0);
+        hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), MVb_.as<double2>(), ldv_,
+                           dy0.as<int>(), Wb_.as<double2>(), ldw_, (const int*)nullptr, pb_, n_, dact.as<int>());
+      }
+      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>(), ldw_, dact.as<int>(), pb_);
+    } else {
+      // W = L1^-1 L2 Y: for Y = [a; b], W = [b; K^-1 (M a - D b)] (one solve with K(k))
+      std::vector<int> pbv(B_, 0);
+      for (int b = 0; b < B_; ++b) pbv[b] = iter_[b] ? pb_ : 0;
+      DevBuf dpb = ints(pbv);
+      {
+        KTimer t(ctx_, "vec", 0, 0);
+        // W_a <- Y_b ;  W_b <- (M Y_a) = the a half of L2 Y
+        hipLaunchKernelGGL(k_axpy_half, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), Vb_.as<double2>(), ldv_,
+                           dy0.as<int>(), n_, Wb_.as<double2>(), ldw_, (const int*)nullptr, 0LL, pb_, n_, nvec_, 1.0,
+                           0.0, dact.as<int>());
+        hipLaunchKernelGGL(k_axpy_half, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), MVb_.as<double2>(), ldv_,
+                           dy0.as<int>(), 0LL, Wb_.as<double2>(), ldw_, (const int*)nullptr, n_, pb_, n_, nvec_, 1.0,
+                           0.0, dact.as<int>());
+      }
+      // W_b += (-D) Y_b
+      spmv_cols(dptr_, Wb_.as<double2>(), ldw_, dpb, pb_, Wb_.as<double2>() + n_, true);
+      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>() + n_, ldw_, dact.as<int>(), pb_, nvec_);
     }
-    trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>(), ldw_, dact.as<int>(), pb_);
     for (int b = 0; b < B_; ++b)
       if (iter_[b]) probs_[b]->steps += 1;
     append_or_retry(iter_);
@@ -1688,10 +1906,10 @@ class BatchLanczos {
   // the slots between admissions (best when the whole queue is known: the
   // factorizations stay fully batched); smaller values let problems that
   // converge early hand their slots on.
-  void run_slots(std::vector<HostProblem*> queue, int capacity, int refill) {
+  void run_slots(std::vector<HostProblem*> queue, int capacity, int refill, bool qep = false) {
     int nevmax = 1;
     for (auto* hp : queue) nevmax = std::max(nevmax, hp->nev);
-    slot_setup(capacity, nevmax);
+    slot_setup(capacity, nevmax, qep);
     refill = std::max(1, std::min(refill, capacity));
     size_t next = 0;
     while (true) {
@@ -1714,6 +1932,32 @@ class BatchLanczos {
 
   // Appends W; a problem whose block lost rank (an invariant subspace was
   // reached) continues from fresh random directions, up to three times.
+  // Coriolis pencil: W <- Op^2 W for the selected slots.  A random block has
+  // O(1) content in the stiff modes, which the L1 residuals amplify by
+  // |omega_max / omega|; two applications of Op damp it by (omega / omega_max)^2
+  // before it enters the basis.
+  void smooth_block(const std::vector<int>& sel) {
+    std::vector<int> pbv(B_, 0);
+    for (int b = 0; b < B_; ++b) pbv[b] = sel[b] ? pb_ : 0;
+    DevBuf dsel = ints(sel), dpb = ints(pbv);
+    BatchBufs bb = bm_.bufs(P_, B_);
+    const dim3 g((n_ + 255) / 256, B_);
+    for (int rep = 0; rep < 2; ++rep) {
+      spmv_cols(mk_ptrs_, Wb_.as<double2>(), ldw_, dpb, pb_, MWb_.as<double2>());  // M a
+      {
+        KTimer t(ctx_, "vec", 0, 0);
+        hipLaunchKernelGGL(k_axpy_half, g, dim3(256), 0, stream(), Wb_.as<double2>(), ldw_, (const int*)nullptr, n_,
+                           Wb_.as<double2>(), ldw_, (const int*)nullptr, 0LL, pb_, n_, nvec_, 1.0, 0.0,
+                           dsel.as<int>());  // a <- b
+        hipLaunchKernelGGL(k_axpy_half, g, dim3(256), 0, stream(), MWb_.as<double2>(), ldw_, (const int*)nullptr, 0LL,
+                           Wb_.as<double2>(), ldw_, (const int*)nullptr, n_, pb_, n_, nvec_, 1.0, 0.0,
+                           dsel.as<int>());  // b <- M a
+      }
+      spmv_cols(dptr_, Wb_.as<double2>(), ldw_, dpb, pb_, Wb_.as<double2>() + n_, true);  // += (-D) b
+      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>() + n_, ldw_, dsel.as<int>(), pb_, nvec_);
+    }
+  }
+
   void append_or_retry(std::vector<int>& act) {
     std::vector<int> bad = append_block(act);
     for (int attempt = 1; attempt <= 3 && !bad.empty(); ++attempt) {
@@ -1723,6 +1967,7 @@ class BatchLanczos {
         salt[b] = 1000 * probs_[b]->steps + attempt;
       }
       random_block(sub, salt);
+      if (qep_) smooth_block(sub);
       bad = append_block(sub);
     }
     for (int b : bad) fail_block(b, act);
@@ -1737,7 +1982,10 @@ class BatchLanczos {
 
   // X[:, c0:c1] = V S[:, c0:c1] for the problems in sel; K X and M X by fresh
   // products (no drift of K V, M V across restarts)
-  void ritz_vectors(const std::vector<int>& sel, const std::vector<std::vector<double>>& Sv, int c0, int c1) {
+  // For the Coriolis problem with mu given (mu = 1 / omega per column), the a
+  // half is set to omega b: z = [omega x; x] with x = b, the vector delivered.
+  void ritz_vectors(const std::vector<int>& sel, const std::vector<std::vector<double>>& Sv, int c0, int c1,
+                    const std::vector<std::vector<double>>* mu = nullptr) {
     std::vector<int> nvc(B_, 0);
     for (int b = 0; b < B_; ++b)
       if (sel[b]) nvc[b] = nv_[b];
@@ -1747,22 +1995,33 @@ class BatchLanczos {
       std::vector<double2> C((size_t)B_ * mmax_ * 16, make_double2(0.0, 0.0));
       for (int b = 0; b < B_; ++b)
         if (sel[b])
-          for (int a = 0; a < nv_[b]; ++a)
-            for (int c = 0; c < kc && q0 + c < nv_[b]; ++c) {
-              const size_t s = (size_t)a * nv_[b] + q0 + c;
+          for (int a = 0; a < nvc[b]; ++a)
+            for (int c = 0; c < kc && q0 + c < nvc[b]; ++c) {
+              const size_t s = (size_t)a * nvc[b] + q0 + c;
               C[((size_t)b * mmax_ + a) * 16 + c] = make_double2(Sv[b][2 * s], Sv[b][2 * s + 1]);
             }
       std::vector<int> off(B_, q0);
       DevBuf doff = ints(off);
       tsmm(Vb_.as<double2>(), ldv_, dnv, C, kc, 1.0, 0.0, Xb_.as<double2>(), ldk_, &doff, dsel);
+      if (qep_ && mu) {
+        for (int b = 0; b < B_; ++b)
+          if (sel[b])
+            for (int a = 0; a < nvc[b]; ++a)
+              for (int c = 0; c < kc && q0 + c < nvc[b]; ++c) {
+                const double m = (*mu)[b][q0 + c], w = m != 0.0 ? 1.0 / m : 0.0;
+                double2& x = C[((size_t)b * mmax_ + a) * 16 + c];
+                x = make_double2(w * x.x, w * x.y);
+              }
+        tsmm(Vb_.as<double2>() + n_, ldv_, dnv, C, kc, 1.0, 0.0, Xb_.as<double2>(), ldk_, &doff, dsel, n_);
+      }
     }
     std::vector<int> ncol(B_, 0);
     for (int b = 0; b < B_; ++b)
       if (sel[b]) ncol[b] = c1 - c0;
     DevBuf dn = ints(ncol);
-    const long long o = (long long)c0 * n_;
-    spmv_cols(kptr_, Xb_.as<double2>() + o, ldk_, dn, c1 - c0, KXb_.as<double2>() + o);
-    spmv_cols(mk_ptrs_, Xb_.as<double2>() + o, ldk_, dn, c1 - c0, MXb_.as<double2>() + o);
+    const long long o = (long long)c0 * nvec_;
+    apply_A(Xb_.as<double2>() + o, ldk_, dn, c1 - c0, KXb_.as<double2>() + o);
+    apply_B(Xb_.as<double2>() + o, ldk_, dn, c1 - c0, MXb_.as<double2>() + o);
   }
 
   // Rayleigh-Ritz on span(V) for the problems near a full basis or restarted;
@@ -1786,6 +2045,8 @@ class BatchLanczos {
       HostTimer h(ctx_, "rr_eig");
       parallel_for(B_, [&](int b) {
         if (!rr[b]) return;
+        // (V^H A V, V^H B V): (K, M), or the Coriolis definite pencil (Bc, L2)
+        // whose eigenvalues are mu = 1 / omega
         const int m = nv_[b];
         std::vector<double> H(2 * (size_t)m * m), G(2 * (size_t)m * m);
         for (int i = 0; i < m; ++i)
@@ -1797,6 +2058,22 @@ class BatchLanczos {
             G[2 * ((size_t)i * m + j) + 1] = y.imag();
           }
         ok[b] = gen_eig_chol(m, H, G, lam[b], Sv[b]) ? 1 : 0;
+        if (ok[b] && qep_) {  // the pencil's wanted pairs: largest |mu| = smallest |omega| first
+          std::vector<int> idx(m);
+          std::iota(idx.begin(), idx.end(), 0);
+          std::stable_sort(idx.begin(), idx.end(),
+                           [&](int x, int y) { return std::fabs(lam[b][x]) > std::fabs(lam[b][y]); });
+          std::vector<double> l2(m), s2(Sv[b].size());
+          for (int q = 0; q < m; ++q) {
+            l2[q] = lam[b][idx[q]];
+            for (int a = 0; a < m; ++a) {
+              s2[2 * ((size_t)a * m + q)] = Sv[b][2 * ((size_t)a * m + idx[q])];
+              s2[2 * ((size_t)a * m + q) + 1] = Sv[b][2 * ((size_t)a * m + idx[q]) + 1];
+            }
+          }
+          lam[b].swap(l2);
+          Sv[b].swap(s2);
+        }
       });
     }
     for (int b = 0; b < B_; ++b)
@@ -1813,15 +2090,23 @@ class BatchLanczos {
         ncmax = std::max(ncmax, ncheck[b]);
       }
     if (ncmax == 0) return;
-    ritz_vectors(rr, Sv, 0, ncmax);
+    ritz_vectors(rr, Sv, 0, ncmax, &lam);
+    std::vector<std::vector<double>> theta(B_);
+    for (int b = 0; b < B_; ++b)
+      if (rr[b]) theta[b].assign(lam[b].begin(), lam[b].begin() + ncheck[b]);
     std::vector<double> lamv((size_t)B_ * nkeep_, 0.0);
     for (int b = 0; b < B_; ++b)
-      for (int c = 0; c < ncheck[b]; ++c) lamv[(size_t)b * nkeep_ + c] = lam[b][c];
+      for (int c = 0; c < ncheck[b]; ++c) lamv[(size_t)b * nkeep_ + c] = qep_ ? 1.0 / theta[b][c] : theta[b][c];
     DevBuf dl = upload(pool(), lamv, stream()), dnc = ints(ncheck);
     DevBuf part(pool(), sizeof(double) * 3 * (size_t)B_ * nkeep_ * nchunk_);
-    hipLaunchKernelGGL(k_residual_partial, dim3(nchunk_, B_, ncmax), dim3(256), 0, stream(), Xb_.as<double2>(),
-                       KXb_.as<double2>(), MXb_.as<double2>(), ldk_, n_, dl.as<double>(), nkeep_, dnc.as<int>(),
-                       part.as<double>(), nchunk_);
+    if (qep_)  // ||K x + omega D x - omega^2 M x|| / ||K x|| = ||(L2 z)_b - omega (Bc z)_b|| / ||(L2 z)_b||
+      hipLaunchKernelGGL(k_residual_partial, dim3(nchunk_, B_, ncmax), dim3(256), 0, stream(),
+                         Xb_.as<double2>() + n_, MXb_.as<double2>() + n_, KXb_.as<double2>() + n_, ldk_, n_, nvec_,
+                         dl.as<double>(), nkeep_, dnc.as<int>(), part.as<double>(), nchunk_);
+    else
+      hipLaunchKernelGGL(k_residual_partial, dim3(nchunk_, B_, ncmax), dim3(256), 0, stream(), Xb_.as<double2>(),
+                         KXb_.as<double2>(), MXb_.as<double2>(), ldk_, nvec_, nvec_, dl.as<double>(), nkeep_,
+                         dnc.as<int>(), part.as<double>(), nchunk_);
     std::vector<double> hpart(3 * (size_t)B_ * nkeep_ * nchunk_);
     TM_HIP(hipMemcpyAsync(hpart.data(), part.p, sizeof(double) * hpart.size(), hipMemcpyDeviceToHost, stream()));
     TM_HIP(hipStreamSynchronize(stream()));
@@ -1856,17 +2141,24 @@ class BatchLanczos {
         for (int b = 0; b < B_; ++b) nc2[b] = std::max(0, std::min(pb_, need_floor[b] - c0));
         DevBuf dnc2 = ints(nc2);
         const int cm = std::min(pb_, nfmax - c0);
-        // |K||x| and |M||x| into the (consumed) W, MW buffers, pb columns at a time
-        hipLaunchKernelGGL(k_spmv_abs, dim3((3 * S_.nf + 255) / 256, B_, cm), dim3(256), 0, stream(), S_.nf,
-                           M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), kptr_.as<const double2*>(),
-                           Xb_.as<double2>() + (long long)c0 * n_, ldk_, n_, dnc2.as<int>(), Wb_.as<double2>(), ldw_);
-        hipLaunchKernelGGL(k_spmv_abs, dim3((3 * S_.nf + 255) / 256, B_, cm), dim3(256), 0, stream(), S_.nf,
-                           M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), mk_ptrs_.as<const double2*>(),
-                           Xb_.as<double2>() + (long long)c0 * n_, ldk_, n_, dnc2.as<int>(), MWb_.as<double2>(),
-                           ldw_);
+        // |K||x|, |M||x| (and |D||x|) into the (consumed) W, MW (KW) buffers, pb
+        // columns at a time; x is the b half for the Coriolis problem
+        const double2* xs = Xb_.as<double2>() + (qep_ ? n_ : 0LL) + (long long)c0 * nvec_;
+        const dim3 ga((3 * S_.nf + 255) / 256, B_, cm);
+        hipLaunchKernelGGL(k_spmv_abs, ga, dim3(256), 0, stream(), S_.nf, M_.d_rowptr.as<int>(),
+                           M_.d_colidx.as<int>(), kptr_.as<const double2*>(), xs, ldk_, nvec_, dnc2.as<int>(),
+                           Wb_.as<double2>(), ldw_);
+        hipLaunchKernelGGL(k_spmv_abs, ga, dim3(256), 0, stream(), S_.nf, M_.d_rowptr.as<int>(),
+                           M_.d_colidx.as<int>(), mk_ptrs_.as<const double2*>(), xs, ldk_, nvec_, dnc2.as<int>(),
+                           MWb_.as<double2>(), ldw_);
+        if (qep_)
+          hipLaunchKernelGGL(k_spmv_abs, ga, dim3(256), 0, stream(), S_.nf, M_.d_rowptr.as<int>(),
+                             M_.d_colidx.as<int>(), dptr_.as<const double2*>(), xs, ldk_, nvec_, dnc2.as<int>(),
+                             KWb_.as<double2>(), ldw_);
         hipLaunchKernelGGL(k_floor_partial, dim3(nchunk_, B_, cm), dim3(256), 0, stream(), Wb_.as<double2>(),
-                           MWb_.as<double2>(), ldw_, n_, dl.as<double>() + c0, nkeep_, dnc2.as<int>(),
-                           fpart.as<double>() + (size_t)c0 * nchunk_, nchunk_);
+                           qep_ ? KWb_.as<double2>() : (const double2*)nullptr, MWb_.as<double2>(), ldw_, n_, nvec_,
+                           dl.as<double>() + c0, nkeep_, dnc2.as<int>(), fpart.as<double>() + (size_t)c0 * nchunk_,
+                           nchunk_);
       }
       std::vector<double> fl((size_t)B_ * nkeep_ * nchunk_);
       TM_HIP(hipMemcpyAsync(fl.data(), fpart.p, sizeof(double) * fl.size(), hipMemcpyDeviceToHost, stream()));
@@ -1890,13 +2182,15 @@ class BatchLanczos {
         bool good = r <= std::max(kBlockTol, 4.0 * fl);
         // stagnated (no 2x gain since the previous check) at the rounding level of
         // the basis: accept; the extraction refines what is left above 1e-8 with
-        // fresh triangular solves
-        if (!good && r <= kBlockStallMax && c < (int)prev_res_[b].size() && r > 0.5 * prev_res_[b][c])
+        // fresh triangular solves.  The Coriolis problem has no such refinement:
+        // it accepts a stall only below kQepStallMax.
+        if (!good && r <= (qep_ ? kQepStallMax : kBlockStallMax) && c < (int)prev_res_[b].size() &&
+            r > 0.5 * prev_res_[b][c])
           good = true;
         if (!good && first_bad < 0) first_bad = c;
         if (trace_)
           std::fprintf(stderr, "[tmhip] k=%d rpm=%.0f nv=%d rst=%d c=%d lam=%.10e res=%.2e floor=%.2e\n", hp->k,
-                       hp->in.rpm, nv_[b], hp->restarts, c, lam[b][c], r, fl);
+                       hp->in.rpm, nv_[b], hp->restarts, c, theta[b][c], r, fl);
       }
       prev_res_[b] = res[b];
       const bool converged = first_bad < 0 && ncheck[b] >= hp->nev;
@@ -1904,7 +2198,7 @@ class BatchLanczos {
         hp->converged = converged;
         hp->done = true;
         hp->n_extract = ncheck[b];
-        hp->theta_sorted_lambda.assign(lam[b].begin(), lam[b].begin() + ncheck[b]);
+        hp->theta_sorted_lambda = theta[b];
         act[b] = 0;
         continue;
       }
@@ -1928,12 +2222,12 @@ class BatchLanczos {
     {
       DevBuf dr = ints(restart);
       KTimer t(ctx_, "vec", 0, 0);
-      hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), Xb_.as<double2>(), ldk_,
-                         (const int*)nullptr, Vb_.as<double2>(), ldv_, (const int*)nullptr, nkeep_, n_, dr.as<int>());
-      hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), KXb_.as<double2>(), ldk_,
-                         (const int*)nullptr, KVb_.as<double2>(), ldv_, (const int*)nullptr, nkeep_, n_, dr.as<int>());
-      hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), MXb_.as<double2>(), ldk_,
-                         (const int*)nullptr, MVb_.as<double2>(), ldv_, (const int*)nullptr, nkeep_, n_, dr.as<int>());
+      hipLaunchKernelGGL(k_copy_cols, dim3((nvec_ + 255) / 256, B_), dim3(256), 0, stream(), Xb_.as<double2>(), ldk_,
+                         (const int*)nullptr, Vb_.as<double2>(), ldv_, (const int*)nullptr, nkeep_, nvec_, dr.as<int>());
+      hipLaunchKernelGGL(k_copy_cols, dim3((nvec_ + 255) / 256, B_), dim3(256), 0, stream(), KXb_.as<double2>(), ldk_,
+                         (const int*)nullptr, KVb_.as<double2>(), ldv_, (const int*)nullptr, nkeep_, nvec_, dr.as<int>());
+      hipLaunchKernelGGL(k_copy_cols, dim3((nvec_ + 255) / 256, B_), dim3(256), 0, stream(), MXb_.as<double2>(), ldk_,
+                         (const int*)nullptr, MVb_.as<double2>(), ldv_, (const int*)nullptr, nkeep_, nvec_, dr.as<int>());
     }
     // projected matrices of the kept block, recomputed: X^H K X and X^H M X
     std::vector<int> nk(B_, 0);
@@ -1942,9 +2236,9 @@ class BatchLanczos {
     DevBuf dr = ints(restart), dnk = ints(nk);
     for (int c0 = 0; c0 < nkeep_; c0 += 16) {
       const int kb = std::min(16, nkeep_ - c0);
-      std::vector<double2> hk = tsgemm(Vb_.as<double2>(), ldv_, dnk, nkeep_, KVb_.as<double2>() + (long long)c0 * n_,
+      std::vector<double2> hk = tsgemm(Vb_.as<double2>(), ldv_, dnk, nkeep_, KVb_.as<double2>() + (long long)c0 * nvec_,
                                        ldv_, kb, dr);
-      std::vector<double2> hm = tsgemm(Vb_.as<double2>(), ldv_, dnk, nkeep_, MVb_.as<double2>() + (long long)c0 * n_,
+      std::vector<double2> hm = tsgemm(Vb_.as<double2>(), ldv_, dnk, nkeep_, MVb_.as<double2>() + (long long)c0 * nvec_,
                                        ldv_, kb, dr);
       for (int b = 0; b < B_; ++b)
         if (restart[b])
@@ -1973,34 +2267,48 @@ class BatchLanczos {
 // run the single-vector Lanczos.
 static constexpr int kBlockMinFree = 256;
 
-static bool use_block(const Symbolic& S, const tm_problem& p) {
-  if (p.method != TM_METHOD_BLOCK || p.conv_policy == TM_CONV_REFERENCE || S.n_free_dofs < kBlockMinFree)
-    return false;
+static bool is_qep(const Symbolic& S, const tm_problem& p) {
   HostProblem hp;
   hp.in = p;
   fill_policy(S, hp);
+  return hp.qep;
+}
+
+// the Coriolis pencil always runs the block method (2n >= 128 vectors' room)
+static constexpr int kQepMinFree = 64;
+
+static bool use_block(const Symbolic& S, const tm_problem& p) {
+  HostProblem hp;
+  hp.in = p;
+  fill_policy(S, hp);
+  if (hp.qep) {
+    if (S.n_free_dofs < kQepMinFree) throw ArgError("the Coriolis QEP needs at least 64 free DOFs");
+    if (hp.nev > 24) throw ArgError("the Coriolis QEP is limited to 22 modes per harmonic");
+    return true;
+  }
+  if (p.method != TM_METHOD_BLOCK || p.conv_policy == TM_CONV_REFERENCE || S.n_free_dofs < kBlockMinFree)
+    return false;
   return hp.nev <= 24;  // the kept Ritz block (32 vectors) must cover nev with room to spare
 }
 
-int64_t problem_footprint(const Operator& op, int num_modes, int harmonic, int method) {
+static int64_t problem_footprint(const Operator& op, const tm_problem& p) {
   const Symbolic& S = op.mesh->S;
   const Plan& P = op.mesh->plan;
   HostProblem hp;
-  hp.in = tm_problem{};
-  hp.in.harmonic = harmonic;
-  hp.in.num_modes = num_modes;
-  hp.in.method = method;
+  hp.in = p;
   fill_policy(S, hp);
   const int64_t n = 3LL * S.nf;
   const bool block = use_block(S, hp.in);
-  const int pb = hp.nev <= 6 ? 8 : 16;
+  const int pb = (hp.nev <= 6 && !hp.qep) ? 8 : 16;
   const int pcols = std::min(16, std::max(block ? pb : 1, hp.nev));
   int64_t bytes = 16 * (P.panel_entries + P.upool_entries + pcols * P.tvec_entries + P.inv_scratch_entries);
   if (block) {
     const int mmax = pb == 8 ? 48 : 64;
-    bytes += 3 * 16 * (int64_t)mmax * n;             // V, MV, KV
-    bytes += 3 * 16 * (int64_t)pb * n;               // W, MW, KW
-    bytes += 4 * 16 * (int64_t)(mmax / 2) * n;       // Ritz X, KX, MX, refinement
+    const int64_t nv = hp.qep ? 2 * n : n;           // pencil vectors are 2n long
+    bytes += 3 * 16 * (int64_t)mmax * nv;            // V, MV, KV
+    bytes += 3 * 16 * (int64_t)pb * nv;              // W, MW, KW
+    bytes += 4 * 16 * (int64_t)(mmax / 2) * nv;      // Ritz X, KX, MX, refinement
+    if (hp.qep) bytes += 16 * 9 * (int64_t)op.nblk;  // D(k)
   } else {
     bytes += 2 * 16 * (int64_t)(hp.ncv + 1) * n;     // V, MV
     bytes += 2 * 16 * n;                             // W, MW
@@ -2008,6 +2316,14 @@ int64_t problem_footprint(const Operator& op, int num_modes, int harmonic, int m
   }
   bytes += 2 * 16 * 9 * (int64_t)op.nblk;            // K(k), M(k)
   return bytes;
+}
+
+int64_t problem_footprint(const Operator& op, int num_modes, int harmonic, int method) {
+  tm_problem p{};
+  p.harmonic = harmonic;
+  p.num_modes = num_modes;
+  p.method = method;
+  return problem_footprint(op, p);
 }
 
 void solve_problems(Context* ctx, const tm_problem* problems, int n, tm_result* results) {
@@ -2031,7 +2347,8 @@ void solve_problems(Context* ctx, const tm_problem* problems, int n, tm_result* 
     if (problems[i].harmonic < 0 || problems[i].harmonic > max_k)
       throw ArgError("harmonic index out of range 0..N/2");
     const bool blk = use_block(op->mesh->S, problems[i]);
-    const auto key = std::make_pair(blk ? (const void*)op->mesh : (const void*)op, blk ? 1 : 0);
+    const bool qep = blk && is_qep(op->mesh->S, problems[i]);
+    const auto key = std::make_pair(blk ? (const void*)op->mesh : (const void*)op, blk ? (qep ? 2 : 1) : 0);
     if (!groups.count(key)) {
       order.push_back(key);
       rep[key] = op;
@@ -2044,7 +2361,7 @@ void solve_problems(Context* ctx, const tm_problem* problems, int n, tm_result* 
     int64_t per = 0;
     for (auto* hp : list) {
       fill_policy(op->mesh->S, *hp);
-      per = std::max<int64_t>(per, problem_footprint(*op, hp->in.num_modes, hp->k, hp->in.method));
+      per = std::max<int64_t>(per, problem_footprint(*op, hp->in));
     }
     std::stable_sort(list.begin(), list.end(), [](HostProblem* a, HostProblem* b) { return a->nev < b->nev; });
     size_t free_b = 0, total_b = 0;
@@ -2056,12 +2373,13 @@ void solve_problems(Context* ctx, const tm_problem* problems, int n, tm_result* 
       // one block size per slot group (Lanczos: lockstep batches)
       size_t j = i;
       const int pb_i = list[i]->nev <= 6 ? 8 : 16;
-      if (key.second == 1) {
-        while (j < list.size() && (list[j]->nev <= 6 ? 8 : 16) == pb_i) ++j;
+      if (key.second >= 1) {
+        const bool qep = key.second == 2;
+        while (j < list.size() && (qep || (list[j]->nev <= 6 ? 8 : 16) == pb_i)) ++j;
         std::vector<HostProblem*> queue(list.begin() + i, list.begin() + j);
         BatchLanczos bl(ctx, *op, {}, true);
         const int cap = static_cast<int>(std::min(max_batch, queue.size()));
-        bl.run_slots(queue, cap, cap);
+        bl.run_slots(queue, cap, cap, qep);
       } else {
         while (j < list.size() && j - i < max_batch) ++j;
         std::vector<HostProblem*> chunk(list.begin() + i, list.begin() + j);
@@ -2120,6 +2438,7 @@ struct Server {
     const Mesh* mesh = nullptr;
     Operator* rep = nullptr;  // any operator of the mesh (plan, pattern)
     int pbclass = 8;
+    bool qep = false;
     std::unique_ptr<SubContext> sub;
     std::unique_ptr<BatchLanczos> bl;
     std::deque<HostProblem*> queue;  // guarded by srv->mu
@@ -2140,7 +2459,7 @@ struct Server {
   std::map<uint64_t, std::unique_ptr<Req>> reqs;
   std::map<HostProblem*, uint64_t> ticket_of;
   uint64_t next_ticket = 1;
-  std::map<std::pair<const Mesh*, int>, std::unique_ptr<Group>> groups;
+  std::map<std::pair<const Mesh*, int>, std::unique_ptr<Group>> groups;  // (mesh, block size + 1000 qep)
 
   Server(Context* c, int slots, double f) : ctx(c), max_slots(std::max(1, std::min(64, slots))), frac(f) {
     if (!(frac > 0.0 && frac <= 0.95)) frac = 0.85;
@@ -2243,7 +2562,7 @@ struct Server {
           }
           if (!g->bl) {
             for (const HostProblem* h : g->queue)
-              per = std::max<int64_t>(per, problem_footprint(*g->rep, h->in.num_modes, h->k, TM_METHOD_BLOCK));
+              per = std::max<int64_t>(per, problem_footprint(*g->rep, h->in));
             per = std::max<int64_t>(per, 1) * 21 / 20;
             const double avail = std::min(budget - reserved, 0.25 * budget);
             const int cap = (int)std::min<double>(max_slots, std::floor(avail / (double)per));
@@ -2259,7 +2578,7 @@ struct Server {
         }
         if (need_setup) {
           g->bl = std::make_unique<BatchLanczos>(&g->sub->c, *g->rep, std::vector<HostProblem*>{}, true);
-          g->bl->slot_setup((int)per, g->pbclass == 8 ? 6 : 24);
+          g->bl->slot_setup((int)per, g->pbclass == 8 ? 6 : 24, g->qep);
         }
         {
           std::lock_guard<std::mutex> lk(mu);
@@ -2308,15 +2627,16 @@ struct Server {
             small[op].push_back(hp);
             continue;
           }
-          const int pbc = hp->nev <= 6 ? 8 : 16;
+          const int pbc = (hp->nev <= 6 && !hp->qep) ? 8 : 16;
           std::lock_guard<std::mutex> g(mu);
-          auto& gp = groups[{op->mesh, pbc}];
+          auto& gp = groups[{op->mesh, pbc + (hp->qep ? 1000 : 0)}];
           if (!gp) {
             gp = std::make_unique<Group>();
             gp->srv = this;
             gp->mesh = op->mesh;
             gp->rep = op;
             gp->pbclass = pbc;
+            gp->qep = hp->qep;
             gp->sub = std::make_unique<SubContext>(ctx->device);
             Group* raw = gp.get();
             gp->th = std::thread([this, raw] { run_group(raw); });

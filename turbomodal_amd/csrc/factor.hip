@@ -749,7 +749,7 @@ static void trisolve_levels(Context* ctx, const Plan& P, const SolveArgs& sa, in
 }
 
 void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs& bb, double2* X,
-                    long long ldx, const int* active, int p) {
+                    long long ldx, const int* active, int p, long long colstride) {
   if (p < 1 || p > PMAX) throw ArgError("trisolve: 1 <= p <= 16 right-hand sides");
   SolveArgs sa;
   sa.sn = P.d_sn.as<DevSn>();
@@ -761,7 +761,7 @@ void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs&
   sa.TE = P.tvec_entries;
   sa.X = X;
   sa.ldx = ldx;
-  sa.n = 3LL * M.S.nf;
+  sa.n = colstride > 0 ? colstride : 3LL * M.S.nf;  // column stride of X
   sa.active = active;
   sa.p = p;
   const int B = bb.B;

@@ -143,7 +143,7 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
 // in-place solve A X = B for each problem's block X_b = X + b*ldx: p columns of
 // length n = 3 nf, column c at X_b + c*n.  bb.T must hold p * tvec_entries per problem.
 void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs& bb, double2* X,
-                    long long ldx, const int* active, int p = 1);
+                    long long ldx, const int* active, int p = 1, long long colstride = -1);
 
 struct HostProblem;
 void create_operator(Context* ctx, Mesh& M, Operator& op);

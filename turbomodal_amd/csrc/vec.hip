@@ -249,7 +249,8 @@ __global__ void k_build_phase_blocks(int nf, const int* __restrict__ rowptr,
                                      const unsigned char* __restrict__ pinned,
                                      const double* __restrict__ Xc, const double* __restrict__ Xp,
                                      const double* __restrict__ dXc, const double* __restrict__ dXp,
-                                     double s, double pre, double pim, double2* __restrict__ out) {
+                                     double s, double pre, double pim, double2* __restrict__ out,
+                                     int skew, double2 scale) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= nf) return;
   for (int sl = rowptr[I]; sl < rowptr[I + 1]; ++sl) {
@@ -264,7 +265,9 @@ __global__ void k_build_phase_blocks(int nf, const int* __restrict__ rowptr,
           p += s * dXp[e1];
           q += s * dXp[e2];
         }
-        double2 v = make_double2(c + pre * (p + q), pim * (p - q));
+        // symmetric X: Xc + P Xp + conj(P) Xp^T; skew X: Xc + P Xp - conj(P) Xp^T
+        double2 v = skew ? make_double2(c + pre * (p - q), pim * (p + q)) : make_double2(c + pre * (p + q), pim * (p - q));
+        v = cmul(scale, v);
         if (pinned[3 * I + ci] || pinned[3 * J + cj]) v = make_double2(0.0, 0.0);
         out[(long long)sl * 9 + 3 * ci + cj] = v;
       }
@@ -300,7 +303,7 @@ __global__ void k_spmv_multi(int nf, const int* __restrict__ rowptr, const int* 
 __global__ __launch_bounds__(256) void k_residual_partial(const double2* __restrict__ X,
                                                           const double2* __restrict__ KX,
                                                           const double2* __restrict__ MX,
-                                                          long long ldb, long long n,
+                                                          long long ldb, long long n, long long cs,
                                                           const double* __restrict__ lam, int lam_ld,
                                                           const int* __restrict__ ncol,
                                                           double* __restrict__ part, int nchunk) {
@@ -311,7 +314,7 @@ __global__ __launch_bounds__(256) void k_residual_partial(const double2* __restr
   const double l = lam[(long long)b * lam_ld + c];
   double num = 0.0, den = 0.0, xm = 0.0;
   for (long long i = i0 + threadIdx.x; i < i0 + kDotChunk && i < n; i += 256) {
-    const long long o = b * ldb + (long long)c * n + i;
+    const long long o = b * ldb + (long long)c * cs + i;
     double2 k = KX[o], m = MX[o], x = X[o];
     double rx = k.x - l * m.x, ry = k.y - l * m.y;
     num += rx * rx + ry * ry;
@@ -419,7 +422,7 @@ __global__ void k_flag_cols(const int* __restrict__ ncol, int q, int B, int* __r
 // block owns 16 columns of A and all kb columns of B for one chunk of rows.
 __global__ __launch_bounds__(256) void k_tsgemm_partial(const double2* __restrict__ A, long long lda_b,
                                                         const double2* __restrict__ Bm, long long ldb_b,
-                                                        long long n, const int* __restrict__ ka_b, int kb,
+                                                        long long n, long long cs, const int* __restrict__ ka_b, int kb,
                                                         const int* __restrict__ act, int kamax,
                                                         double2* __restrict__ part, int nchunk) {
   const int b = blockIdx.y;
@@ -442,8 +445,8 @@ __global__ __launch_bounds__(256) void k_tsgemm_partial(const double2* __restric
     __syncthreads();
     for (int idx = tid; idx < 64 * 16; idx += 256) {
       const int r = idx % 64, c = idx / 64;
-      As[r][c] = (r < nr && c < na) ? Ab[(long long)(a0 + c) * n + rr + r] : make_double2(0.0, 0.0);
-      Bs[r][c] = (r < nr && c < kb) ? Bb[(long long)c * n + rr + r] : make_double2(0.0, 0.0);
+      As[r][c] = (r < nr && c < na) ? Ab[(long long)(a0 + c) * cs + rr + r] : make_double2(0.0, 0.0);
+      Bs[r][c] = (r < nr && c < kb) ? Bb[(long long)c * cs + rr + r] : make_double2(0.0, 0.0);
     }
     __syncthreads();
     for (int r = 0; r < nr; ++r) {
@@ -476,7 +479,7 @@ __global__ void k_tsgemm_reduce(const double2* __restrict__ part, const int* __r
 // Y_b[:, c] = beta Y_b[:, c] + sum_{a < ka_b} A_b[:, a] C_b[a][c], c < kc
 // (C_b row-major kamax x 16); one thread per row
 __global__ __launch_bounds__(256) void k_tsmm(const double2* __restrict__ A, long long lda_b, long long n,
-                                              const double2* __restrict__ C, const int* __restrict__ ka_b,
+                                              long long cs, const double2* __restrict__ C, const int* __restrict__ ka_b,
                                               int kc, const int* __restrict__ act, int kamax, double alpha,
                                               double beta, double2* __restrict__ Y, long long ldy_b,
                                               const int* __restrict__ yoff) {
@@ -493,16 +496,16 @@ __global__ __launch_bounds__(256) void k_tsmm(const double2* __restrict__ A, lon
 #pragma unroll
   for (int c = 0; c < 16; ++c) acc[c] = make_double2(0.0, 0.0);
   for (int a = 0; a < ka; ++a) {
-    const double2 x = Ab[(long long)a * n + i];
+    const double2 x = Ab[(long long)a * cs + i];
 #pragma unroll
     for (int c = 0; c < 16; ++c) acc[c] = cadd(acc[c], cmul(x, Cs[a * 16 + c]));
   }
-  double2* Yb = Y + b * ldy_b + (yoff ? (long long)yoff[b] * n : 0LL);
+  double2* Yb = Y + b * ldy_b + (yoff ? (long long)yoff[b] * cs : 0LL);
 #pragma unroll
   for (int c = 0; c < 16; ++c)
     if (c < kc) {
-      const double2 y = beta != 0.0 ? Yb[(long long)c * n + i] : make_double2(0.0, 0.0);
-      Yb[(long long)c * n + i] = make_double2(beta * y.x + alpha * acc[c].x, beta * y.y + alpha * acc[c].y);
+      const double2 y = beta != 0.0 ? Yb[(long long)c * cs + i] : make_double2(0.0, 0.0);
+      Yb[(long long)c * cs + i] = make_double2(beta * y.x + alpha * acc[c].x, beta * y.y + alpha * acc[c].y);
     }
 }
 
@@ -515,7 +518,7 @@ __global__ __launch_bounds__(256) void k_spmv_cols(int nf, const int* __restrict
                                                    const double2* const* __restrict__ blocks,
                                                    const double2* __restrict__ X, long long ldx, long long n,
                                                    const int* __restrict__ ncol, double2* __restrict__ Y,
-                                                   long long ldy) {
+                                                   long long ldy, int accumulate) {
   const int b = blockIdx.y;
   const int nc = ncol[b];
   if (nc <= 0) return;
@@ -550,10 +553,50 @@ __global__ __launch_bounds__(256) void k_spmv_cols(int nf, const int* __restrict
   for (int c = 0; c < PB; ++c)
     if (c < nc)
 #pragma unroll
-      for (int rr = 0; rr < 3; ++rr) y[(long long)c * n + rr] = acc[rr][c];
+      for (int rr = 0; rr < 3; ++rr) {
+        double2* yy = y + (long long)c * n + rr;
+        *yy = accumulate ? cadd(*yy, acc[rr][c]) : acc[rr][c];
+      }
 }
 template __global__ void k_spmv_cols<8>(int, const int*, const int*, const double2* const*, const double2*,
-                                        long long, long long, const int*, double2*, long long);
+                                        long long, long long, const int*, double2*, long long, int);
+
+// dst_b[doff + i, d0_b + c] = alpha src_b[soff + i, s0_b + c] + beta dst (i < len, c < ncol),
+// columns `stride` apart: halves of the 2n-long vectors of the Coriolis pencil
+__global__ void k_axpy_half(const double2* __restrict__ src, long long lds, const int* __restrict__ s0v,
+                            long long soff, double2* __restrict__ dst, long long ldd, const int* __restrict__ d0v,
+                            long long doff, int ncol, long long len, long long stride, double alpha, double beta,
+                            const int* __restrict__ act) {
+  const int b = blockIdx.y;
+  if (act && !act[b]) return;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  const int s0 = s0v ? s0v[b] : 0, d0 = d0v ? d0v[b] : 0;
+  for (int c = 0; c < ncol; ++c) {
+    const double2 x = src[b * lds + (long long)(s0 + c) * stride + soff + i];
+    double2* y = dst + b * ldd + (long long)(d0 + c) * stride + doff + i;
+    const double2 yo = beta != 0.0 ? *y : make_double2(0.0, 0.0);
+    *y = make_double2(alpha * x.x + beta * yo.x, alpha * x.y + beta * yo.y);
+  }
+}
+
+// per column: Re(x_head^H y_head) over the first len entries (chunk partials), columns stride apart
+__global__ __launch_bounds__(256) void k_head_dot(const double2* __restrict__ X, const double2* __restrict__ Y,
+                                                  long long ldb, long long stride, long long len,
+                                                  const int* __restrict__ ncol, int ldc, double* __restrict__ part,
+                                                  int nchunk) {
+  const int b = blockIdx.y, c = blockIdx.z;
+  if (c >= ncol[b]) return;
+  __shared__ double sh[8];
+  const long long i0 = (long long)blockIdx.x * kDotChunk;
+  double s = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i0 + kDotChunk && i < len; i += 256) {
+    const long long o = b * ldb + (long long)c * stride + i;
+    s += X[o].x * Y[o].x + X[o].y * Y[o].y;
+  }
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) part[((long long)b * ldc + c) * nchunk + blockIdx.x] = t;
+}
 
 // dst_b[:, d0_b + c] = src_b[:, s0_b + c] for c < ncol (problems with act_b); the
 // per-problem column offsets s0v / d0v are optional (else 0)
@@ -571,7 +614,7 @@ __global__ void k_copy_cols(const double2* __restrict__ src, long long lds, cons
 // Y_b[:, c] = |A_b| |X_b[:, c]| (entrywise moduli; real result in .x) for c < ncol_b
 __global__ void k_spmv_abs(int nf, const int* __restrict__ rowptr, const int* __restrict__ colidx,
                            const double2* const* __restrict__ blocks, const double2* __restrict__ X,
-                           long long ldx, long long n, const int* __restrict__ ncol, double2* __restrict__ Y,
+                           long long ldx, long long cs, const int* __restrict__ ncol, double2* __restrict__ Y,
                            long long ldy) {
   const int b = blockIdx.y;
   const int c = blockIdx.z;
@@ -580,19 +623,21 @@ __global__ void k_spmv_abs(int nf, const int* __restrict__ rowptr, const int* __
   if (row >= 3 * nf) return;
   const int I = row / 3, cc = row % 3;
   const double2* Mb = blocks[b];
-  const double2* x = X + b * ldx + (long long)c * n;
+  const double2* x = X + b * ldx + (long long)c * cs;
   double acc = 0.0;
   for (int s = rowptr[I]; s < rowptr[I + 1]; ++s) {
     const int J = colidx[s];
     const double2* blk = Mb + (long long)s * 9 + 3 * cc;
     for (int q = 0; q < 3; ++q) acc += hypot(blk[q].x, blk[q].y) * hypot(x[3LL * J + q].x, x[3LL * J + q].y);
   }
-  Y[b * ldy + (long long)c * n + row] = make_double2(acc, 0.0);
+  Y[b * ldy + (long long)c * cs + row] = make_double2(acc, 0.0);
 }
 
-// per column: sum_i (AK_i + |lam| AM_i)^2 (real parts), chunk partials
-__global__ __launch_bounds__(256) void k_floor_partial(const double2* __restrict__ AK, const double2* __restrict__ AM,
-                                                       long long ldb, long long n, const double* __restrict__ lam,
+// per column: sum_i (AK_i + |lam| AM_i)^2, or with AD: sum_i (AK_i + |lam| AD_i + lam^2 AM_i)^2
+// (real parts of n entries, columns cs apart), chunk partials
+__global__ __launch_bounds__(256) void k_floor_partial(const double2* __restrict__ AK, const double2* __restrict__ AD,
+                                                       const double2* __restrict__ AM, long long ldb, long long n,
+                                                       long long cs, const double* __restrict__ lam,
                                                        int lam_ld, const int* __restrict__ ncol,
                                                        double* __restrict__ part, int nchunk) {
   const int b = blockIdx.y, c = blockIdx.z;
@@ -602,8 +647,8 @@ __global__ __launch_bounds__(256) void k_floor_partial(const double2* __restrict
   const double l = fabs(lam[(long long)b * lam_ld + c]);
   double s = 0.0;
   for (long long i = i0 + threadIdx.x; i < i0 + kDotChunk && i < n; i += 256) {
-    const long long o = b * ldb + (long long)c * n + i;
-    const double t = AK[o].x + l * AM[o].x;
+    const long long o = b * ldb + (long long)c * cs + i;
+    const double t = AD ? AK[o].x + l * AD[o].x + l * l * AM[o].x : AK[o].x + l * AM[o].x;
     s += t * t;
   }
   const double ts = block_sum(s, sh);

@@ -46,12 +46,13 @@ __global__ void k_combine(const double2* V, long long ldb, long long n, const do
                           long long cstride, const int* nk, const int* nc, double2* out, long long ldo);
 __global__ void k_build_phase_blocks(int nf, const int* rowptr, const int* colidx, const int* tslot,
                                      const unsigned char* pinned, const double* Xc, const double* Xp,
-                                     const double* dXc, const double* dXp, double s, double pre,
-                                     double pim, double2* out);
+                                     const double* dXc, const double* dXp, double s, double pre, double pim,
+                                     double2* out, int skew, double2 scale);
 __global__ void k_spmv_multi(int nf, const int* rowptr, const int* colidx, const double2* const* blocks,
                              const double2* X, long long ldb, long long n, const int* ncol, double2* Y);
+// per column c: ||K x - lam M x||^2, ||K x||^2, Re(x^H M x) over n entries (chunk partials), columns cs apart
 __global__ void k_residual_partial(const double2* X, const double2* KX, const double2* MX, long long ldb, long long n,
-                                   const double* lam, int lam_ld, const int* ncol, double* part,
+                                   long long cs, const double* lam, int lam_ld, const int* ncol, double* part,
                                    int nchunk);
 __global__ void k_expand_modes(int nn, const int* free_of_full, const int* node_class, double cr,
                                double sr, int c1, int c2, int cax, const double2* X, long long n,
@@ -66,22 +67,30 @@ __global__ void k_copy_col(const double2* src, long long lds, long long n, int q
                            long long ldd);
 __global__ void k_flag_cols(const int* ncol, int q, int B, int* act);
 
+// (A_b^H B_b) partial sums over row chunks of n rows; columns cs apart
 __global__ void k_tsgemm_partial(const double2* A, long long lda_b, const double2* Bm, long long ldb_b, long long n,
-                                 const int* ka_b, int kb, const int* act, int kamax, double2* part, int nchunk);
+                                 long long cs, const int* ka_b, int kb, const int* act, int kamax, double2* part, int nchunk);
 __global__ void k_tsgemm_reduce(const double2* part, const int* ka_b, int kb, const int* act, int kamax, int nchunk,
                                 double2* C);
-__global__ void k_tsmm(const double2* A, long long lda_b, long long n, const double2* C, const int* ka_b, int kc,
+__global__ void k_tsmm(const double2* A, long long lda_b, long long n, long long cs, const double2* C, const int* ka_b, int kc,
                        const int* act, int kamax, double alpha, double beta, double2* Y, long long ldy_b,
                        const int* yoff);
 __global__ void k_copy_cols(const double2* src, long long lds, const int* s0v, double2* dst, long long ldd,
                             const int* d0v, int ncol, long long n, const int* act);
 template <int PB>
 __global__ void k_spmv_cols(int nf, const int* rowptr, const int* colidx, const double2* const* blocks,
-                            const double2* X, long long ldx, long long n, const int* ncol, double2* Y, long long ldy);
+                            const double2* X, long long ldx, long long n, const int* ncol, double2* Y, long long ldy,
+                            int accumulate);
+__global__ void k_axpy_half(const double2* src, long long lds, const int* s0v, long long soff, double2* dst,
+                            long long ldd, const int* d0v, long long doff, int ncol, long long len, long long stride,
+                            double alpha, double beta, const int* act);
+__global__ void k_head_dot(const double2* X, const double2* Y, long long ldb, long long stride, long long len,
+                           const int* ncol, int ldc, double* part, int nchunk);
 __global__ void k_spmv_abs(int nf, const int* rowptr, const int* colidx, const double2* const* blocks,
                            const double2* X, long long ldx, long long n, const int* ncol, double2* Y,
                            long long ldy);
-__global__ void k_floor_partial(const double2* AK, const double2* AM, long long ldb, long long n, const double* lam,
+__global__ void k_floor_partial(const double2* AK, const double2* AD, const double2* AM, long long ldb, long long n,
+                                long long cs, const double* lam,
                                 int lam_ld, const int* ncol, double* part, int nchunk);
 // Host: H x = lambda G x, G Hermitian positive definite (Cholesky reduction + herm_eig);
 // false when G is not numerically positive definite.  X[k*m+q] (re, im) = component k of vector q.
