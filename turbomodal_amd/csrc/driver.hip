@@ -1542,7 +1542,7 @@ class BatchLanczos {
         shifted = ints(h);
         dn = &shifted;
       }
-      hipLaunchKernelGGL(k_spmv_cols<8>, dim3((S_.nf + 255) / 256, B_), dim3(256), 0, stream(), S_.nf,
+      hipLaunchKernelGGL(k_spmv_cols<8>, dim3((S_.nf + 3) / 4, B_), dim3(256), 0, stream(), S_.nf,
                          M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), blocks.as<const double2*>(), X + off, ld,
                          cstride, dn->as<int>(), Y + off, ld, accumulate ? 1 : 0);
     }

@@ -221,8 +221,18 @@ __global__ __launch_bounds__(256) void k_panel_trsm(FactorArgs fa, const RowTask
 }
 
 // F[i][j] -= sum_{k in [tk, tk+kd)} L[i][k] d_k conj(L[j][k]) on a 64x64 tile,
-// source columns staged through LDS 32 at a time (one read-modify-write of the
-// tile per super-panel instead of one per 32 columns)
+// source columns staged through LDS (one read-modify-write of the
+// tile per super-panel instead of one per staged chunk).  The products run on the
+// FP64 matrix cores: each wave owns a 32x32 quarter of the tile as 2x2 blocks of
+// 16x16, and every complex block product is four real v_mfma_f64_16x16x4
+// (re re + im im into the real part, im re - re im into the imaginary part).
+// Operand layout of v_mfma_f64_16x16x4: lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; result register j of lane l is C[4 j + l / 16][l % 16].
+// Columns are staged SCHUR_KC at a time (34 KB of LDS per block, so three to
+// four blocks share a CU and one's staging overlaps another's products).
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int SCHUR_KC = 16;
+
 __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTask* __restrict__ tasks,
                                                     int t0, int kb) {
   (void)kb;
@@ -234,19 +244,24 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
   const int jlim = t.jend < 0 ? m : min(t.jend, w);
   double2* Pn = fa.L + b * fa.PE + s.poff;
   double2* Un = fa.U + b * fa.UE + s.uoff;
-  __shared__ double2 Li[TILE][NBMAX + 1];
-  __shared__ double2 Lj[TILE][NBMAX + 1];
+  __shared__ double2 Li[TILE][SCHUR_KC + 1];
+  __shared__ double2 Lj[TILE][SCHUR_KC + 1];
   const int tid = threadIdx.x;
-  const int tx = tid % 16, ty = tid / 16;
-  double2 acc[4][4];
+  const int lane = tid & 63, wv = tid >> 6;
+  const int r0 = 32 * (wv & 1), q0 = 32 * (wv >> 1);  // this wave's rows / columns in the tile
+  const int l16 = lane & 15, l4 = lane >> 4;
+  v4d cr[2][2], ci[2][2];
 #pragma unroll
-  for (int p = 0; p < 4; ++p)
+  for (int p = 0; p < 2; ++p)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[p][q] = make_double2(0.0, 0.0);
-  for (int c0 = t.tk; c0 < t.tk + t.kd; c0 += NBMAX) {
-    const int jb = min(NBMAX, t.tk + t.kd - c0);
+    for (int q = 0; q < 2; ++q) {
+      cr[p][q] = v4d{0.0, 0.0, 0.0, 0.0};
+      ci[p][q] = v4d{0.0, 0.0, 0.0, 0.0};
+    }
+  for (int c0 = t.tk; c0 < t.tk + t.kd; c0 += SCHUR_KC) {
+    const int jb = min(SCHUR_KC, t.tk + t.kd - c0);
     __syncthreads();
-    for (int idx = tid; idx < TILE * NBMAX; idx += 256) {
+    for (int idx = tid; idx < TILE * SCHUR_KC; idx += 256) {
       const int ii = idx % TILE, k = idx / TILE;
       const int gi = t.i0 + ii, gj = t.j0 + ii;
       if (k < jb) {
@@ -260,30 +275,38 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
       }
     }
     __syncthreads();
-#pragma unroll 4
-    for (int k = 0; k < NBMAX; ++k) {
-      double2 a[4], bq[4];
+    const int kend = (jb + 3) & ~3;  // entries past jb are zero
+    for (int kk = 0; kk < kend; kk += 4) {
+      const int kx = kk + l4;
+      double2 a[2], bq[2];
 #pragma unroll
-      for (int p = 0; p < 4; ++p) a[p] = Li[tx + 16 * p][k];
+      for (int p = 0; p < 2; ++p) a[p] = Li[r0 + 16 * p + l16][kx];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) bq[q] = Lj[ty + 16 * q][k];
+      for (int q = 0; q < 2; ++q) bq[q] = Lj[q0 + 16 * q + l16][kx];
 #pragma unroll
-      for (int p = 0; p < 4; ++p)
+      for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc[p][q] = cadd(acc[p][q], cmulc(a[p], bq[q]));
+        for (int q = 0; q < 2; ++q) {
+          cr[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p].x, bq[q].x, cr[p][q], 0, 0, 0);
+          cr[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p].y, bq[q].y, cr[p][q], 0, 0, 0);
+          ci[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p].y, bq[q].x, ci[p][q], 0, 0, 0);
+          ci[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[p].x, bq[q].y, ci[p][q], 0, 0, 0);
+        }
     }
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int gj = t.j0 + ty + 16 * q;
+  for (int q = 0; q < 2; ++q) {
+    const int gj = t.j0 + q0 + 16 * q + l16;
     if (gj >= jlim) continue;
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      const int gi = t.i0 + tx + 16 * p;
-      if (gi >= m || gi < gj) continue;
-      double2* dst = front_at(Pn, Un, m, w, r, gi, gj);
-      *dst = csub(*dst, acc[p][q]);
-    }
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int gi = t.i0 + r0 + 16 * p + 4 * j + l4;
+        if (gi >= m || gi < gj) continue;
+        double2* dst = front_at(Pn, Un, m, w, r, gi, gj);
+        *dst = make_double2(dst->x - cr[p][q][j], dst->y - ci[p][q][j]);
+      }
   }
 }
 

@@ -509,9 +509,11 @@ __global__ __launch_bounds__(256) void k_tsmm(const double2* __restrict__ A, lon
     }
 }
 
-// Y_b[:, c] = A_b X_b[:, c] for c < ncol_b <= PB: one thread per node row reads
-// its 3x3 blocks once and applies them to every column; each gathered x triple
-// serves the node's three rows
+// Y_b[:, c] (+)= A_b X_b[:, c] for c < ncol_b <= 8: one wave per node row.  Lane
+// l takes block slot l / 8 and column l % 8: the wave reads eight consecutive 3x3
+// blocks of the row (contiguous) per pass, each lane gathers the x triple of its
+// block column for its column, and the eight block slots are summed by
+// shuffles.  Columns are n apart in X and Y, problems ldx / ldy apart.
 template <int PB>
 __global__ __launch_bounds__(256) void k_spmv_cols(int nf, const int* __restrict__ rowptr,
                                                    const int* __restrict__ colidx,
@@ -519,44 +521,38 @@ __global__ __launch_bounds__(256) void k_spmv_cols(int nf, const int* __restrict
                                                    const double2* __restrict__ X, long long ldx, long long n,
                                                    const int* __restrict__ ncol, double2* __restrict__ Y,
                                                    long long ldy, int accumulate) {
+  static_assert(PB == 8, "eight columns per lane group");
   const int b = blockIdx.y;
   const int nc = ncol[b];
   if (nc <= 0) return;
-  const int I = blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= nf) return;
+  const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (I >= nf) return;  // whole waves leave together
+  const int lane = threadIdx.x & 63, c = lane & 7, sl = lane >> 3;
   const double2* Mb = blocks[b];
-  const double2* x = X + b * ldx;
-  double2 acc[3][PB];
+  const double2* x = X + b * ldx + (long long)c * n;
+  double2 acc[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  const int s1 = rowptr[I + 1];
+  if (c < nc)
+    for (int s = rowptr[I] + sl; s < s1; s += 8) {
+      const double2* blk = Mb + (long long)s * 9;
+      const long long J3 = 3LL * colidx[s];
+      const double2 x0 = x[J3], x1 = x[J3 + 1], x2 = x[J3 + 2];
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr)
+        acc[rr] = cadd(acc[rr], cadd(cmul(blk[3 * rr], x0), cadd(cmul(blk[3 * rr + 1], x1), cmul(blk[3 * rr + 2], x2))));
+    }
 #pragma unroll
   for (int rr = 0; rr < 3; ++rr)
 #pragma unroll
-    for (int c = 0; c < PB; ++c) acc[rr][c] = make_double2(0.0, 0.0);
-  for (int s = rowptr[I]; s < rowptr[I + 1]; ++s) {
-    const long long J3 = 3LL * colidx[s];
-    const double2* blk = Mb + (long long)s * 9;
-    double2 a[9];
+    for (int o = 8; o < 64; o <<= 1) {
+      acc[rr].x += __shfl_xor(acc[rr].x, o, 64);
+      acc[rr].y += __shfl_xor(acc[rr].y, o, 64);
+    }
+  if (sl == 0 && c < nc) {
+    double2* y = Y + b * ldy + (long long)c * n + 3LL * I;
 #pragma unroll
-    for (int e = 0; e < 9; ++e) a[e] = blk[e];
-#pragma unroll
-    for (int c = 0; c < PB; ++c)
-      if (c < nc) {
-        const double2* xc = x + (long long)c * n + J3;
-        const double2 x0 = xc[0], x1 = xc[1], x2 = xc[2];
-#pragma unroll
-        for (int rr = 0; rr < 3; ++rr)
-          acc[rr][c] = cadd(acc[rr][c],
-                            cadd(cmul(a[3 * rr], x0), cadd(cmul(a[3 * rr + 1], x1), cmul(a[3 * rr + 2], x2))));
-      }
+    for (int rr = 0; rr < 3; ++rr) y[rr] = accumulate ? cadd(y[rr], acc[rr]) : acc[rr];
   }
-  double2* y = Y + b * ldy + 3LL * I;
-#pragma unroll
-  for (int c = 0; c < PB; ++c)
-    if (c < nc)
-#pragma unroll
-      for (int rr = 0; rr < 3; ++rr) {
-        double2* yy = y + (long long)c * n + rr;
-        *yy = accumulate ? cadd(*yy, acc[rr][c]) : acc[rr][c];
-      }
 }
 template __global__ void k_spmv_cols<8>(int, const int*, const int*, const double2* const*, const double2*,
                                         long long, long long, const int*, double2*, long long, int);
