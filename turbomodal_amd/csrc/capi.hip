@@ -92,6 +92,7 @@ void tm_context_destroy(tm_context* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  ctx->deliver.reset();  // drains its copies
   for (auto& p : ctx->pending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);

@@ -32,6 +32,150 @@ namespace tme {
 // ---------------------------------------------------------------------------
 // timing
 // ---------------------------------------------------------------------------
+Deliverer::Deliverer(int device) : device_(device) {
+  TM_HIP(hipSetDevice(device));
+  TM_HIP(hipStreamCreateWithFlags(&cs_, hipStreamNonBlocking));
+  th_ = std::thread([this] { run(); });
+}
+
+Deliverer::~Deliverer() {
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    stop_ = true;
+  }
+  cv_.notify_all();
+  if (th_.joinable()) th_.join();
+  for (auto& f : free_) (void)hipFree(f.second);
+  for (auto e : events_) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(cs_);
+}
+
+void* Deliverer::stage(size_t n, size_t& cap) {
+  n = DevPool::round(n);
+  std::unique_lock<std::mutex> lk(mu_);
+  cv_done_.wait(lk, [&] { return inflight_ == 0 || inflight_ + n <= kMaxInFlight || !err_.empty(); });
+  if (!err_.empty()) throw HipError("result copy: " + err_);
+  auto it = free_.lower_bound(n);
+  void* p = nullptr;
+  if (it != free_.end() && it->first <= 2 * n + (size_t(4) << 20)) {
+    p = it->second;
+    cap = it->first;
+    free_.erase(it);
+  } else {
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess && !free_.empty()) {  // give the idle blocks back and retry
+      (void)hipGetLastError();
+      for (auto& f : free_) (void)hipFree(f.second);
+      free_.clear();
+      e = hipMalloc(&p, n);
+    }
+    if (e != hipSuccess) throw HipError("hipMalloc(" + std::to_string(n) + " B): " + hipGetErrorString(e));
+    cap = n;
+  }
+  inflight_ += cap;
+  return p;
+}
+
+void Deliverer::unstage(void* p, size_t cap) {
+  std::lock_guard<std::mutex> g(mu_);
+  free_.emplace(cap, p);
+  inflight_ -= cap;
+  cv_done_.notify_all();
+}
+
+void Deliverer::enqueue(hipStream_t s, Job j) {
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    if (!events_.empty()) {
+      j.ev = events_.back();
+      events_.pop_back();
+    }
+  }
+  if (!j.ev) TM_HIP(hipEventCreateWithFlags(&j.ev, hipEventDisableTiming));
+  TM_HIP(hipEventRecord(j.ev, s));
+  std::lock_guard<std::mutex> g(mu_);
+  q_.push_back(std::move(j));
+  cv_.notify_all();
+}
+
+void Deliverer::copy(hipStream_t s, void* src, size_t cap, void* dst, size_t n) {
+  Job j;
+  j.src = src;
+  j.cap = cap;
+  j.dst = dst;
+  j.n = n;
+  enqueue(s, std::move(j));
+}
+
+void Deliverer::then(hipStream_t s, std::function<void(const std::string&)> fn) {
+  Job j;
+  j.fn = std::move(fn);
+  enqueue(s, std::move(j));
+}
+
+void Deliverer::run() {
+  (void)hipSetDevice(device_);
+  while (true) {
+    Job j;
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_.wait(lk, [&] { return stop_ || !q_.empty(); });
+      if (q_.empty()) return;
+      j = std::move(q_.front());
+      q_.pop_front();
+      ++active_;
+    }
+    std::string e;
+    hipError_t r = hipSuccess;
+    if (j.dst) {
+      r = hipStreamWaitEvent(cs_, j.ev, 0);
+      if (r == hipSuccess) r = hipMemcpyAsync(j.dst, j.src, j.n, hipMemcpyDeviceToHost, cs_);
+      if (r == hipSuccess) r = hipStreamSynchronize(cs_);
+    } else {
+      r = hipEventSynchronize(j.ev);
+    }
+    if (r != hipSuccess) e = hipGetErrorString(r);
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      if (!e.empty() && err_.empty()) err_ = e;
+      if (j.fn && e.empty()) e = err_;
+    }
+    if (j.fn) {
+      try {
+        j.fn(e);
+      } catch (const std::exception& x) {
+        std::lock_guard<std::mutex> g(mu_);
+        if (err_.empty()) err_ = x.what();
+      }
+    }
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      if (j.src) {
+        free_.emplace(j.cap, j.src);
+        inflight_ -= j.cap;
+      }
+      events_.push_back(j.ev);
+      --active_;
+    }
+    cv_done_.notify_all();
+  }
+}
+
+void Deliverer::drain_quiet() {
+  std::unique_lock<std::mutex> lk(mu_);
+  cv_done_.wait(lk, [&] { return q_.empty() && active_ == 0; });
+}
+
+void Deliverer::drain() {
+  drain_quiet();
+  std::lock_guard<std::mutex> g(mu_);
+  if (!err_.empty()) {
+    const std::string e = err_;
+    err_.clear();
+    throw HipError("result copy: " + e);
+  }
+}
+
 hipEvent_t Context::get_event() {
   if (!event_pool.empty()) {
     hipEvent_t e = event_pool.back();
@@ -1309,21 +1453,39 @@ class BatchLanczos {
       if (out->frequencies) std::copy(freq.begin(), freq.end(), out->frequencies);
       if (out->whirl) std::copy(whirl.begin(), whirl.end(), out->whirl);
       if (out->modal_mass) std::copy(mass.begin(), mass.end(), out->modal_mass);
-      if (out->mode_shapes && nm > 0) {
-        DevBuf dsrc = upload(pool(), src, stream()), dconj = upload(pool(), conjq, stream());
-        DevBuf dout(pool(), sizeof(double2) * 3 * (size_t)S_.nn * nm);
-        const double alpha = 2.0 * M_PI / S_.N;
-        const long long tot = (long long)S_.nn * nm;
-        hipLaunchKernelGGL(k_expand_modes, dim3((tot + 255) / 256), dim3(256), 0, stream(), S_.nn,
-                           M_.d_free_of_full.as<int>(), M_.d_node_class.as<int>(), std::cos(alpha),
-                           std::sin(alpha), S_.c1, S_.c2, S_.cax, X + b * ldx, nvec_,
-                           dsrc.as<int>(), dconj.as<int>(), nm, hp->pre, hp->pim, dout.as<double2>());
-        TM_HIP(hipMemcpyAsync(out->mode_shapes, dout.p, sizeof(double2) * 3 * (size_t)S_.nn * nm,
-                              hipMemcpyDeviceToHost, stream()));
-        TM_HIP(hipStreamSynchronize(stream()));
-      }
+      if (out->mode_shapes && nm > 0) expand_and_send(b, hp, X, ldx, src, conjq, {});
     }
     TM_HIP(hipStreamSynchronize(stream()));
+  }
+
+  // Mode shapes of problem b (columns src of X, conjugated where conjq, times
+  // scale[q] when given) expanded to all nodes on the device and sent to the
+  // caller's buffer by the context's deliverer (the copy overlaps later work).
+  void expand_and_send(int b, HostProblem* hp, const double2* X, long long ldx, const std::vector<int>& src,
+                       const std::vector<int>& conjq, const std::vector<double>& scale) {
+    const int nm = static_cast<int>(src.size());
+    const size_t bytes = sizeof(double2) * 3 * (size_t)S_.nn * nm;
+    DevBuf dsrc = upload(pool(), src, stream()), dconj = upload(pool(), conjq, stream());
+    DevBuf dscale;
+    if (!scale.empty()) dscale = upload(pool(), scale, stream());
+    Deliverer& dl = ctx_->deliverer();
+    size_t cap = 0;
+    void* stage = dl.stage(bytes, cap);
+    try {
+      const double alpha = 2.0 * M_PI / S_.N;
+      const long long tot = (long long)S_.nn * nm;
+      hipLaunchKernelGGL(k_expand_modes, dim3((tot + 255) / 256), dim3(256), 0, stream(), S_.nn,
+                         M_.d_free_of_full.as<int>(), M_.d_node_class.as<int>(), std::cos(alpha), std::sin(alpha),
+                         S_.c1, S_.c2, S_.cax, X + b * ldx, nvec_, dsrc.as<int>(), dconj.as<int>(), nm, hp->pre,
+                         hp->pim, dscale.p ? dscale.as<double>() : (const double*)nullptr,
+                         static_cast<double2*>(stage));
+      TM_HIP(hipGetLastError());
+      dl.copy(stream(), stage, cap, hp->out->mode_shapes, bytes);
+    } catch (...) {
+      TM_HIP(hipStreamSynchronize(stream()));
+      dl.unstage(stage, cap);
+      throw;
+    }
   }
 
   // Results of a Coriolis problem (src/modal_solver.cpp:258-307, src/cyclic_solver.cpp:1058-1096):
@@ -1379,24 +1541,10 @@ class BatchLanczos {
     out->converged = hp->converged ? 1 : 0;
     out->restarts = hp->restarts;
     out->status = TM_OK;
-    if (out->mode_shapes) {
-      DevBuf dsrc = upload(pool(), keep, stream()), dconj = upload(pool(), conjq, stream());
-      DevBuf dout(pool(), sizeof(double2) * 3 * (size_t)S_.nn * nm);
-      const double alpha = 2.0 * M_PI / S_.N;
-      const long long tot = (long long)S_.nn * nm;
-      hipLaunchKernelGGL(k_expand_modes, dim3((tot + 255) / 256), dim3(256), 0, stream(), S_.nn,
-                         M_.d_free_of_full.as<int>(), M_.d_node_class.as<int>(), std::cos(alpha), std::sin(alpha),
-                         S_.c1, S_.c2, S_.cax, X + b * ldx + n_, nvec_, dsrc.as<int>(), dconj.as<int>(), nm, hp->pre,
-                         hp->pim, dout.as<double2>());
-      TM_HIP(hipMemcpyAsync(out->mode_shapes, dout.p, sizeof(double2) * 3 * (size_t)S_.nn * nm,
-                            hipMemcpyDeviceToHost, stream()));
-      TM_HIP(hipStreamSynchronize(stream()));
-      double2* ms = reinterpret_cast<double2*>(out->mode_shapes);
-      for (long long r = 0; r < 3LL * S_.nn; ++r)
-        for (int q = 0; q < nm; ++q) {
-          const double w = om[keep[q]];
-          ms[r * nm + q] = make_double2(w * ms[r * nm + q].x, w * ms[r * nm + q].y);
-        }
+    if (out->mode_shapes) {  // omega b
+      std::vector<double> sc(nm);
+      for (int q = 0; q < nm; ++q) sc[q] = om[keep[q]];
+      expand_and_send(b, hp, X + n_, ldx, keep, conjq, sc);
     }
   }
 
@@ -2326,7 +2474,21 @@ int64_t problem_footprint(const Operator& op, int num_modes, int harmonic, int m
   return problem_footprint(op, p);
 }
 
+static void solve_problems_impl(Context* ctx, const tm_problem* problems, int n, tm_result* results);
+
+// Mode shapes reach the caller's buffers through the context's deliverer; every
+// copy has landed when this returns, also when it throws.
 void solve_problems(Context* ctx, const tm_problem* problems, int n, tm_result* results) {
+  try {
+    solve_problems_impl(ctx, problems, n, results);
+  } catch (...) {
+    if (ctx->deliver) ctx->deliver->drain_quiet();
+    throw;
+  }
+  if (ctx->deliver) ctx->deliver->drain();
+}
+
+static void solve_problems_impl(Context* ctx, const tm_problem* problems, int n, tm_result* results) {
   std::vector<HostProblem> hps(n);
   // batches: problems of one operator and one method, ordered by nev so a
   // batch shares its block size and basis length
@@ -2418,6 +2580,7 @@ struct SubContext {
   }
   ~SubContext() {
     (void)hipStreamSynchronize(c.stream);
+    c.deliver.reset();  // drains its copies
     c.pool.trim();
     for (auto& p : c.pending) {
       (void)hipEventDestroy(p.a);
@@ -2594,7 +2757,12 @@ struct Server {
         if (!adm.empty()) g->bl->admit(adm);
         if (g->bl->busy()) {
           g->bl->block_step();
-          finish(g->bl->retire());
+          std::vector<HostProblem*> fin = g->bl->retire();
+          if (!fin.empty())  // done once their mode shapes have landed
+            g->sub->c.deliverer().then(g->sub->c.stream, [this, fin](const std::string& e) {
+              if (e.empty()) finish(fin);
+              else fail("result copy: " + e);
+            });
         }
       }
     } catch (const std::exception& e) {
@@ -2647,6 +2815,7 @@ struct Server {
         for (auto& kv : small) {
           BatchLanczos bl(ctx, *kv.first, kv.second, false);
           bl.run();
+          ctx->deliverer().drain();
           finish(kv.second);
         }
       }

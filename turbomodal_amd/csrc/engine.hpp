@@ -3,11 +3,16 @@
 
 #include <hip/hip_runtime.h>
 
+#include <condition_variable>
 #include <cstdint>
+#include <deque>
+#include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/tmhip.h"
@@ -149,6 +154,54 @@ inline DevBuf upload(const std::vector<T>& v, hipStream_t s) {
   return upload<T>(nullptr, v, s);
 }
 
+// Device-to-host delivery of results off the compute stream.  A kernel writes a
+// staging block on the compute stream; a worker thread waits for that work on
+// its own stream, copies the block into the caller's host buffer and recycles
+// it, while the compute stream goes on with the next batch.  Callbacks queued
+// with then() run on the worker once everything queued before them is done.
+class Deliverer {
+ public:
+  explicit Deliverer(int device);
+  ~Deliverer();
+  Deliverer(const Deliverer&) = delete;
+  Deliverer& operator=(const Deliverer&) = delete;
+  // a staging block of at least n bytes (cap: its size); waits while more than
+  // kMaxInFlight bytes are waiting for their copies
+  void* stage(size_t n, size_t& cap);
+  void unstage(void* p, size_t cap);  // a block that will not be copied
+  // copy n bytes of src to host dst once the work queued so far on s is done
+  void copy(hipStream_t s, void* src, size_t cap, void* dst, size_t n);
+  // fn("") after the work queued so far on s and every earlier copy; fn(message) on a copy error
+  void then(hipStream_t s, std::function<void(const std::string&)> fn);
+  void drain();        // waits for every queued job; throws the first copy error
+  void drain_quiet();  // waits, keeps the error for the next drain
+  static constexpr size_t kMaxInFlight = size_t(1) << 30;
+
+ private:
+  struct Job {
+    hipEvent_t ev = nullptr;
+    void* src = nullptr;
+    size_t cap = 0;
+    void* dst = nullptr;
+    size_t n = 0;
+    std::function<void(const std::string&)> fn;
+  };
+  void enqueue(hipStream_t s, Job j);
+  void run();
+  int device_;
+  hipStream_t cs_ = nullptr;
+  std::thread th_;
+  std::mutex mu_;
+  std::condition_variable cv_, cv_done_;
+  std::deque<Job> q_;
+  std::multimap<size_t, void*> free_;
+  std::vector<hipEvent_t> events_;
+  size_t inflight_ = 0;
+  int active_ = 0;
+  bool stop_ = false;
+  std::string err_;
+};
+
 // Per-kernel event timing (bench roofline) -----------------------------------
 struct KernelStat {
   int64_t launches = 0;
@@ -166,8 +219,13 @@ struct Context {
   struct Pending { std::string name; hipEvent_t a, b; double bytes, flops; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
+  std::unique_ptr<Deliverer> deliver;  // created on first use
   hipEvent_t get_event();
   void flush_timing();  // synchronises and folds pending events into stats
+  Deliverer& deliverer() {
+    if (!deliver) deliver = std::make_unique<Deliverer>(device);
+    return *deliver;
+  }
 };
 
 // RAII scope that adds its host wall time to stats["host:" + name] (profiling only).

@@ -333,12 +333,14 @@ __global__ __launch_bounds__(256) void k_residual_partial(const double2* __restr
 }
 
 // full-sector mode shapes: out[dof][q] (row-major, ld = nm_ld) = T0 x_q with right
-// rows times P; x_q = X[:, src_q], conjugated when dup_q (degenerate duplication)
+// rows times P; x_q = scale_q X[:, src_q] (scale optional), conjugated when dup_q
+// (degenerate duplication)
 __global__ void k_expand_modes(int nn, const int* __restrict__ free_of_full,
                                const int* __restrict__ node_class, double cr, double sr, int c1,
                                int c2, int cax, const double2* __restrict__ X, long long n,
                                const int* __restrict__ src, const int* __restrict__ conjq, int nm,
-                               double pre, double pim, double2* __restrict__ out) {
+                               double pre, double pim, const double* __restrict__ scale,
+                               double2* __restrict__ out) {
   const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (idx >= (long long)nn * nm) return;
   const int i = (int)(idx / nm), q = (int)(idx % nm);
@@ -346,8 +348,9 @@ __global__ void k_expand_modes(int nn, const int* __restrict__ free_of_full,
   double2 v[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
   if (p >= 0) {
     const double2* x = X + (long long)src[q] * n + 3LL * p;
+    const double f = scale ? scale[q] : 1.0;
     for (int c = 0; c < 3; ++c) {
-      v[c] = x[c];
+      v[c] = make_double2(f * x[c].x, f * x[c].y);
       if (conjq[q]) v[c].y = -v[c].y;
     }
     if (node_class[i] == TM_NODE_RIGHT) {

@@ -57,7 +57,7 @@ __global__ void k_residual_partial(const double2* X, const double2* KX, const do
 __global__ void k_expand_modes(int nn, const int* free_of_full, const int* node_class, double cr,
                                double sr, int c1, int c2, int cax, const double2* X, long long n,
                                const int* src, const int* conjq, int nm, double pre, double pim,
-                               double2* out);
+                               const double* scale, double2* out);
 __global__ void k_real_to_complex(const double* x, double2* y, long long n);
 __global__ void k_gram_partial(const double2* Y, const double2* Z, long long ldb, long long n, const int* ncol,
                                int nmax, double2* part, int nchunk);
