@@ -258,29 +258,46 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
       cr[p][q] = v4d{0.0, 0.0, 0.0, 0.0};
       ci[p][q] = v4d{0.0, 0.0, 0.0, 0.0};
     }
+  // staging of one chunk: this thread's SCHUR_KC * TILE / 256 elements of each
+  // panel, loaded one chunk ahead so the loads overlap the previous products
+  constexpr int PER = TILE * SCHUR_KC / 256;
+  double2 ri[PER], rj[PER];
+  auto fetch = [&](int c0) {
+    const int jb = min(SCHUR_KC, t.tk + t.kd - c0);
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int idx = tid + 256 * e, ii = idx % TILE, k = idx / TILE;
+      const int gi = t.i0 + ii, gj = t.j0 + ii;
+      const bool in = k < jb;
+      const long long col = (long long)(c0 + k) * m;
+      ri[e] = in && gi < m ? Pn[gi + col] : make_double2(0.0, 0.0);
+      rj[e] = in && gj < m ? Pn[gj + col] : make_double2(0.0, 0.0);
+    }
+  };
+  __shared__ double dk[SCHUR_KC];
+  fetch(t.tk);
   for (int c0 = t.tk; c0 < t.tk + t.kd; c0 += SCHUR_KC) {
     const int jb = min(SCHUR_KC, t.tk + t.kd - c0);
     __syncthreads();
-    for (int idx = tid; idx < TILE * SCHUR_KC; idx += 256) {
-      const int ii = idx % TILE, k = idx / TILE;
-      const int gi = t.i0 + ii, gj = t.j0 + ii;
-      if (k < jb) {
-        const double d = Pn[(c0 + k) + (long long)(c0 + k) * m].x;
-        const double2 a = gi < m ? Pn[gi + (long long)(c0 + k) * m] : make_double2(0.0, 0.0);
-        Li[ii][k] = make_double2(a.x * d, a.y * d);
-        Lj[ii][k] = gj < m ? Pn[gj + (long long)(c0 + k) * m] : make_double2(0.0, 0.0);
-      } else {
-        Li[ii][k] = make_double2(0.0, 0.0);
-        Lj[ii][k] = make_double2(0.0, 0.0);
-      }
+    if (tid < SCHUR_KC) dk[tid] = tid < jb ? Pn[(c0 + tid) + (long long)(c0 + tid) * m].x : 0.0;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int idx = tid + 256 * e, ii = idx % TILE, k = idx / TILE;
+      Li[ii][k] = ri[e];
+      Lj[ii][k] = rj[e];
     }
     __syncthreads();
+    if (c0 + SCHUR_KC < t.tk + t.kd) fetch(c0 + SCHUR_KC);
     const int kend = (jb + 3) & ~3;  // entries past jb are zero
     for (int kk = 0; kk < kend; kk += 4) {
       const int kx = kk + l4;
+      const double d = dk[kx];
       double2 a[2], bq[2];
 #pragma unroll
-      for (int p = 0; p < 2; ++p) a[p] = Li[r0 + 16 * p + l16][kx];
+      for (int p = 0; p < 2; ++p) {
+        const double2 v = Li[r0 + 16 * p + l16][kx];
+        a[p] = make_double2(v.x * d, v.y * d);
+      }
 #pragma unroll
       for (int q = 0; q < 2; ++q) bq[q] = Lj[q0 + 16 * q + l16][kx];
 #pragma unroll
