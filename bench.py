@@ -36,10 +36,10 @@ FP64_PEAK_TFLOPS = 78.6
 def build_solver(workload, precision):
     import turbomodal_amd as tm
     from turbomodal_amd import meshgen
-    name = {"c4": "bladed_c2", "c2": "bladed_c2", "c1": "flat_disk_c1"}[workload]
+    name = {"c4": "bladed_c2", "c2": "bladed_c2", "c1": "flat_disk_c1", "c3": "bladed_c3"}[workload]
     sm = meshgen.config_mesh(name)
     mesh = meshgen.to_core_mesh(sm, tm._core)
-    mat = tm.Material(**meshgen.STEEL)
+    mat = tm.Material(**(meshgen.TITANIUM if workload == "c3" else meshgen.STEEL))
     solver = tm.CyclicSymmetrySolver(mesh, mat)
     return tm, sm, mesh, mat, solver
 
@@ -49,6 +49,8 @@ def workload_spec(workload):
         return np.linspace(0.0, 15000.0, 20), 10
     if workload == "c2":
         return np.array([10000.0]), 10
+    if workload == "c3":
+        return np.array([12000.0]), 20
     return np.array([0.0]), 5
 
 
@@ -176,7 +178,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c4", choices=["c4", "c2", "c1", "c5"])
+    ap.add_argument("--workload", default="c4", choices=["c4", "c2", "c1", "c3", "c5"])
+    ap.add_argument("--coriolis", action="store_true",
+                    help="include_coriolis=True: the gyroscopic QEP for harmonics k > 0")
     ap.add_argument("--c5-requests", type=int, default=240)
     ap.add_argument("--c5-plain", type=int, default=24)
     ap.add_argument("--c5-c3", action="store_true", help="include the ~330k-DOF C3 generator in the stream")
@@ -204,7 +208,7 @@ def main():
           flush=True)
     for w in range(args.warmup):
         t = time.perf_counter()
-        solver.solve_rpm_sweep(list(rpms), num_modes)
+        solver.solve_rpm_sweep(list(rpms), num_modes, include_coriolis=args.coriolis)
         print(f"[bench] warmup {w} {time.perf_counter() - t:.2f}s", file=sys.stderr, flush=True)
     info = solver.engine_info()
 
@@ -215,7 +219,7 @@ def main():
     t0 = time.perf_counter()
     for s in range(args.steps):
         ts = time.perf_counter()
-        results = solver.solve_rpm_sweep(list(rpms), num_modes)
+        results = solver.solve_rpm_sweep(list(rpms), num_modes, include_coriolis=args.coriolis)
         lat += [r.solve_seconds for row in results for r in row]
         print(f"[bench] step {s} {time.perf_counter() - ts:.2f}s", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
@@ -255,7 +259,9 @@ def main():
         "dtype": "c128 (complex f64)",
         "data": "synthetic (seeded Kuhn TET10 lattice meshes, steel)",
         "config": {"workload": {"c4": "rpm sweep bladed sector 20 rpm x 19 ND", "c2": "bladed sector 19 ND @10000 rpm",
-                                "c1": "flat disk 19 ND @0 rpm"}[args.workload],
+                                "c1": "flat disk 19 ND @0 rpm",
+                                "c3": "doubled bladed sector (Ti) 19 ND @12000 rpm"}[args.workload]
+                   + (" + Coriolis QEP (k > 0)" if args.coriolis else ""),
                    "n_dof": int(sm.num_dof), "n_free": int(info["n_free"]), "num_sectors": int(sm.num_sectors),
                    "num_modes": num_modes, "problems_per_step": n_problems, "precision": args.precision,
                    "eigensolver": args.method,
@@ -275,6 +281,8 @@ def main():
         "cpu_baseline": None,
     }
 
+    if args.workload == "c3" or args.coriolis:  # hours at 330k DOF; the oracle sample is the Hermitian path
+        args.no_cpu_baseline = True
     if not args.no_cpu_baseline:
         rpm_s = float(rpms[len(rpms) * 2 // 3])
         print(f"[bench] cpu baseline sample at {rpm_s:.0f} rpm", file=sys.stderr, flush=True)
