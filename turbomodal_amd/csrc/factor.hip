@@ -42,6 +42,19 @@ __device__ __forceinline__ double2 cconjmul(double2 a, double2 b) {
 }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// acc += a b and acc += conj(a) b as four FMAs each
+__device__ __forceinline__ void cmac(double2& acc, double2 a, double2 b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+}
+__device__ __forceinline__ void cmacc(double2& acc, double2 a, double2 b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(-a.y, b.x, acc.y);
+}
 
 // front element (i, j), i >= j: panel for pivot columns, update matrix otherwise
 __device__ __forceinline__ double2* front_at(double2* Pn, double2* Un, int m, int w, int r, int i,
@@ -540,12 +553,31 @@ __global__ __launch_bounds__(64) void k_fwd_inv(SolveArgs sa, const RowTask* __r
     }
     __syncthreads();
     if (i < t.r1) {
-      const int jend = min(nj, i - j0 + 1);
-      for (int jj = 0; jj < jend; ++jj) {
-        const int j = j0 + jj;
-        const double2 l = j == i ? make_double2(1.0, 0.0) : Pn[i + (long long)j * m];
-        for (int c = 0; c < PM; ++c)
-          if (c < p) acc[c] = cadd(acc[c], cmul(l, xs[jj][c]));
+      if (j0 + nj <= t.r0) {  // strictly below this task's diagonal: every lane takes all nj
+        const double2* lp = Pn + i + (long long)j0 * m;
+        int jj = 0;
+        for (; jj + 4 <= nj; jj += 4) {  // four loads in flight; entries past p are zero
+          double2 l[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) l[u] = lp[(long long)(jj + u) * m];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], xs[jj + u][c]);
+        }
+        for (; jj < nj; ++jj) {
+          const double2 l = lp[(long long)jj * m];
+#pragma unroll
+          for (int c = 0; c < PM; ++c) cmac(acc[c], l, xs[jj][c]);
+        }
+      } else {
+        const int jend = min(nj, i - j0 + 1);
+        for (int jj = 0; jj < jend; ++jj) {
+          const int j = j0 + jj;
+          const double2 l = j == i ? make_double2(1.0, 0.0) : Pn[i + (long long)j * m];
+#pragma unroll
+          for (int c = 0; c < PM; ++c) cmac(acc[c], l, xs[jj][c]);
+        }
       }
     }
   }
@@ -578,12 +610,24 @@ __global__ __launch_bounds__(256) void k_fwd_upd(SolveArgs sa, const RowTask* __
         ys[jj][c] = (jj < nj && c < p) ? *tcol(sa, b, c, s.toff + r + j0 + jj) : make_double2(0.0, 0.0);
       }
       __syncthreads();
-      if (upd)
-        for (int jj = 0; jj < nj; ++jj) {
-          const double2 l = Pn[i + (long long)(j0 + jj) * m];
-          for (int c = 0; c < PM; ++c)
-            if (c < p) acc[c] = cadd(acc[c], cmul(l, ys[jj][c]));
+      if (upd) {
+        const double2* lp = Pn + i + (long long)j0 * m;
+        int jj = 0;
+        for (; jj + 8 <= nj; jj += 8) {  // eight loads in flight; entries past p are zero
+          double2 l[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) l[u] = lp[(long long)(jj + u) * m];
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], ys[jj + u][c]);
         }
+        for (; jj < nj; ++jj) {
+          const double2 l = lp[(long long)jj * m];
+#pragma unroll
+          for (int c = 0; c < PM; ++c) cmac(acc[c], l, ys[jj][c]);
+        }
+      }
     }
   }
   if (i < t.r1 && i < w) {
@@ -656,19 +700,21 @@ __global__ __launch_bounds__(256) void k_bwd_tile(SolveArgs sa, const RowTask* _
       vs[ii][c] = (ii < ni && c < p) ? *tcol(sa, b, c, off) : make_double2(0.0, 0.0);
     }
     __syncthreads();
+    // rows past ni are zero in At and vs; in stage 1 only rows strictly below
+    // the diagonal (i0 + k > j) count
+    const int kmin = stage == 1 ? max(0, min(TR, j - i0 + 1)) : 0;
     if (ROWSPLIT) {
-      for (int k = part; k < ni; k += NP) {
-        double2 a = At[k][jj];
-        if (stage == 1 && i0 + k <= j) a = make_double2(0.0, 0.0);  // strictly below the diagonal
+      for (int k = part; k < TR; k += NP) {
+        const double2 a = k >= kmin ? At[k][jj] : make_double2(0.0, 0.0);
 #pragma unroll
-        for (int u = 0; u < CPT; ++u) acc[u] = cadd(acc[u], cconjmul(a, vs[k][u]));
+        for (int u = 0; u < CPT; ++u) cmacc(acc[u], a, vs[k][u]);
       }
     } else {
-      for (int k = 0; k < ni; ++k) {
-        double2 a = At[k][jj];
-        if (stage == 1 && i0 + k <= j) a = make_double2(0.0, 0.0);
+#pragma unroll 8
+      for (int k = 0; k < TR; ++k) {
+        const double2 a = k >= kmin ? At[k][jj] : make_double2(0.0, 0.0);
 #pragma unroll
-        for (int u = 0; u < CPT; ++u) acc[u] = cadd(acc[u], cconjmul(a, vs[k][part + NP * u]));
+        for (int u = 0; u < CPT; ++u) cmacc(acc[u], a, vs[k][part + NP * u]);
       }
     }
   }
