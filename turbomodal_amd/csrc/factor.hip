@@ -227,7 +227,12 @@ __global__ __launch_bounds__(256) void k_panel_trsm(FactorArgs fa, const RowTask
   for (int j = 0; j < jb; ++j) {
     double2 v = make_double2(0.0, 0.0);
 #pragma unroll
-    for (int k = 0; k < NBMAX; ++k) v = cadd(v, cmulc(x[k], Li[j][k]));
+    for (int k = 0; k < NBMAX; ++k) {  // v += x_k conj(L_jk)
+      v.x = fma(x[k].x, Li[j][k].x, v.x);
+      v.x = fma(x[k].y, Li[j][k].y, v.x);
+      v.y = fma(x[k].y, Li[j][k].x, v.y);
+      v.y = fma(-x[k].x, Li[j][k].y, v.y);
+    }
     const double inv = 1.0 / dk[j];
     Pn[i + (long long)(kb + j) * m] = make_double2(v.x * inv, v.y * inv);
   }
@@ -403,7 +408,7 @@ __device__ __forceinline__ void tile_gemm(double2 (&acc)[4][4], int k0, int k1, 
       for (int p = 0; p < 4; ++p) a[p] = As[tx + 16 * p][kk];
       for (int q = 0; q < 4; ++q) bb[q] = Bs[kk][ty + 16 * q];
       for (int p = 0; p < 4; ++p)
-        for (int q = 0; q < 4; ++q) acc[p][q] = cadd(acc[p][q], cmul(a[p], bb[q]));
+        for (int q = 0; q < 4; ++q) cmac(acc[p][q], a[p], bb[q]);
     }
   }
 }

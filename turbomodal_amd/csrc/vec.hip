@@ -18,6 +18,19 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 __device__ __forceinline__ double2 cconjmul(double2 a, double2 b) {  // conj(a) b
   return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
+// acc += a b and acc += conj(a) b as four FMAs each
+__device__ __forceinline__ void cmac(double2& acc, double2 a, double2 b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+}
+__device__ __forceinline__ void cmacc(double2& acc, double2 a, double2 b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(-a.y, b.x, acc.y);
+}
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -452,11 +465,10 @@ __global__ __launch_bounds__(256) void k_tsgemm_partial(const double2* __restric
       Bs[r][c] = (r < nr && c < kb) ? Bb[(long long)c * cs + rr + r] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    for (int r = 0; r < nr; ++r) {
-      const double2 u = cconjmul(As[r][oa], Bs[r][oc]);
-      re += u.x;
-      im += u.y;
-    }
+    double2 u = make_double2(re, im);
+    for (int r = 0; r < nr; ++r) cmacc(u, As[r][oa], Bs[r][oc]);
+    re = u.x;
+    im = u.y;
   }
   if (oa < na && oc < kb)
     part[(((long long)b * kamax + a0 + oa) * 16 + oc) * nchunk + blockIdx.x] = make_double2(re, im);
@@ -501,7 +513,7 @@ __global__ __launch_bounds__(256) void k_tsmm(const double2* __restrict__ A, lon
   for (int a = 0; a < ka; ++a) {
     const double2 x = Ab[(long long)a * cs + i];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) acc[c] = cadd(acc[c], cmul(x, Cs[a * 16 + c]));
+    for (int c = 0; c < 16; ++c) cmac(acc[c], x, Cs[a * 16 + c]);
   }
   double2* Yb = Y + b * ldy_b + (yoff ? (long long)yoff[b] * cs : 0LL);
 #pragma unroll
@@ -541,8 +553,11 @@ __global__ __launch_bounds__(256) void k_spmv_cols(int nf, const int* __restrict
       const long long J3 = 3LL * colidx[s];
       const double2 x0 = x[J3], x1 = x[J3 + 1], x2 = x[J3 + 2];
 #pragma unroll
-      for (int rr = 0; rr < 3; ++rr)
-        acc[rr] = cadd(acc[rr], cadd(cmul(blk[3 * rr], x0), cadd(cmul(blk[3 * rr + 1], x1), cmul(blk[3 * rr + 2], x2))));
+      for (int rr = 0; rr < 3; ++rr) {
+        cmac(acc[rr], blk[3 * rr], x0);
+        cmac(acc[rr], blk[3 * rr + 1], x1);
+        cmac(acc[rr], blk[3 * rr + 2], x2);
+      }
     }
 #pragma unroll
   for (int rr = 0; rr < 3; ++rr)
