@@ -98,8 +98,13 @@ __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const Asm
   double2* Un = fa.U + b * fa.UE + s.uoff;
   const int tid = threadIdx.x;
   const int j0 = 3 * t.n0, j1 = 3 * t.n1;
-  for (int j = j0; j < j1; ++j)
-    for (int i = j + tid; i < m; i += 256) *front_at(Pn, Un, m, w, r, i, j) = make_double2(0.0, 0.0);
+  {  // zero the lower part of columns [j0, j1): (row, column) pairs over all threads
+    const int nrow = m - j0, tot = nrow * (j1 - j0);  // < 2^31: m < 65536, j1 - j0 <= 48
+    for (int idx = tid; idx < tot; idx += 256) {
+      const int i = j0 + idx % nrow, j = j0 + idx / nrow;
+      if (i >= j) *front_at(Pn, Un, m, w, r, i, j) = make_double2(0.0, 0.0);
+    }
+  }
   __syncthreads();
   const ProbParam pp = fa.params[b];
   const int nent = (t.a1 - t.a0) * 9;
@@ -135,13 +140,15 @@ __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const Asm
     const int rc = 3 * cs.nstr;
     const double2* Uc = fa.U + b * fa.UE + cs.uoff;
     const int* rm = fa.relmap + cs.rel0;
-    for (int jc = 3 * t.clo[k]; jc < 3 * t.chi[k]; ++jc) {
-      const int gj = 3 * rm[jc / 3] + jc % 3;
-      for (int ic = jc + tid; ic < rc; ic += 256) {
-        const int gi = 3 * rm[ic / 3] + ic % 3;
-        double2* dst = front_at(Pn, Un, m, w, r, gi, gj);
-        *dst = cadd(*dst, Uc[ic + (long long)jc * rc]);
-      }
+    // extend-add of the child's update columns [jlo, jhi): (row, column) pairs
+    // over all threads; each front entry has one writer per child
+    const int jlo = 3 * t.clo[k], jhi = 3 * t.chi[k], nrow = rc - jlo, tot = nrow * (jhi - jlo);
+    for (int idx = tid; idx < tot; idx += 256) {
+      const int ic = jlo + idx % nrow, jc = jlo + idx / nrow;
+      if (ic < jc) continue;
+      const int gi = 3 * rm[ic / 3] + ic % 3, gj = 3 * rm[jc / 3] + jc % 3;
+      double2* dst = front_at(Pn, Un, m, w, r, gi, gj);
+      *dst = cadd(*dst, Uc[ic + (long long)jc * rc]);
     }
     __syncthreads();
   }
