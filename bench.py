@@ -54,6 +54,23 @@ def workload_spec(workload):
     return np.array([0.0]), 5
 
 
+def measured_traffic(args):
+    """HBM bytes per triangular-sweep launch of this workload, as the FETCH_SIZE and
+    WRITE_SIZE counters read it (rocprofv3 --pmc, scaled by a plain copy kernel);
+    recorded with its commands in profiles/ by a separate profiled run, since the
+    counters cannot be read inside the timed run."""
+    if args.workload != "c4" or args.method != "block" or args.coriolis:
+        return None
+    path = os.path.join(ROOT, "profiles", "traffic_c4_rpm_sweep_fp64_block.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(sm, rpm, num_modes, ks=(0, 1)):
     """The oracle (numpy/scipy restatement of the reference algorithm, the
     reference's own nev/ncv/tolerance policy) on a bounded sample, one core."""
@@ -178,7 +195,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c4", choices=["c4", "c2", "c1", "c3", "c5"])
+    ap.add_argument("--workload", default="c4", choices=["c4", "c2", "c1", "c3", "c5", "hbmcal"],
+                    help="hbmcal: one plain 256 MiB streaming copy kernel, the reference point that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters to bytes")
     ap.add_argument("--coriolis", action="store_true",
                     help="include_coriolis=True: the gyroscopic QEP for harmonics k > 0")
     ap.add_argument("--c5-requests", type=int, default=240)
@@ -196,6 +215,14 @@ def main():
         sys.exit(2)
     if args.workload == "c5":
         run_c5(args)
+        return
+    if args.workload == "hbmcal":
+        import torch
+        x = torch.ones(32 << 20, dtype=torch.float64, device="cuda")  # 256 MiB
+        y = torch.empty_like(x)
+        torch.mul(x, 1.0, out=y)  # one streaming kernel: 256 MiB read, 256 MiB written
+        torch.cuda.synchronize()
+        print(json.dumps({"workload": "hbmcal", "read_bytes": x.numel() * 8, "write_bytes": y.numel() * 8}))
         return
 
     tm, sm, mesh, mat, solver = build_solver(args.workload, args.precision)
@@ -240,7 +267,8 @@ def main():
         ach = bytes_per / (avg_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": "trisolve (L, D, L^H sweep pair)", "achieved": round(ach, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                    "traffic": None, "avg_launch_ms": round(avg_ms, 3), "alg_bytes_per_launch": bytes_per}
+                    "traffic": measured_traffic(args), "avg_launch_ms": round(avg_ms, 3),
+                    "alg_bytes_per_launch": bytes_per}
     factor_rate = None
     if fac.get("launches"):
         factor_rate = fac["alg_flops"] / (fac["total_ms"] * 1e-3) / 1e12
