@@ -258,10 +258,19 @@ __global__ __launch_bounds__(256) void k_panel_trsm(FactorArgs fa, const RowTask
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int SCHUR_KC = 16;
 
+// Work-groups are dealt round-robin over the eight XCDs (each with its own L2):
+// in a row of n blocks, the blocks with the same x mod 8 share an XCD.  Block x
+// takes task xcd_task(x, n), so every XCD runs a contiguous run of the task list,
+// whose neighbouring tasks share their source panels.
+__device__ __forceinline__ int xcd_task(int x, int n) {
+  const int c = x & 7, q = n >> 3, rem = n & 7;
+  return c * q + min(c, rem) + (x >> 3);
+}
+
 __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTask* __restrict__ tasks,
                                                     int t0, int kb) {
   (void)kb;
-  const TileTask t = tasks[t0 + blockIdx.x];
+  const TileTask t = tasks[t0 + xcd_task(blockIdx.x, gridDim.x)];
   const int b = blockIdx.y;
   if (fa.active && !fa.active[b]) return;
   const DevSn s = fa.sn[t.sn];

@@ -200,9 +200,14 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
             P.row_tasks.push_back({id, r0, std::min(m, r0 + kRowChunk)});
           const int se = std::min(w, ks + KW);           // end of this front's super-panel
           if (last || kb + NB >= w) {
-            // super-panel complete: trailing matrix, depth se - ks
-            for (int j0 = se; j0 < m; j0 += TL)
-              for (int i0 = j0; i0 < m; i0 += TL) P.tile_tasks.push_back({id, i0, j0, ks, se - ks, -1});
+            // super-panel complete: trailing matrix, depth se - ks, in groups of
+            // 4 x 4 tiles so neighbouring tasks share row and column panels
+            const int G = 4 * TL;
+            for (int J = se; J < m; J += G)
+              for (int I = J; I < m; I += G)
+                for (int j0 = J; j0 < std::min(m, J + G); j0 += TL)
+                  for (int i0 = std::max(I, j0); i0 < std::min(m, I + G); i0 += TL)
+                    P.tile_tasks.push_back({id, i0, j0, ks, se - ks, -1});
           } else {
             // the rest of the super-panel, columns [t0, se), depth jb
             for (int j0 = t0; j0 < se; j0 += TL)
