@@ -132,3 +132,47 @@ def test_generator_sizes():
         assert vol.min() > 0
         m = oracle_model(sm.nodes, sm.elements, sm.node_sets, sm.num_sectors)
         assert len(m.pairs) == len(sm.node_sets["left_boundary"])
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_grid_matching_equals_greedy_scan(seed):
+    """The grid search of match_boundary_nodes returns the reference's greedy
+    nearest-unused pairing (src/mesh.cpp:289-333), restated as a full scan in the
+    oracle: jittered partners, partners beyond the tolerance, right nodes with no
+    partner, two right nodes at the same distance from a left node, and a left
+    node whose nearest right node is taken by an earlier left node."""
+    import turbomodal_amd as tm
+    from oracle import cyclic
+    rng = np.random.default_rng(seed)
+    N = 24
+    a = 2 * np.pi / N
+    nl = 300
+    r = rng.uniform(0.1, 0.3, nl)
+    z = rng.uniform(0.0, 0.05, nl)
+    left = np.stack([r, np.zeros(nl), z], 1)
+    rot = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    right = left @ rot.T
+    right += rng.normal(0, 1e-6, right.shape) * (rng.uniform(0, 1, (nl, 1)) < 0.5)  # within tolerance
+    right[:10] += 1e-2                                                            # beyond tolerance
+    extra = rng.uniform(0.1, 0.3, (20, 1)) * np.array([[np.cos(a), np.sin(a), 0]]) + [0, 0, 0.07]
+    twin = right[50] + np.array([0.0, 0.0, 2e-6])                                 # tie candidate
+    right = np.vstack([right, extra, twin, right[60] + 1e-7])                     # a near-duplicate
+    X = np.vstack([left, right, np.zeros((10, 3))])
+    nL, nR = len(left), len(right)
+    lid = list(range(nL))
+    rid = list(range(nL, nL + nR))
+    perm = rng.permutation(nR)
+    rid = [rid[p] for p in perm]
+    T = np.tile(np.arange(nL + nR, nL + nR + 10, dtype=np.int32), (1, 1))
+    sets = []
+    for name, ids in (("left_boundary", lid), ("right_boundary", rid)):
+        ns = tm.NodeSet()
+        ns.name = name
+        ns.node_ids = ids
+        sets.append(ns)
+    mesh = tm.Mesh()
+    mesh.load_from_arrays(X, T, sets, N, 2)
+    l2, r2, pairs, free = cyclic.match_boundary_nodes(X, lid, rid, N)
+    assert [tuple(p) for p in mesh.matched_pairs] == [tuple(p) for p in pairs]
+    assert list(mesh.left_boundary) == l2 and list(mesh.right_boundary) == r2
+    assert sorted(mesh.free_boundary) == free

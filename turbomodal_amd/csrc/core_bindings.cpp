@@ -25,6 +25,7 @@
 #include <memory>
 #include <mutex>
 #include <set>
+#include <unordered_map>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -272,19 +273,48 @@ struct Mesh {
     std::vector<char> used(R, 0);
     matched_pairs.clear();
     std::vector<int> nl, nr, fl;
+    // The reference pairs each left node, in order, with the nearest unused
+    // right node and keeps the pair when it lies within the tolerance
+    // (src/mesh.cpp:289-333).  A pair is only kept within the tolerance, so a
+    // uniform grid of cell size = the largest tolerance, searched over the 27
+    // cells around the left node, finds the same node (ties: the first index)
+    // in O(L + R) instead of O(L R).
+    auto tol_of = [&](double l1, double l2) { return std::max(1e-6, 1e-4 * std::sqrt(l1 * l1 + l2 * l2)); };
+    double cell = 1e-6;
+    for (int lid : left_boundary)
+      cell = std::max(cell, tol_of(nodes[3 * (size_t)lid + c1], nodes[3 * (size_t)lid + c2]));
+    auto cell_of = [&](double v) { return static_cast<long long>(std::floor(v / cell)); };
+    auto key = [](long long i, long long j, long long k) {
+      return ((i & 0x1fffff) << 42) | ((j & 0x1fffff) << 21) | (k & 0x1fffff);
+    };
+    std::unordered_map<long long, std::vector<int>> grid;
+    grid.reserve(2 * R);
+    for (size_t j = 0; j < R; ++j) grid[key(cell_of(rp[j][0]), cell_of(rp[j][1]), cell_of(rp[j][2]))].push_back((int)j);
     for (int lid : left_boundary) {
       const double l1 = nodes[3 * (size_t)lid + c1], l2 = nodes[3 * (size_t)lid + c2],
                    la = nodes[3 * (size_t)lid + cax];
+      const long long g1 = cell_of(l1), g2 = cell_of(l2), g3 = cell_of(la);
       double best = std::numeric_limits<double>::max();
       long best_j = -1;
-      for (size_t j = 0; j < R; ++j) {
-        if (used[j]) continue;
-        const double d1 = l1 - rp[j][0], d2 = l2 - rp[j][1], d3 = la - rp[j][2];
-        const double d = std::sqrt(d1 * d1 + d2 * d2 + d3 * d3);
-        if (d < best) { best = d; best_j = static_cast<long>(j); }
+      for (long long d1c = -1; d1c <= 1; ++d1c)
+        for (long long d2c = -1; d2c <= 1; ++d2c)
+          for (long long d3c = -1; d3c <= 1; ++d3c) {
+            auto it = grid.find(key(g1 + d1c, g2 + d2c, g3 + d3c));
+            if (it == grid.end()) continue;
+            for (int j : it->second) {
+              if (used[j]) continue;
+              const double d1 = l1 - rp[j][0], d2 = l2 - rp[j][1], d3 = la - rp[j][2];
+              const double d = std::sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+              if (d < best || (d == best && j < best_j)) {
+                best = d;
+                best_j = j;
+              }
+            }
+          }
+      if (best_j < 0 || best > tol_of(l1, l2)) {
+        fl.push_back(lid);
+        continue;
       }
-      const double tol = std::max(1e-6, 1e-4 * std::sqrt(l1 * l1 + l2 * l2));
-      if (best_j < 0 || best > tol) { fl.push_back(lid); continue; }
       used[best_j] = 1;
       nl.push_back(lid);
       nr.push_back(right_boundary[best_j]);
