@@ -652,15 +652,23 @@ void create_operator(Context* ctx, Mesh& M, Operator& op) {
   TM_HIP(hipStreamSynchronize(st));
 }
 
+// M(k) blocks of the operator, cached per harmonic.  Operators are shared by
+// the server's slot groups (each on its own thread and stream): the first user
+// builds the blocks on its stream and records an event that every later user's
+// stream waits for, so no host synchronisation is needed.
 const double2* harmonic_mass(Context* ctx, Operator& op, int k) {
-  // operators are shared by the server's slot groups, each on its own thread and stream
   static std::mutex mu;
   std::lock_guard<std::mutex> g(mu);
   auto it = op.Mk_cache.find(k);
-  if (it != op.Mk_cache.end()) return it->second.as<double2>();
+  if (it != op.Mk_cache.end()) {
+    TM_HIP(hipStreamWaitEvent(ctx->stream, it->second.ready, 0));
+    return it->second.b.as<double2>();
+  }
   Mesh& M = *op.mesh;
   const Symbolic& S = M.S;
-  DevBuf b(sizeof(double2) * 9 * op.nblk);
+  Operator::MkEntry e;
+  e.b.alloc(sizeof(double2) * 9 * op.nblk);
+  DevBuf& b = e.b;
   double alpha = 2.0 * M_PI / S.N;
   hipLaunchKernelGGL(k_build_phase_blocks, dim3((S.nf + 127) / 128), dim3(128), 0, ctx->stream, S.nf,
                      M.d_rowptr.as<int>(), M.d_colidx.as<int>(), M.d_tslot.as<int>(),
@@ -668,9 +676,10 @@ const double2* harmonic_mass(Context* ctx, Operator& op, int k) {
                      (const double*)nullptr, (const double*)nullptr, 0.0, std::cos(k * alpha),
                      std::sin(k * alpha), b.as<double2>(), 0, make_double2(1.0, 0.0));
   TM_HIP(hipGetLastError());
-  TM_HIP(hipStreamSynchronize(ctx->stream));  // complete before other streams read it
+  TM_HIP(hipEventCreateWithFlags(&e.ready, hipEventDisableTiming));
+  TM_HIP(hipEventRecord(e.ready, ctx->stream));
   const double2* p = b.as<double2>();
-  op.Mk_cache.emplace(k, std::move(b));
+  op.Mk_cache.emplace(k, std::move(e));
   return p;
 }
 
@@ -2608,6 +2617,9 @@ struct Server {
     double reserved = 0;             // guarded by srv->mu
     std::thread th;
     bool quit = false;               // guarded by srv->mu
+    // counters (TMHIP_SERVER_STATS=1 prints them when the server stops)
+    long admits = 0, admitted = 0, steps = 0, slot_steps = 0;
+    double t_admit = 0, t_step = 0;
   };
 
   Context* ctx;
@@ -2754,9 +2766,19 @@ struct Server {
               g->queue.pop_front();
             }
         }
-        if (!adm.empty()) g->bl->admit(adm);
+        const auto ta = std::chrono::steady_clock::now();
+        if (!adm.empty()) {
+          g->bl->admit(adm);
+          g->admits += 1;
+          g->admitted += (long)adm.size();
+        }
+        const auto tb = std::chrono::steady_clock::now();
+        g->t_admit += std::chrono::duration<double>(tb - ta).count();
         if (g->bl->busy()) {
+          g->steps += 1;
+          g->slot_steps += g->bl->capacity() - g->bl->free_slots();
           g->bl->block_step();
+          g->t_step += std::chrono::duration<double>(std::chrono::steady_clock::now() - tb).count();
           std::vector<HostProblem*> fin = g->bl->retire();
           if (!fin.empty())  // done once their mode shapes have landed
             g->sub->c.deliverer().then(g->sub->c.stream, [this, fin](const std::string& e) {
@@ -2830,6 +2852,15 @@ struct Server {
     }
     for (auto& kv : groups)
       if (kv.second->th.joinable()) kv.second->th.join();
+    if (std::getenv("TMHIP_SERVER_STATS"))
+      for (auto& kv : groups) {
+        const Group& g = *kv.second;
+        std::fprintf(stderr,
+                     "[tmhip] group nf=%d pb=%d qep=%d: %ld admissions of %ld problems (%.1f s), %ld steps "
+                     "(%.1f s, %.1f slots busy on average)\n",
+                     g.mesh->S.nf, g.pbclass, g.qep ? 1 : 0, g.admits, g.admitted, g.t_admit, g.steps, g.t_step,
+                     g.steps ? (double)g.slot_steps / g.steps : 0.0);
+      }
     groups.clear();
   }
 };

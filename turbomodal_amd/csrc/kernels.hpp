@@ -123,7 +123,19 @@ struct Operator {
   size_t nblk = 0;
   DevBuf Kc, Kp, dKc, dKp, Mc, Mp, Gc, Gp;
   std::vector<double> u_static;       // 3*nn at unit omega
-  std::map<int, DevBuf> Mk_cache;     // harmonic -> complex M(k) blocks
+  // harmonic -> complex M(k) blocks, built on the stream of their first user;
+  // `ready` marks the end of the build for the other streams
+  struct MkEntry {
+    DevBuf b;
+    hipEvent_t ready = nullptr;
+    MkEntry() = default;
+    MkEntry(MkEntry&& o) noexcept : b(std::move(o.b)), ready(o.ready) { o.ready = nullptr; }
+    MkEntry(const MkEntry&) = delete;
+    ~MkEntry() {
+      if (ready) (void)hipEventDestroy(ready);
+    }
+  };
+  std::map<int, MkEntry> Mk_cache;
 };
 
 // ---- factorization (factor.hip) ------------------------------------------------
