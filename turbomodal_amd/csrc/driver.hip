@@ -2301,7 +2301,7 @@ class BatchLanczos {
         // |K||x|, |M||x| (and |D||x|) into the (consumed) W, MW (KW) buffers, pb
         // columns at a time; x is the b half for the Coriolis problem
         const double2* xs = Xb_.as<double2>() + (qep_ ? n_ : 0LL) + (long long)c0 * nvec_;
-        const dim3 ga((3 * S_.nf + 255) / 256, B_, cm);
+        const dim3 ga((S_.nf + 3) / 4, B_, (cm + 7) / 8);
         hipLaunchKernelGGL(k_spmv_abs, ga, dim3(256), 0, stream(), S_.nf, M_.d_rowptr.as<int>(),
                            M_.d_colidx.as<int>(), kptr_.as<const double2*>(), xs, ldk_, nvec_, dnc2.as<int>(),
                            Wb_.as<double2>(), ldw_);

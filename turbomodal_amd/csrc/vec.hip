@@ -626,25 +626,44 @@ __global__ void k_copy_cols(const double2* __restrict__ src, long long lds, cons
 }
 
 // Y_b[:, c] = |A_b| |X_b[:, c]| (entrywise moduli; real result in .x) for c < ncol_b
-__global__ void k_spmv_abs(int nf, const int* __restrict__ rowptr, const int* __restrict__ colidx,
-                           const double2* const* __restrict__ blocks, const double2* __restrict__ X,
-                           long long ldx, long long cs, const int* __restrict__ ncol, double2* __restrict__ Y,
-                           long long ldy) {
+__global__ __launch_bounds__(256) void k_spmv_abs(int nf, const int* __restrict__ rowptr,
+                                                  const int* __restrict__ colidx,
+                                                  const double2* const* __restrict__ blocks,
+                                                  const double2* __restrict__ X, long long ldx, long long cs,
+                                                  const int* __restrict__ ncol, double2* __restrict__ Y,
+                                                  long long ldy) {
+  // one wave per node row, lanes = 8 block slots x 8 columns (columns 8 blockIdx.z ..),
+  // as k_spmv_cols: the block row is read once for all the columns
   const int b = blockIdx.y;
-  const int c = blockIdx.z;
-  if (c >= ncol[b]) return;
-  const int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= 3 * nf) return;
-  const int I = row / 3, cc = row % 3;
+  const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (I >= nf) return;
+  const int lane = threadIdx.x & 63, c = 8 * blockIdx.z + (lane & 7), sl = lane >> 3;
+  const bool on = c < ncol[b];
   const double2* Mb = blocks[b];
   const double2* x = X + b * ldx + (long long)c * cs;
-  double acc = 0.0;
-  for (int s = rowptr[I]; s < rowptr[I + 1]; ++s) {
-    const int J = colidx[s];
-    const double2* blk = Mb + (long long)s * 9 + 3 * cc;
-    for (int q = 0; q < 3; ++q) acc += hypot(blk[q].x, blk[q].y) * hypot(x[3LL * J + q].x, x[3LL * J + q].y);
+  double acc[3] = {0.0, 0.0, 0.0};
+  const int s1 = rowptr[I + 1];
+  if (on)
+    for (int s = rowptr[I] + sl; s < s1; s += 8) {
+      const double2* blk = Mb + (long long)s * 9;
+      const long long J3 = 3LL * colidx[s];
+      double ax[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) ax[q] = hypot(x[J3 + q].x, x[J3 + q].y);
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[rr] = fma(hypot(blk[3 * rr + q].x, blk[3 * rr + q].y), ax[q], acc[rr]);
+    }
+#pragma unroll
+  for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+    for (int o = 8; o < 64; o <<= 1) acc[rr] += __shfl_xor(acc[rr], o, 64);
+  if (sl == 0 && on) {
+    double2* y = Y + b * ldy + (long long)c * cs + 3LL * I;
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) y[rr] = make_double2(acc[rr], 0.0);
   }
-  Y[b * ldy + (long long)c * cs + row] = make_double2(acc, 0.0);
 }
 
 // per column: sum_i (AK_i + |lam| AM_i)^2, or with AD: sum_i (AK_i + |lam| AD_i + lam^2 AM_i)^2
