@@ -1376,7 +1376,9 @@ class BatchLanczos {
         double worst = 0;
         for (int c = 0; c < nc[b]; ++c) worst = std::max(worst, hp->residual[c]);
         // block results are already Rayleigh-Ritz optimal on a large basis: refine
-        // only those left above the bar
+        // only those left above the bar.  (A pair accepted within 4 x its FP64
+        // floor estimate still gains: the estimate bounds the floor from above;
+        // on the C2 sweep refinement takes the largest residual from 1.2e-7 to 2e-8.)
         if (!qep_ && (worst > (block_ ? kBlockStall : kRefineTol) || !(worst == worst))) {
           nr[b] = nc[b];
           any = true;
