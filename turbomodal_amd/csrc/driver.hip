@@ -686,6 +686,12 @@ const double2* harmonic_mass(Context* ctx, Operator& op, int k) {
 // ---------------------------------------------------------------------------
 // batched Lanczos
 // ---------------------------------------------------------------------------
+// Vectors per block of the block method: 8 for problems of at most 6 wanted
+// pairs, else 16 (16 for all took as many sweeps on the C2 sweep, with more
+// restarts, and each sweep costs more)
+constexpr int kNarrowNev = 6;
+inline int block_width(int nev, bool qep) { return (nev <= kNarrowNev && !qep) ? 8 : 16; }
+
 struct HostProblem {
   tm_problem in;
   tm_result* out = nullptr;
@@ -801,7 +807,7 @@ class BatchLanczos {
     nchunk_ = static_cast<int>((n_ + kDotChunk - 1) / kDotChunk);
     // right-hand sides per triangular sweep: the Krylov block, and the Ritz
     // vectors of the refinement step (at most 16 per sweep)
-    const int pcols = std::min(16, std::max(block_ ? (nevmax_ <= 6 ? 8 : 16) : 1, nevmax_));
+    const int pcols = std::min(16, std::max(block_ ? block_width(nevmax_, false) : 1, nevmax_));
     bm_.alloc(pool(), P_, B_, pcols);
     act_.alloc(pool(), sizeof(int) * B_);
     // params and M(k)
@@ -1603,7 +1609,7 @@ class BatchLanczos {
   std::vector<std::vector<double>> prev_res_;
 
   void block_setup() {
-    pb_ = (nevmax_ <= 6 && !qep_) ? 8 : 16;
+    pb_ = block_width(nevmax_, qep_);
     mmax_ = pb_ == 8 ? 48 : 64;
     nkeep_ = mmax_ / 2;
     static_assert(kTsMaxRows >= 64, "tsmm coefficient rows");
@@ -1911,7 +1917,7 @@ class BatchLanczos {
     nevmax_ = nevmax;
     ncvmax_ = 1;
     nchunk_ = static_cast<int>((nvec_ + kDotChunk - 1) / kDotChunk);
-    const int pcols = std::min(16, std::max((nevmax_ <= 6 && !qep_) ? 8 : 16, nevmax_));
+    const int pcols = std::min(16, std::max(block_width(nevmax_, qep_), nevmax_));
     bm_.alloc(pool(), P_, B_, pcols);
     act_.alloc(pool(), sizeof(int) * B_);
     h_params_.assign(B_, ProbParam{1.0, 0.0, 0.0, 1.0});
@@ -2458,7 +2464,7 @@ static int64_t problem_footprint(const Operator& op, const tm_problem& p) {
   fill_policy(S, hp);
   const int64_t n = 3LL * S.nf;
   const bool block = use_block(S, hp.in);
-  const int pb = (hp.nev <= 6 && !hp.qep) ? 8 : 16;
+  const int pb = block_width(hp.nev, hp.qep);
   const int pcols = std::min(16, std::max(block ? pb : 1, hp.nev));
   int64_t bytes = 16 * (P.panel_entries + P.upool_entries + pcols * P.tvec_entries + P.inv_scratch_entries);
   if (block) {
@@ -2545,10 +2551,10 @@ static void solve_problems_impl(Context* ctx, const tm_problem* problems, int n,
     for (size_t i = 0; i < list.size();) {
       // one block size per slot group (Lanczos: lockstep batches)
       size_t j = i;
-      const int pb_i = list[i]->nev <= 6 ? 8 : 16;
+      const int pb_i = block_width(list[i]->nev, false);
       if (key.second >= 1) {
         const bool qep = key.second == 2;
-        while (j < list.size() && (qep || (list[j]->nev <= 6 ? 8 : 16) == pb_i)) ++j;
+        while (j < list.size() && (qep || block_width(list[j]->nev, false) == pb_i)) ++j;
         std::vector<HostProblem*> queue(list.begin() + i, list.begin() + j);
         BatchLanczos bl(ctx, *op, {}, true);
         const int cap = static_cast<int>(std::min(max_batch, queue.size()));
@@ -2819,7 +2825,7 @@ struct Server {
             small[op].push_back(hp);
             continue;
           }
-          const int pbc = (hp->nev <= 6 && !hp->qep) ? 8 : 16;
+          const int pbc = block_width(hp->nev, hp->qep);
           std::lock_guard<std::mutex> g(mu);
           auto& gp = groups[{op->mesh, pbc + (hp->qep ? 1000 : 0)}];
           if (!gp) {
