@@ -708,19 +708,39 @@ __global__ __launch_bounds__(256) void k_bwd_tile(SolveArgs sa, const RowTask* _
 #pragma unroll
   for (int u = 0; u < CPT; ++u) acc[u] = make_double2(0.0, 0.0);
   const int ib = stage == 0 ? w : j0, ie = stage == 0 ? m : w;
-  for (int i0 = ib; i0 < ie; i0 += TR) {
+  // a tile's loads are issued one tile ahead (registers), so they overlap the
+  // products of the previous tile
+  constexpr int APER = TR * 64 / 256, VPER = (TR * PM + 255) / 256;
+  double2 ra[APER], rv[VPER];
+  auto fetch = [&](int i0) {
     const int ni = min(TR, ie - i0);
-    __syncthreads();
-    for (int idx = tid; idx < TR * 64; idx += 64 * NP) {
-      const int ii = idx % TR, jc = idx / TR;
-      At[ii][jc] = (ii < ni && jc < nj) ? Pn[(i0 + ii) + (long long)(j0 + jc) * m] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int e = 0; e < APER; ++e) {
+      const int idx = tid + 256 * e, ii = idx % TR, jc = idx / TR;
+      ra[e] = (ii < ni && jc < nj) ? Pn[(i0 + ii) + (long long)(j0 + jc) * m] : make_double2(0.0, 0.0);
     }
-    for (int idx = tid; idx < TR * PM; idx += 64 * NP) {
-      const int ii = idx % TR, c = idx / TR;
+#pragma unroll
+    for (int e = 0; e < VPER; ++e) {
+      const int idx = tid + 256 * e, ii = idx % TR, c = idx / TR;
       const long long off = stage == 0 ? s.toff + (i0 + ii - w) : s.toff + r + (i0 + ii);
-      vs[ii][c] = (ii < ni && c < p) ? *tcol(sa, b, c, off) : make_double2(0.0, 0.0);
+      rv[e] = (idx < TR * PM && ii < ni && c < p) ? *tcol(sa, b, c, off) : make_double2(0.0, 0.0);
+    }
+  };
+  if (ib < ie) fetch(ib);
+  for (int i0 = ib; i0 < ie; i0 += TR) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < APER; ++e) {
+      const int idx = tid + 256 * e;
+      At[idx % TR][idx / TR] = ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < VPER; ++e) {
+      const int idx = tid + 256 * e;
+      if (idx < TR * PM) vs[idx % TR][idx / TR] = rv[e];
     }
     __syncthreads();
+    if (i0 + TR < ie) fetch(i0 + TR);
     // rows past ni are zero in At and vs; in stage 1 only rows strictly below
     // the diagonal (i0 + k > j) count
     const int kmin = stage == 1 ? max(0, min(TR, j - i0 + 1)) : 0;
