@@ -1201,8 +1201,7 @@ class BatchLanczos {
                  const DevBuf& kp) {
     (void)kp;
     DevBuf dnc = upload(pool(), nc, stream());
-    apply_A(X, ldx, dnc, nemax, KX);
-    apply_B(X, ldx, dnc, nemax, MX);
+    apply_AB(X, ldx, dnc, nemax, KX, MX);
     std::vector<double> lam((size_t)B_ * nemax, 0.0);
     for (int b = 0; b < B_; ++b)
       for (int c = 0; c < nc[b]; ++c) {
@@ -1690,26 +1689,17 @@ class BatchLanczos {
 
   // Y_b[:, c] (+)= A_b X_b[:, c], c < ncol_b (<= cmax), n rows; problems ld apart,
   // columns cs_ apart, for X and Y alike
+  // (cmul ncol_b columns, cstride apart: default cs_; cmul = 2 with cstride
+  // n_ treats the two halves of each 2n-long column as two columns)
   void spmv_cols(const DevBuf& blocks, const double2* X, long long ld, const DevBuf& ncol, int cmax, double2* Y,
-                 bool accumulate = false) {
+                 bool accumulate = false, long long cstride = -1, int cmul = 1) {
     KTimer t(ctx_, "spmv_m", 0, 0);
-    const long long cstride = cs_ > 0 ? cs_ : n_;
+    if (cstride < 0) cstride = cs_ > 0 ? cs_ : n_;
     for (int c0 = 0; c0 < cmax; c0 += 8) {
       const long long off = (long long)c0 * cstride;
-      std::vector<int> nc;
-      const DevBuf* dn = &ncol;
-      DevBuf shifted;
-      if (c0 > 0) {  // columns c0.. of each problem
-        std::vector<int> h(B_);
-        TM_HIP(hipMemcpyAsync(h.data(), ncol.p, sizeof(int) * B_, hipMemcpyDeviceToHost, stream()));
-        TM_HIP(hipStreamSynchronize(stream()));
-        for (int& x : h) x = std::max(0, x - c0);
-        shifted = ints(h);
-        dn = &shifted;
-      }
       hipLaunchKernelGGL(k_spmv_cols<8>, dim3((S_.nf + 3) / 4, B_), dim3(256), 0, stream(), S_.nf,
                          M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), blocks.as<const double2*>(), X + off, ld,
-                         cstride, dn->as<int>(), Y + off, ld, accumulate ? 1 : 0);
+                         cstride, ncol.as<int>(), cmul, c0, Y + off, ld, accumulate ? 1 : 0);
     }
   }
 
@@ -1728,6 +1718,25 @@ class BatchLanczos {
     spmv_cols(mk_ptrs_, X, ld, ncol, cmax, Y);
     if (qep_) spmv_cols(kptr_, X + n_, ld, ncol, cmax, Y + n_);
   }
+  // both products, A z into AX and B z into BX.  For the Coriolis pencil one
+  // pass of M over both halves gives M a and M b; then Bc z = [M b; M a - D b]
+  // and L2 z = [M a; K b] need only K b and D b besides.
+  void apply_AB(const double2* X, long long ld, const DevBuf& ncol, int cmax, double2* AX, double2* BX) {
+    if (!qep_) {
+      apply_A(X, ld, ncol, cmax, AX);
+      apply_B(X, ld, ncol, cmax, BX);
+      return;
+    }
+    spmv_cols(mk_ptrs_, X, ld, ncol, 2 * cmax, BX, false, n_, 2);  // [M a; M b]
+    {
+      KTimer t(ctx_, "vec", 0, 0);
+      hipLaunchKernelGGL(k_qep_swap_halves, dim3((n_ + 255) / 256, B_, cmax), dim3(256), 0, stream(), BX, AX, ld,
+                         n_, nvec_, ncol.as<int>());
+    }
+    spmv_cols(dptr_, X + n_, ld, ncol, cmax, AX + n_, true);  // + (-D) b
+    spmv_cols(kptr_, X + n_, ld, ncol, cmax, BX + n_);        // K b
+  }
+
   // A z: K x, or Bc z = [M b; M a - D b] (dptr_ holds -D)
   void apply_A(const double2* X, long long ld, const DevBuf& ncol, int cmax, double2* Y) {
     if (!qep_) {
@@ -1784,8 +1793,7 @@ class BatchLanczos {
       std::vector<int> pok(B_, 0);
       for (int b = 0; b < B_; ++b) pok[b] = ok[b] ? pb_ : 0;
       DevBuf dpok = ints(pok);
-      apply_B(MWb_.as<double2>(), ldw_, dpok, pb_, Wb_.as<double2>());
-      apply_A(MWb_.as<double2>(), ldw_, dpok, pb_, KWb_.as<double2>());
+      apply_AB(MWb_.as<double2>(), ldw_, dpok, pb_, KWb_.as<double2>(), Wb_.as<double2>());
       KTimer t(ctx_, "vec", 0, 0);
       hipLaunchKernelGGL(k_copy_cols, dim3((nvec_ + 255) / 256, B_), dim3(256), 0, stream(), Wb_.as<double2>(), ldw_,
                          (const int*)nullptr, MVb_.as<double2>(), ldv_, doff.as<int>(), pb_, nvec_, dok.as<int>());
@@ -2185,8 +2193,7 @@ class BatchLanczos {
       if (sel[b]) ncol[b] = c1 - c0;
     DevBuf dn = ints(ncol);
     const long long o = (long long)c0 * nvec_;
-    apply_A(Xb_.as<double2>() + o, ldk_, dn, c1 - c0, KXb_.as<double2>() + o);
-    apply_B(Xb_.as<double2>() + o, ldk_, dn, c1 - c0, MXb_.as<double2>() + o);
+    apply_AB(Xb_.as<double2>() + o, ldk_, dn, c1 - c0, KXb_.as<double2>() + o, MXb_.as<double2>() + o);
   }
 
   // Rayleigh-Ritz on span(V) for the problems near a full basis or restarted;

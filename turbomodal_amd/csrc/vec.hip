@@ -524,7 +524,7 @@ __global__ __launch_bounds__(256) void k_tsmm(const double2* __restrict__ A, lon
     }
 }
 
-// Y_b[:, c] (+)= A_b X_b[:, c] for c < ncol_b <= 8: one wave per node row.  Lane
+// Y_b[:, c] (+)= A_b X_b[:, c] for c < cmul ncol_b - c0 (X, Y point at column c0): one wave per node row.  Lane
 // l takes block slot l / 8 and column l % 8: the wave reads eight consecutive 3x3
 // blocks of the row (contiguous) per pass, each lane gathers the x triple of its
 // block column for its column, and the eight block slots are summed by
@@ -534,11 +534,11 @@ __global__ __launch_bounds__(256) void k_spmv_cols(int nf, const int* __restrict
                                                    const int* __restrict__ colidx,
                                                    const double2* const* __restrict__ blocks,
                                                    const double2* __restrict__ X, long long ldx, long long n,
-                                                   const int* __restrict__ ncol, double2* __restrict__ Y,
-                                                   long long ldy, int accumulate) {
+                                                   const int* __restrict__ ncol, int cmul, int c0,
+                                                   double2* __restrict__ Y, long long ldy, int accumulate) {
   static_assert(PB == 8, "eight columns per lane group");
   const int b = blockIdx.y;
-  const int nc = ncol[b];
+  const int nc = min(PB, cmul * ncol[b] - c0);
   if (nc <= 0) return;
   const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (I >= nf) return;  // whole waves leave together
@@ -573,7 +573,20 @@ __global__ __launch_bounds__(256) void k_spmv_cols(int nf, const int* __restrict
   }
 }
 template __global__ void k_spmv_cols<8>(int, const int*, const int*, const double2* const*, const double2*,
-                                        long long, long long, const int*, double2*, long long, int);
+                                        long long, long long, const int*, int, int, double2*, long long, int);
+
+// Coriolis pencil: from BX = [M a; M b] (per column), AX = [M b; M a] (the
+// -D b term is added after), c < ncol_b, n rows per half, columns cs apart
+__global__ void k_qep_swap_halves(const double2* __restrict__ BX, double2* __restrict__ AX, long long ld, long long n,
+                                  long long cs, const int* __restrict__ ncol) {
+  const int b = blockIdx.y, c = blockIdx.z;
+  if (c >= ncol[b]) return;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long o = b * ld + (long long)c * cs;
+  AX[o + i] = BX[o + n + i];
+  AX[o + n + i] = BX[o + i];
+}
 
 // dst_b[doff + i, d0_b + c] = alpha src_b[soff + i, s0_b + c] + beta dst (i < len, c < ncol),
 // columns `stride` apart: halves of the 2n-long vectors of the Coriolis pencil

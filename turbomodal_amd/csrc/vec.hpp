@@ -79,8 +79,10 @@ __global__ void k_copy_cols(const double2* src, long long lds, const int* s0v, d
                             const int* d0v, int ncol, long long n, const int* act);
 template <int PB>
 __global__ void k_spmv_cols(int nf, const int* rowptr, const int* colidx, const double2* const* blocks,
-                            const double2* X, long long ldx, long long n, const int* ncol, double2* Y, long long ldy,
-                            int accumulate);
+                            const double2* X, long long ldx, long long n, const int* ncol, int cmul, int c0, double2* Y,
+                            long long ldy, int accumulate);
+__global__ void k_qep_swap_halves(const double2* BX, double2* AX, long long ld, long long n, long long cs,
+                                  const int* ncol);
 __global__ void k_axpy_half(const double2* src, long long lds, const int* s0v, long long soff, double2* dst,
                             long long ldd, const int* d0v, long long doff, int ncol, long long len, long long stride,
                             double alpha, double beta, const int* act);
