@@ -48,6 +48,12 @@ void upload_mesh(Mesh& M, hipStream_t s) {
   M.d_rowptr = upload(S.rowptr, s);
   M.d_colidx = upload(S.colidx, s);
   M.d_tslot = upload(S.tslot, s);
+  {
+    std::vector<int> row(S.colidx.size());
+    for (int I = 0; I + 1 < (int)S.rowptr.size(); ++I)
+      for (int q = S.rowptr[I]; q < S.rowptr[I + 1]; ++q) row[q] = I;
+    M.d_slot_row = upload(row, s);
+  }
   M.d_kc_ptr = upload(S.kc_ptr, s);
   M.d_kc_src = upload(S.kc_src, s);
   M.d_kp_ptr = upload(S.kp_ptr, s);

@@ -670,8 +670,8 @@ const double2* harmonic_mass(Context* ctx, Operator& op, int k) {
   e.b.alloc(sizeof(double2) * 9 * op.nblk);
   DevBuf& b = e.b;
   double alpha = 2.0 * M_PI / S.N;
-  hipLaunchKernelGGL(k_build_phase_blocks, dim3((S.nf + 127) / 128), dim3(128), 0, ctx->stream, S.nf,
-                     M.d_rowptr.as<int>(), M.d_colidx.as<int>(), M.d_tslot.as<int>(),
+  hipLaunchKernelGGL(k_build_phase_blocks, dim3((9 * (long long)op.nblk + 255) / 256), dim3(256), 0, ctx->stream,
+                     (long long)op.nblk, M.d_slot_row.as<int>(), M.d_colidx.as<int>(), M.d_tslot.as<int>(),
                      M.d_pinned.as<unsigned char>(), op.Mc.as<double>(), op.Mp.as<double>(),
                      (const double*)nullptr, (const double*)nullptr, 0.0, std::cos(k * alpha),
                      std::sin(k * alpha), b.as<double2>(), 0, make_double2(1.0, 0.0));
@@ -1636,16 +1636,17 @@ class BatchLanczos {
       HostProblem* hp = probs_[b];
       const Operator& sop = *static_cast<const Operator*>(hp->in.op);
       if (!kblk_[b].p) kblk_[b].alloc(pool(), sizeof(double2) * 9 * op_.nblk);
-      hipLaunchKernelGGL(k_build_phase_blocks, dim3((S_.nf + 127) / 128), dim3(128), 0, stream(), S_.nf,
-                         M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), M_.d_tslot.as<int>(),
+      hipLaunchKernelGGL(k_build_phase_blocks, dim3((9 * (long long)op_.nblk + 255) / 256), dim3(256), 0, stream(),
+                         (long long)op_.nblk, M_.d_slot_row.as<int>(), M_.d_colidx.as<int>(), M_.d_tslot.as<int>(),
                          M_.d_pinned.as<unsigned char>(), sop.Kc.as<double>(), sop.Kp.as<double>(),
                          sop.dKc.as<double>(), sop.dKp.as<double>(), hp->s, hp->pre, hp->pim,
                          kblk_[b].as<double2>(), 0, make_double2(1.0, 0.0));
       kptr_host_[b] = kblk_[b].as<double2>();
       if (qep_) {  // -D(k) = -i 2 omega G(k), G skew: Gc + P Gp - conj(P) Gp^T
         if (!dblk_[b].p) dblk_[b].alloc(pool(), sizeof(double2) * 9 * op_.nblk);
-        hipLaunchKernelGGL(k_build_phase_blocks, dim3((S_.nf + 127) / 128), dim3(128), 0, stream(), S_.nf,
-                           M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), M_.d_tslot.as<int>(),
+        hipLaunchKernelGGL(k_build_phase_blocks, dim3((9 * (long long)op_.nblk + 255) / 256), dim3(256), 0,
+                           stream(), (long long)op_.nblk, M_.d_slot_row.as<int>(), M_.d_colidx.as<int>(),
+                           M_.d_tslot.as<int>(),
                            M_.d_pinned.as<unsigned char>(), sop.Gc.as<double>(), sop.Gp.as<double>(),
                            (const double*)nullptr, (const double*)nullptr, 0.0, hp->pre, hp->pim,
                            dblk_[b].as<double2>(), 1, make_double2(0.0, -2.0 * hp->omega));

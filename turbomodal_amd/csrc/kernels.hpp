@@ -108,7 +108,7 @@ struct Mesh {
   Context* ctx = nullptr;
   Symbolic S;
   Plan plan;
-  DevBuf d_coords, d_elements, d_rowptr, d_colidx, d_tslot;
+  DevBuf d_coords, d_elements, d_rowptr, d_colidx, d_tslot, d_slot_row;  // slot_row: block row of each slot
   DevBuf d_kc_ptr, d_kc_src, d_kp_ptr, d_kp_src, d_f_ptr, d_f_src;
   DevBuf d_pinned, d_free_of_full, d_node_class;
   DevBuf d_struct, d_relmap, d_asm_slot, d_asm_pos;

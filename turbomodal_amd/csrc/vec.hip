@@ -257,34 +257,32 @@ __global__ void k_combine(const double2* __restrict__ V, long long ldb, long lon
 }
 
 // M(k) blocks: M_c + P M_p + conj(P) M_p^T(t), pinned rows/cols zero
-__global__ void k_build_phase_blocks(int nf, const int* __restrict__ rowptr,
+__global__ void k_build_phase_blocks(long long nslot, const int* __restrict__ slot_row,
                                      const int* __restrict__ colidx, const int* __restrict__ tslot,
                                      const unsigned char* __restrict__ pinned,
                                      const double* __restrict__ Xc, const double* __restrict__ Xp,
                                      const double* __restrict__ dXc, const double* __restrict__ dXp,
                                      double s, double pre, double pim, double2* __restrict__ out,
                                      int skew, double2 scale) {
-  const int I = blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= nf) return;
-  for (int sl = rowptr[I]; sl < rowptr[I + 1]; ++sl) {
-    const int J = colidx[sl];
-    const long long tt = tslot[sl];
-    for (int ci = 0; ci < 3; ++ci)
-      for (int cj = 0; cj < 3; ++cj) {
-        const long long e1 = (long long)sl * 9 + 3 * ci + cj, e2 = tt * 9 + 3 * cj + ci;
-        double c = Xc[e1], p = Xp[e1], q = Xp[e2];
-        if (dXc) {
-          c += s * dXc[e1];
-          p += s * dXp[e1];
-          q += s * dXp[e2];
-        }
-        // symmetric X: Xc + P Xp + conj(P) Xp^T; skew X: Xc + P Xp - conj(P) Xp^T
-        double2 v = skew ? make_double2(c + pre * (p - q), pim * (p + q)) : make_double2(c + pre * (p + q), pim * (p - q));
-        v = cmul(scale, v);
-        if (pinned[3 * I + ci] || pinned[3 * J + cj]) v = make_double2(0.0, 0.0);
-        out[(long long)sl * 9 + 3 * ci + cj] = v;
-      }
+  // one thread per entry of a 3x3 slot block (coalesced reads and writes)
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= 9 * nslot) return;
+  const long long sl = idx / 9;
+  const int e = (int)(idx - 9 * sl), ci = e / 3, cj = e % 3;
+  const int I = slot_row[sl], J = colidx[sl];
+  const long long tt = tslot[sl];
+  const long long e1 = idx, e2 = tt * 9 + 3 * cj + ci;
+  double c = Xc[e1], p = Xp[e1], q = Xp[e2];
+  if (dXc) {
+    c += s * dXc[e1];
+    p += s * dXp[e1];
+    q += s * dXp[e2];
   }
+  // symmetric X: Xc + P Xp + conj(P) Xp^T; skew X: Xc + P Xp - conj(P) Xp^T
+  double2 v = skew ? make_double2(c + pre * (p - q), pim * (p + q)) : make_double2(c + pre * (p + q), pim * (p - q));
+  v = cmul(scale, v);
+  if (pinned[3 * I + ci] || pinned[3 * J + cj]) v = make_double2(0.0, 0.0);
+  out[idx] = v;
 }
 
 // multi-column SpMV with one complex block array: Y[:, c] = A X[:, c], c < ncol_b

@@ -44,7 +44,7 @@ __global__ void k_normalize_next(LzState st, long long n);
 __global__ void k_advance(LzState st, int B);
 __global__ void k_combine(const double2* V, long long ldb, long long n, const double* C, int ldc,
                           long long cstride, const int* nk, const int* nc, double2* out, long long ldo);
-__global__ void k_build_phase_blocks(int nf, const int* rowptr, const int* colidx, const int* tslot,
+__global__ void k_build_phase_blocks(long long nslot, const int* slot_row, const int* colidx, const int* tslot,
                                      const unsigned char* pinned, const double* Xc, const double* Xp,
                                      const double* dXc, const double* dXp, double s, double pre, double pim,
                                      double2* out, int skew, double2 scale);
