@@ -1694,13 +1694,25 @@ class BatchLanczos {
   // n_ treats the two halves of each 2n-long column as two columns)
   void spmv_cols(const DevBuf& blocks, const double2* X, long long ld, const DevBuf& ncol, int cmax, double2* Y,
                  bool accumulate = false, long long cstride = -1, int cmul = 1) {
+    spmv_pair(blocks, nullptr, X, ld, ncol, cmax, Y, nullptr, accumulate, false, cstride, cmul);
+  }
+  // both Y1 (+)= A1 X and Y2 (+)= A2 X (blocks2 optional) from one pass over X
+  void spmv_pair(const DevBuf& blocks1, const DevBuf* blocks2, const double2* X, long long ld, const DevBuf& ncol,
+                 int cmax, double2* Y1, double2* Y2, bool acc1, bool acc2, long long cstride = -1, int cmul = 1) {
     KTimer t(ctx_, "spmv_m", 0, 0);
     if (cstride < 0) cstride = cs_ > 0 ? cs_ : n_;
-    for (int c0 = 0; c0 < cmax; c0 += 8) {
-      const long long off = (long long)c0 * cstride;
-      hipLaunchKernelGGL(k_spmv_cols<8>, dim3((S_.nf + 3) / 4, B_), dim3(256), 0, stream(), S_.nf,
-                         M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), blocks.as<const double2*>(), X + off, ld,
-                         cstride, ncol.as<int>(), cmul, c0, Y + off, ld, accumulate ? 1 : 0);
+    const long long ntask = (long long)((S_.nf + 3) / 4) * B_;
+    if (ntask > std::numeric_limits<int>::max()) throw ArgError("spmv: too many (row quad, problem) tasks for one launch");
+    const double2* const* b2 = blocks2 ? blocks2->as<const double2*>() : nullptr;
+    for (int c0 = 0; c0 < cmax; c0 += 16) {  // cmax: the most columns (cmul ncol_b) of any problem
+      if (cmax - c0 > 8)
+        hipLaunchKernelGGL(k_spmv_pair<2>, dim3((unsigned)ntask), dim3(256), 0, stream(), S_.nf, B_,
+                           M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), blocks1.as<const double2*>(), b2, X, ld,
+                           cstride, ncol.as<int>(), cmul, c0, Y1, Y2, ld, acc1 ? 1 : 0, acc2 ? 1 : 0);
+      else
+        hipLaunchKernelGGL(k_spmv_pair<1>, dim3((unsigned)ntask), dim3(256), 0, stream(), S_.nf, B_,
+                           M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), blocks1.as<const double2*>(), b2, X, ld,
+                           cstride, ncol.as<int>(), cmul, c0, Y1, Y2, ld, acc1 ? 1 : 0, acc2 ? 1 : 0);
     }
   }
 
@@ -1724,8 +1736,7 @@ class BatchLanczos {
   // and L2 z = [M a; K b] need only K b and D b besides.
   void apply_AB(const double2* X, long long ld, const DevBuf& ncol, int cmax, double2* AX, double2* BX) {
     if (!qep_) {
-      apply_A(X, ld, ncol, cmax, AX);
-      apply_B(X, ld, ncol, cmax, BX);
+      spmv_pair(kptr_, &mk_ptrs_, X, ld, ncol, cmax, AX, BX, false, false);
       return;
     }
     spmv_cols(mk_ptrs_, X, ld, ncol, 2 * cmax, BX, false, n_, 2);  // [M a; M b]
@@ -1734,8 +1745,7 @@ class BatchLanczos {
       hipLaunchKernelGGL(k_qep_swap_halves, dim3((n_ + 255) / 256, B_, cmax), dim3(256), 0, stream(), BX, AX, ld,
                          n_, nvec_, ncol.as<int>());
     }
-    spmv_cols(dptr_, X + n_, ld, ncol, cmax, AX + n_, true);  // + (-D) b
-    spmv_cols(kptr_, X + n_, ld, ncol, cmax, BX + n_);        // K b
+    spmv_pair(kptr_, &dptr_, X + n_, ld, ncol, cmax, BX + n_, AX + n_, false, true);  // K b; + (-D) b
   }
 
   // A z: K x, or Bc z = [M b; M a - D b] (dptr_ holds -D)
@@ -1768,8 +1778,10 @@ class BatchLanczos {
         std::vector<double2> C = tsgemm(MVb_.as<double2>(), ldv_, dnv, nvmax, Wb_.as<double2>(), ldw_, pb_, dact);
         tsmm(Vb_.as<double2>(), ldv_, dnv, C, pb_, -1.0, 1.0, Wb_.as<double2>(), ldw_, nullptr, dact);
       }
-    apply_B(Wb_.as<double2>(), ldw_, dpb, pb_, MWb_.as<double2>());
-    if (!qep_) apply_A(Wb_.as<double2>(), ldw_, dpb, pb_, KWb_.as<double2>());
+    if (!qep_)
+      apply_AB(Wb_.as<double2>(), ldw_, dpb, pb_, KWb_.as<double2>(), MWb_.as<double2>());
+    else
+      apply_B(Wb_.as<double2>(), ldw_, dpb, pb_, MWb_.as<double2>());
     std::vector<double2> G = tsgemm(Wb_.as<double2>(), ldw_, dpb, pb_, MWb_.as<double2>(), ldw_, pb_, dact);
     std::vector<double2> Rinv((size_t)B_ * mmax_ * 16, make_double2(0.0, 0.0));
     std::vector<int> bad, ok(act);

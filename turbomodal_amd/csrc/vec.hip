@@ -522,56 +522,109 @@ __global__ __launch_bounds__(256) void k_tsmm(const double2* __restrict__ A, lon
     }
 }
 
-// Y_b[:, c] (+)= A_b X_b[:, c] for c < cmul ncol_b - c0 (X, Y point at column c0): one wave per node row.  Lane
-// l takes block slot l / 8 and column l % 8: the wave reads eight consecutive 3x3
-// blocks of the row (contiguous) per pass, each lane gathers the x triple of its
-// block column for its column, and the eight block slots are summed by
-// shuffles.  Columns are n apart in X and Y, problems ldx / ldy apart.
-template <int PB>
-__global__ __launch_bounds__(256) void k_spmv_cols(int nf, const int* __restrict__ rowptr,
+// Work-groups are dealt round-robin over the eight XCDs (each with its own L2):
+// block x takes task xcd_task(x, n) of n, so every XCD runs a contiguous run of
+// the task list.
+__device__ __forceinline__ int xcd_task(int x, int n) {
+  const int c = x & 7, q = n >> 3, rem = n & 7;
+  return c * q + min(c, rem) + (x >> 3);
+}
+
+// Multi-column SpMV with one or two complex block arrays over the same columns:
+// Y1_b[:, c] (+)= A1_b X_b[:, c] and, when blocks2 is given, Y2_b[:, c] (+)= A2_b X_b[:, c],
+// for c0 <= c < min(c0 + 8 CPL, cmul ncol_b); column c at offset c n.  One wave per
+// node row; lane = (slot lane 0..7, column group 0..7): eight slots of the row in
+// flight, each lane holding CPL columns eight apart, so a block entry loaded by a
+// lane serves CPL columns (and both products share the x loads).  Tasks are
+// (row quad, problem), problem-fastest, each XCD taking a contiguous run: the
+// problems of a row quad run together on one XCD, and those that share a block
+// array (M(k) of one harmonic) find it in that XCD's L2.
+template <int CPL>
+__global__ __launch_bounds__(256) void k_spmv_pair(int nf, int B, const int* __restrict__ rowptr,
                                                    const int* __restrict__ colidx,
-                                                   const double2* const* __restrict__ blocks,
+                                                   const double2* const* __restrict__ blocks1,
+                                                   const double2* const* __restrict__ blocks2,
                                                    const double2* __restrict__ X, long long ldx, long long n,
                                                    const int* __restrict__ ncol, int cmul, int c0,
-                                                   double2* __restrict__ Y, long long ldy, int accumulate) {
-  static_assert(PB == 8, "eight columns per lane group");
-  const int b = blockIdx.y;
-  const int nc = min(PB, cmul * ncol[b] - c0);
-  if (nc <= 0) return;
-  const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (I >= nf) return;  // whole waves leave together
-  const int lane = threadIdx.x & 63, c = lane & 7, sl = lane >> 3;
-  const double2* Mb = blocks[b];
-  const double2* x = X + b * ldx + (long long)c * n;
-  double2 acc[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
-  const int s1 = rowptr[I + 1];
-  if (c < nc)
-    for (int s = rowptr[I] + sl; s < s1; s += 8) {
-      const double2* blk = Mb + (long long)s * 9;
-      const long long J3 = 3LL * colidx[s];
-      const double2 x0 = x[J3], x1 = x[J3 + 1], x2 = x[J3 + 2];
+                                                   double2* __restrict__ Y1, double2* __restrict__ Y2, long long ldy,
+                                                   int acc1, int acc2) {
+  const int t = xcd_task(blockIdx.x, gridDim.x);
+  const int b = t % B, I = (t / B) * 4 + (threadIdx.x >> 6);
+  const int nc = min(8 * CPL, cmul * ncol[b] - c0);
+  if (nc <= 0 || I >= nf) return;  // whole waves leave together
+  const int lane = threadIdx.x & 63, cg = lane & 7, sl = lane >> 3;
+  const double2* A1 = blocks1[b];
+  const double2* A2 = blocks2 ? blocks2[b] : nullptr;
+  const double2* x = X + b * ldx + (long long)(c0 + cg) * n;
+  double2 a1[CPL][3], a2[CPL][3];
 #pragma unroll
-      for (int rr = 0; rr < 3; ++rr) {
-        cmac(acc[rr], blk[3 * rr], x0);
-        cmac(acc[rr], blk[3 * rr + 1], x1);
-        cmac(acc[rr], blk[3 * rr + 2], x2);
+  for (int j = 0; j < CPL; ++j)
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) a1[j][rr] = a2[j][rr] = make_double2(0.0, 0.0);
+  const int s1 = rowptr[I + 1];
+  if (cg < nc)
+    for (int s = rowptr[I] + sl; s < s1; s += 8) {
+      const long long J3 = 3LL * colidx[s];
+      double2 xv[CPL][3];
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) {
+        const bool on = cg + 8 * j < nc;
+        const double2* xj = x + (long long)(8 * j) * n + J3;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) xv[j][k] = on ? xj[k] : make_double2(0.0, 0.0);
+      }
+      const double2* e1 = A1 + (long long)s * 9;
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const double2 e = e1[3 * rr + k];
+#pragma unroll
+          for (int j = 0; j < CPL; ++j) cmac(a1[j][rr], e, xv[j][k]);
+        }
+      if (A2) {
+        const double2* e2 = A2 + (long long)s * 9;
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const double2 e = e2[3 * rr + k];
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) cmac(a2[j][rr], e, xv[j][k]);
+          }
       }
     }
 #pragma unroll
-  for (int rr = 0; rr < 3; ++rr)
+  for (int j = 0; j < CPL; ++j)
 #pragma unroll
-    for (int o = 8; o < 64; o <<= 1) {
-      acc[rr].x += __shfl_xor(acc[rr].x, o, 64);
-      acc[rr].y += __shfl_xor(acc[rr].y, o, 64);
-    }
-  if (sl == 0 && c < nc) {
-    double2* y = Y + b * ldy + (long long)c * n + 3LL * I;
+    for (int rr = 0; rr < 3; ++rr)
 #pragma unroll
-    for (int rr = 0; rr < 3; ++rr) y[rr] = accumulate ? cadd(y[rr], acc[rr]) : acc[rr];
+      for (int o = 8; o < 64; o <<= 1) {
+        a1[j][rr].x += __shfl_xor(a1[j][rr].x, o, 64);
+        a1[j][rr].y += __shfl_xor(a1[j][rr].y, o, 64);
+        if (A2) {
+          a2[j][rr].x += __shfl_xor(a2[j][rr].x, o, 64);
+          a2[j][rr].y += __shfl_xor(a2[j][rr].y, o, 64);
+        }
+      }
+  if (sl != 0) return;
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+    if (cg + 8 * j >= nc) break;
+    const long long o = b * ldy + (long long)(c0 + cg + 8 * j) * n + 3LL * I;
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) Y1[o + rr] = acc1 ? cadd(Y1[o + rr], a1[j][rr]) : a1[j][rr];
+    if (A2)
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr) Y2[o + rr] = acc2 ? cadd(Y2[o + rr], a2[j][rr]) : a2[j][rr];
   }
 }
-template __global__ void k_spmv_cols<8>(int, const int*, const int*, const double2* const*, const double2*,
-                                        long long, long long, const int*, int, int, double2*, long long, int);
+template __global__ void k_spmv_pair<1>(int, int, const int*, const int*, const double2* const*,
+                                        const double2* const*, const double2*, long long, long long, const int*, int,
+                                        int, double2*, double2*, long long, int, int);
+template __global__ void k_spmv_pair<2>(int, int, const int*, const int*, const double2* const*,
+                                        const double2* const*, const double2*, long long, long long, const int*, int,
+                                        int, double2*, double2*, long long, int, int);
 
 // Coriolis pencil: from BX = [M a; M b] (per column), AX = [M b; M a] (the
 // -D b term is added after), c < ncol_b, n rows per half, columns cs apart
@@ -644,7 +697,7 @@ __global__ __launch_bounds__(256) void k_spmv_abs(int nf, const int* __restrict_
                                                   const int* __restrict__ ncol, double2* __restrict__ Y,
                                                   long long ldy) {
   // one wave per node row, lanes = 8 block slots x 8 columns (columns 8 blockIdx.z ..),
-  // as k_spmv_cols: the block row is read once for all the columns
+  // as k_spmv_pair: the block row is read once for all the columns
   const int b = blockIdx.y;
   const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (I >= nf) return;

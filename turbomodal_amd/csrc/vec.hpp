@@ -77,10 +77,11 @@ __global__ void k_tsmm(const double2* A, long long lda_b, long long n, long long
                        const int* yoff);
 __global__ void k_copy_cols(const double2* src, long long lds, const int* s0v, double2* dst, long long ldd,
                             const int* d0v, int ncol, long long n, const int* act);
-template <int PB>
-__global__ void k_spmv_cols(int nf, const int* rowptr, const int* colidx, const double2* const* blocks,
-                            const double2* X, long long ldx, long long n, const int* ncol, int cmul, int c0, double2* Y,
-                            long long ldy, int accumulate);
+template <int CPL>
+__global__ void k_spmv_pair(int nf, int B, const int* rowptr, const int* colidx, const double2* const* blocks1,
+                            const double2* const* blocks2, const double2* X, long long ldx, long long n,
+                            const int* ncol, int cmul, int c0, double2* Y1, double2* Y2, long long ldy, int acc1,
+                            int acc2);
 __global__ void k_qep_swap_halves(const double2* BX, double2* AX, long long ld, long long n, long long cs,
                                   const int* ncol);
 __global__ void k_axpy_half(const double2* src, long long lds, const int* s0v, long long soff, double2* dst,
