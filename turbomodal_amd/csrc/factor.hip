@@ -9,10 +9,12 @@
 // matrix (consumed by the parent).  A tree level is one set of launches over
 // all fronts of that level and all problems of the batch (grid.y = problem):
 //
-//   k_front_assemble  zero the front, add A(k) blocks (formed on the fly from
-//                     the phase-split operator arrays: K_c + s dK_c - sigma M_c
-//                     + P(.)_p + conj(P)(.)_p^T), extend-add the children's
-//                     update matrices (children in fixed order -> deterministic)
+//   k_front_assemble  each front entry written once with the children's update
+//                     entries that land on it (pulled through inverse relmaps,
+//                     children in fixed order -> deterministic), then the A(k)
+//                     blocks added (formed on the fly from the phase-split
+//                     operator arrays: K_c + s dK_c - sigma M_c + P(.)_p +
+//                     conj(P)(.)_p^T)
 //   k_diag_ldl        unblocked LDL^H of a 32-wide diagonal block in LDS
 //   k_panel_trsm      L21 = F21 L11^-H D^-1, one row per thread
 //   k_schur_tile      trailing update F22 -= L21 D L21^H on 64x64 tiles
@@ -73,6 +75,7 @@ struct FactorArgs {
   const DevSn* sn;
   const int* struct_nodes;
   const int* relmap;
+  const int* invmap;
   const int* asm_slot;
   const int* asm_pos;
   const int* tslot;
@@ -98,14 +101,42 @@ __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const Asm
   double2* Un = fa.U + b * fa.UE + s.uoff;
   const int tid = threadIdx.x;
   const int j0 = 3 * t.n0, j1 = 3 * t.n1;
-  {  // zero the lower part of columns [j0, j1): (row, column) pairs over all threads
-    const int nrow = m - j0, tot = nrow * (j1 - j0);  // < 2^31: m < 65536, j1 - j0 <= 48
-    for (int idx = tid; idx < tot; idx += 256) {
-      const int i = j0 + idx % nrow, j = j0 + idx / nrow;
-      if (i >= j) *front_at(Pn, Un, m, w, r, i, j) = make_double2(0.0, 0.0);
+  // children's update entries pulled through the inverse relmaps: every lower
+  // entry of columns [j0, j1) is written once, child0's term first
+  const int* inv[2] = {nullptr, nullptr};
+  const double2* Uc[2] = {nullptr, nullptr};
+  int rc[2] = {0, 0};
+  {
+    int o = s.inv0;
+    for (int k = 0; k < 2; ++k) {
+      const int c = k == 0 ? s.child0 : s.child1;
+      if (c < 0) continue;
+      const DevSn cs = fa.sn[c];
+      inv[k] = fa.invmap + o;
+      o += s.ncol + s.nstr;
+      Uc[k] = fa.U + b * fa.UE + cs.uoff;
+      rc[k] = 3 * cs.nstr;
+    }
+  }
+  for (int j = j0; j < j1; ++j) {
+    int qj[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) qj[k] = inv[k] ? inv[k][j / 3] : -1;
+    const int ioff = j < w ? 0 : w;  // row i of column j at col[i - ioff]
+    double2* col = j < w ? Pn + (long long)j * m : Un + (long long)(j - w) * r;
+    for (int i = j + tid; i < m; i += 256) {
+      double2 v = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (qj[k] < 0) continue;
+        const int qi = inv[k][i / 3];
+        if (qi >= 0) v = cadd(v, Uc[k][(3 * qi + i % 3) + (long long)(3 * qj[k] + j % 3) * rc[k]]);
+      }
+      col[i - ioff] = v;
     }
   }
   __syncthreads();
+  // A(k) entries added (pinned DOFs: identity rows and columns)
   const ProbParam pp = fa.params[b];
   const int nent = (t.a1 - t.a0) * 9;
   for (int idx = tid; idx < nent; idx += 256) {
@@ -116,9 +147,9 @@ __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const Asm
     if (gi < gj) continue;
     const int rnode = rowpos < s.ncol ? s.col0 + rowpos : fa.struct_nodes[s.str0 + rowpos - s.ncol];
     const int cnode = s.col0 + colpos;
-    double2 v;
+    double2* dst = front_at(Pn, Un, m, w, r, gi, gj);
     if (fa.pinned[3 * rnode + ci] || fa.pinned[3 * cnode + cj]) {
-      v = make_double2(gi == gj ? 1.0 : 0.0, 0.0);
+      *dst = make_double2(gi == gj ? 1.0 : 0.0, 0.0);
     } else {
       const long long sl = fa.asm_slot[q];
       const long long tt = fa.tslot[sl];
@@ -128,29 +159,8 @@ __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const Asm
       double c = op.Kc[e1] + pp.s * op.dKc[e1] - pp.sigma * op.Mc[e1];
       double p = op.Kp[e1] + pp.s * op.dKp[e1] - pp.sigma * op.Mp[e1];
       double qv = op.Kp[e2] + pp.s * op.dKp[e2] - pp.sigma * op.Mp[e2];
-      v = make_double2(c + pp.pre * (p + qv), pp.pim * (p - qv));
+      *dst = cadd(*dst, make_double2(c + pp.pre * (p + qv), pp.pim * (p - qv)));
     }
-    *front_at(Pn, Un, m, w, r, gi, gj) = v;
-  }
-  __syncthreads();
-  for (int k = 0; k < 2; ++k) {
-    const int c = k == 0 ? s.child0 : s.child1;
-    if (c < 0) continue;
-    const DevSn cs = fa.sn[c];
-    const int rc = 3 * cs.nstr;
-    const double2* Uc = fa.U + b * fa.UE + cs.uoff;
-    const int* rm = fa.relmap + cs.rel0;
-    // extend-add of the child's update columns [jlo, jhi): (row, column) pairs
-    // over all threads; each front entry has one writer per child
-    const int jlo = 3 * t.clo[k], jhi = 3 * t.chi[k], nrow = rc - jlo, tot = nrow * (jhi - jlo);
-    for (int idx = tid; idx < tot; idx += 256) {
-      const int ic = jlo + idx % nrow, jc = jlo + idx / nrow;
-      if (ic < jc) continue;
-      const int gi = 3 * rm[ic / 3] + ic % 3, gj = 3 * rm[jc / 3] + jc % 3;
-      double2* dst = front_at(Pn, Un, m, w, r, gi, gj);
-      *dst = cadd(*dst, Uc[ic + (long long)jc * rc]);
-    }
-    __syncthreads();
   }
 }
 
@@ -797,6 +807,7 @@ static FactorArgs factor_args(const Mesh& M, const Plan& P, const OpArrays& ops,
   fa.sn = P.d_sn.as<DevSn>();
   fa.struct_nodes = M.d_struct.as<int>();
   fa.relmap = M.d_relmap.as<int>();
+  fa.invmap = P.d_invmap.as<int>();
   fa.asm_slot = M.d_asm_slot.as<int>();
   fa.asm_pos = M.d_asm_pos.as<int>();
   fa.tslot = M.d_tslot.as<int>();

@@ -90,12 +90,14 @@ struct Plan {
   DevBuf d_asm, d_diag, d_row, d_tile, d_level_sn, d_finv, d_fup, d_bcol, d_inv_diag, d_inv_tiles;
   // per-snode device data
   DevBuf d_sn;             // DevSn[]
+  DevBuf d_invmap;         // per front and child: front node position -> child update node, or -1
   int64_t panel_entries = 0, upool_entries = 0, tvec_entries = 0;
   double trisolve_bytes_c16 = 0;  // algorithmic bytes of one L/D/L^H pair (complex fp64)
 };
 
 struct DevSn {
-  int col0, ncol, nstr, str0, rel0, child0, child1, pad;
+  int col0, ncol, nstr, str0, rel0, child0, child1;
+  int inv0;  // offset into invmap: per child, front node -> child update node (-1: none)
   long long poff, uoff;
   long long toff;   // per column: [r update-vector entries][w pivot temp entries]
   long long soff;   // inversion scratch (w x 64) within its level

@@ -6,6 +6,8 @@
 // mesh; every numeric factorization and solve of every problem that shares
 // the mesh replays it.
 #include <algorithm>
+#include <limits>
+#include <stdexcept>
 
 #include "kernels.hpp"
 
@@ -72,13 +74,31 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
     d.rel0 = s.rel0;
     d.child0 = s.child[0];
     d.child1 = s.child[1];
-    d.pad = 0;
+    d.inv0 = 0;
     d.poff = poff;
     poff += int64_t(s.m()) * s.w();
     d.toff = toff;
     toff += s.r() + s.w();
     d.soff = 0;
   }
+  // inverse relmaps: for each child of a front (child0's map, then child1's),
+  // the child's update node at each front node position, or -1
+  std::vector<int> invmap;
+  for (int id = 0; id < nsn; ++id) {
+    const Snode& s = S.sn[id];
+    dsn[id].inv0 = static_cast<int>(invmap.size());
+    const int nodes = s.ncol + s.nstr;
+    for (int k = 0; k < 2; ++k) {
+      const int c = s.child[k];
+      if (c < 0) continue;
+      const size_t base = invmap.size();
+      invmap.resize(base + nodes, -1);
+      const Snode& cs = S.sn[c];
+      for (int q = 0; q < cs.nstr; ++q) invmap[base + S.relmap[cs.rel0 + q]] = q;
+    }
+    if (invmap.size() > (size_t)std::numeric_limits<int>::max()) throw std::length_error("invmap");
+  }
+  if (invmap.empty()) invmap.push_back(-1);
   // inversion scratch: w x 128 per front (X block column | Z = L X_JJ), laid out per level
   int64_t scratch = 0;
   for (const auto& lvl : S.levels) {
@@ -240,6 +260,7 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
   }
   P.trisolve_bytes_c16 = tri_bytes + 5.0 * 3 * S.nf * 16.0;
   P.d_sn = upload(dsn, stream);
+  P.d_invmap = upload(invmap, stream);
   P.d_asm = upload(P.asm_tasks, stream);
   P.d_diag = upload(P.diag_tasks, stream);
   P.d_row = upload(P.row_tasks, stream);
