@@ -207,7 +207,12 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64"])
     ap.add_argument("--method", default="block", choices=["block", "lanczos"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serial-groups", action="store_true",
+                    help="run the block-size slot groups of a sweep one after another instead of side by side "
+                         "(every sweep; the kernels then run alone on the GPU, as in the roofline sweep)")
     args = ap.parse_args()
+    if args.serial_groups:
+        os.environ["TMHIP_SERIAL_GROUPS"] = "1"
     if args.gpus != 1:
         print(f"bench.py: this build measures one MI355X; --gpus {args.gpus} is not supported "
               "(the (rpm, harmonic) problems would shard across GPUs with no collectives)",
@@ -258,17 +263,45 @@ def main():
     value = n_problems / (elapsed / args.steps)
     max_res = max(float(np.max(r.residuals)) for row in results for r in row)
 
+    # The timed steps run a sweep's block-size slot groups side by side on their own
+    # streams, so a launch's event interval there also holds the other group's
+    # kernels.  The per-kernel rates come from one more sweep, right after, with the
+    # groups one after another (each kernel alone on the GPU); the same rate over
+    # the timed steps is reported beside it.
+    kstats = stats
+    if not args.serial_groups:
+        os.environ["TMHIP_SERIAL_GROUPS"] = "1"
+        core.engine_set_profiling(True)
+        core.engine_reset_kernel_stats()
+        t = time.perf_counter()
+        solver.solve_rpm_sweep(list(rpms), num_modes, include_coriolis=args.coriolis)
+        print(f"[bench] roofline sweep (groups serial) {time.perf_counter() - t:.2f}s", file=sys.stderr,
+              flush=True)
+        kstats = core.engine_kernel_stats()
+        core.engine_set_profiling(False)
+        del os.environ["TMHIP_SERIAL_GROUPS"]
+
+    def tri_rate(st):
+        tr = st.get("trisolve", {})
+        if not tr.get("launches"):
+            return None
+        avg = tr["total_ms"] / tr["launches"]
+        per = tr["alg_bytes"] / tr["launches"]
+        return avg, per, per / (avg * 1e-3) / 1e9
+
     tri = stats.get("trisolve", {})
-    fac = stats.get("factor", {})
+    fac = kstats.get("factor", {})
     roofline = None
-    if tri.get("launches"):
-        avg_ms = tri["total_ms"] / tri["launches"]
-        bytes_per = tri["alg_bytes"] / tri["launches"]
-        ach = bytes_per / (avg_ms * 1e-3) / 1e9
+    if tri_rate(kstats):
+        avg_ms, bytes_per, ach = tri_rate(kstats)
         roofline = {"bound": "hbm", "kernel": "trisolve (L, D, L^H sweep pair)", "achieved": round(ach, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                     "traffic": measured_traffic(args), "avg_launch_ms": round(avg_ms, 3),
-                    "alg_bytes_per_launch": bytes_per}
+                    "alg_bytes_per_launch": bytes_per,
+                    "measured_on": ("the timed steps (groups serial)" if args.serial_groups else
+                                    "one sweep after the timed steps with the block-size groups serial")}
+        if not args.serial_groups and tri_rate(stats):
+            roofline["frac_timed_region_groups_concurrent"] = round(tri_rate(stats)[2] / HBM_PEAK_GBS, 4)
     factor_rate = None
     if fac.get("launches"):
         factor_rate = fac["alg_flops"] / (fac["total_ms"] * 1e-3) / 1e12

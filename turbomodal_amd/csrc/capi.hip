@@ -98,6 +98,7 @@ void tm_context_destroy(tm_context* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  ctx->groups.clear();  // the slot groups' contexts
   ctx->deliver.reset();  // drains its copies
   for (auto& p : ctx->pending) {
     (void)hipEventDestroy(p.a);

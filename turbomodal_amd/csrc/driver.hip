@@ -23,6 +23,7 @@
 #include <deque>
 #include <mutex>
 #include <random>
+#include <exception>
 #include <thread>
 
 #include "vec.hpp"
@@ -1256,13 +1257,15 @@ class BatchLanczos {
   void refine(double2* X, double2* Y, double2* KY, double2* MY, long long ldx, int nemax, const std::vector<int>& nr,
               const DevBuf& kp) {
     DevBuf dnr = upload(pool(), nr, stream());
+    const int nact = (int)std::count_if(nr.begin(), nr.end(), [](int v) { return v > 0; });
     // Y = M X, then solve in place column by column
     spmv_cols(mk_ptrs_, X, ldx, dnr, nemax, Y);
     BatchBufs bb = bm_.bufs(P_, B_);
     hipLaunchKernelGGL(k_flag_cols, dim3((B_ + 63) / 64), dim3(64), 0, stream(), dnr.as<int>(), 0, B_,
                        act_.as<int>());
     for (int q0 = 0; q0 < nemax; q0 += bm_.pcols)
-      trisolve_batch(ctx_, M_, P_, bb, Y + (long long)q0 * n_, ldx, act_.as<int>(), std::min(bm_.pcols, nemax - q0));
+      trisolve_batch(ctx_, M_, P_, bb, Y + (long long)q0 * n_, ldx, act_.as<int>(), std::min(bm_.pcols, nemax - q0),
+                     -1, nact);
     spmv_cols(kp, Y, ldx, dnr, nemax, KY);
     spmv_cols(mk_ptrs_, Y, ldx, dnr, nemax, MY);
     const size_t gsz = (size_t)B_ * nemax * nemax * nchunk_;
@@ -2026,6 +2029,7 @@ class BatchLanczos {
     if (!any) return;
     BatchBufs bb = bm_.bufs(P_, B_);
     DevBuf dact = ints(iter_), dy0 = ints(y0_);
+    const int nit = (int)std::count_if(iter_.begin(), iter_.end(), [](int v) { return v != 0; });
     if (!qep_) {
       // W = (K - sigma M)^-1 M Y, Y = V[:, y0 : y0 + pb]
       {
@@ -2033,7 +2037,7 @@ class BatchLanczos {
         hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), MVb_.as<double2>(), ldv_,
                            dy0.as<int>(), Wb_.as<double2>(), ldw_, (const int*)nullptr, pb_, n_, dact.as<int>());
       }
-      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>(), ldw_, dact.as<int>(), pb_);
+      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>(), ldw_, dact.as<int>(), pb_, -1, nit);
     } else {
       // W = L1^-1 L2 Y: for Y = [a; b], W = [b; K^-1 (M a - D b)] (one solve with K(k))
       std::vector<int> pbv(B_, 0);
@@ -2051,7 +2055,7 @@ class BatchLanczos {
       }
       // W_b += (-D) Y_b
       spmv_cols(dptr_, Wb_.as<double2>(), ldw_, dpb, pb_, Wb_.as<double2>() + n_, true);
-      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>() + n_, ldw_, dact.as<int>(), pb_, nvec_);
+      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>() + n_, ldw_, dact.as<int>(), pb_, nvec_, nit);
     }
     for (int b = 0; b < B_; ++b)
       if (iter_[b]) probs_[b]->steps += 1;
@@ -2140,7 +2144,8 @@ class BatchLanczos {
                            dsel.as<int>());  // b <- M a
       }
       spmv_cols(dptr_, Wb_.as<double2>(), ldw_, dpb, pb_, Wb_.as<double2>() + n_, true);  // += (-D) b
-      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>() + n_, ldw_, dsel.as<int>(), pb_, nvec_);
+      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>() + n_, ldw_, dsel.as<int>(), pb_, nvec_,
+                     (int)std::count_if(sel.begin(), sel.end(), [](int v) { return v != 0; }));
     }
   }
 
@@ -2511,6 +2516,126 @@ int64_t problem_footprint(const Operator& op, int num_modes, int harmonic, int m
   return problem_footprint(op, p);
 }
 
+// Releases what a context holds besides its pool's live buffers: waits for its
+// stream, drains its deliverer, frees the cached pool blocks, events and stream.
+static void release_context(Context& c) {
+  (void)hipSetDevice(c.device);
+  (void)hipStreamSynchronize(c.stream);
+  c.groups.clear();
+  c.deliver.reset();  // drains its copies
+  c.pool.trim();
+  for (auto& p : c.pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  c.pending.clear();
+  for (auto ev : c.event_pool) (void)hipEventDestroy(ev);
+  c.event_pool.clear();
+  (void)hipStreamDestroy(c.stream);
+  c.stream = nullptr;
+}
+
+// a context of its own (stream, pool) on the parent's device, for one slot group
+struct SubContext {
+  Context c;
+  explicit SubContext(int device) {
+    c.device = device;
+    TM_HIP(hipSetDevice(device));
+    TM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  }
+  ~SubContext() { release_context(c); }
+};
+
+// The context of concurrent slot group g of ctx, created on first use.
+static Context* group_context(Context* ctx, size_t g) {
+  while (ctx->groups.size() <= g) {
+    std::unique_ptr<Context> c(new Context);
+    c->device = ctx->device;
+    TM_HIP(hipSetDevice(ctx->device));
+    TM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    ctx->groups.emplace_back(c.release(), [](Context* p) {
+      release_context(*p);
+      delete p;
+    });
+  }
+  Context* c = ctx->groups[g].get();
+  c->profiling = ctx->profiling;
+  return c;
+}
+
+// Folds a group context's timings into its parent's.
+static void merge_stats(Context* ctx, Context* g) {
+  g->flush_timing();
+  for (const auto& kv : g->stats) {
+    KernelStat& d = ctx->stats[kv.first];
+    d.launches += kv.second.launches;
+    d.total_ms += kv.second.total_ms;
+    d.bytes += kv.second.bytes;
+    d.flops += kv.second.flops;
+  }
+  g->stats.clear();
+}
+
+// Runs of one mesh's block-method problems (one block size each) as slot groups
+// side by side, each on its own host thread and context, so that one group's
+// host work (Gram reductions, Rayleigh-Ritz, extraction) and small launches
+// overlap the other's kernels; the slot capacities share the memory budget in
+// proportion to the runs' sizes.  One run (or TMHIP_SERIAL_GROUPS=1): one after
+// another on the caller's context.
+static void run_block_groups(Context* ctx, Operator& op, std::vector<std::vector<HostProblem*>>& runs, int64_t per,
+                             bool qep) {
+  per = std::max<int64_t>(per, 1);
+  const char* ser = std::getenv("TMHIP_SERIAL_GROUPS");
+  const bool serial = runs.size() == 1 || (ser && std::atoi(ser) != 0);
+  size_t free_b = 0, total_b = 0;
+  if (serial) {
+    for (auto& g : ctx->groups) g->pool.trim();  // cached there, of no use here
+    for (auto& q : runs) {
+      TM_HIP(hipMemGetInfo(&free_b, &total_b));
+      // blocks cached by the context's pool are reusable by this call
+      const int64_t budget = (int64_t)(0.85 * (double)(free_b + ctx->pool.cached()));
+      const int cap = (int)std::max<int64_t>(1, std::min<int64_t>({64, (int64_t)q.size(), budget / per}));
+      BatchLanczos bl(ctx, op, {}, true);
+      bl.run_slots(q, cap, cap, qep);
+    }
+    return;
+  }
+  ctx->pool.trim();
+  size_t total = 0;
+  int64_t cached = 0;
+  for (auto& q : runs) total += q.size();
+  std::vector<Context*> gc(runs.size());
+  for (size_t g = 0; g < runs.size(); ++g) {
+    gc[g] = group_context(ctx, g);
+    cached += (int64_t)gc[g]->pool.cached();
+  }
+  TM_HIP(hipMemGetInfo(&free_b, &total_b));
+  const double budget = 0.85 * (double)((int64_t)free_b + cached);
+  std::vector<std::exception_ptr> err(runs.size());
+  std::vector<std::thread> th;
+  for (size_t g = 0; g < runs.size(); ++g)
+    th.emplace_back([&, g] {
+      Context* c = gc[g];
+      try {
+        TM_HIP(hipSetDevice(c->device));
+        const int64_t share = (int64_t)(budget * (double)runs[g].size() / (double)total);
+        const int cap = (int)std::max<int64_t>(1, std::min<int64_t>({64, (int64_t)runs[g].size(), share / per}));
+        {
+          BatchLanczos bl(c, op, {}, true);
+          bl.run_slots(runs[g], cap, cap, qep);
+        }
+        if (c->deliver) c->deliver->drain();
+      } catch (...) {
+        err[g] = std::current_exception();
+        if (c->deliver) c->deliver->drain_quiet();
+      }
+    });
+  for (auto& t : th) t.join();
+  for (Context* c : gc) merge_stats(ctx, c);
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+}
+
 static void solve_problems_impl(Context* ctx, const tm_problem* problems, int n, tm_result* results);
 
 // Mode shapes reach the caller's buffers through the context's deliverer; every
@@ -2563,28 +2688,47 @@ static void solve_problems_impl(Context* ctx, const tm_problem* problems, int n,
       per = std::max<int64_t>(per, problem_footprint(*op, hp->in));
     }
     std::stable_sort(list.begin(), list.end(), [](HostProblem* a, HostProblem* b) { return a->nev < b->nev; });
+    if (key.second >= 1) {
+      // block method: one run per block size (the Coriolis group is one run)
+      const bool qep = key.second == 2;
+      std::vector<std::vector<HostProblem*>> runs;
+      for (size_t i = 0; i < list.size();) {
+        size_t j = i;
+        const int pb_i = block_width(list[i]->nev, false);
+        while (j < list.size() && (qep || block_width(list[j]->nev, false) == pb_i)) ++j;
+        runs.emplace_back(list.begin() + i, list.begin() + j);
+        i = j;
+      }
+      {  // runs of >= 128 problems in two interleaved parts (TMHIP_GROUP_SPLIT: 1..4 parts)
+        const char* sp = std::getenv("TMHIP_GROUP_SPLIT");
+        const int nsplit = sp ? std::max(1, std::min(4, std::atoi(sp))) : 2;
+        std::vector<std::vector<HostProblem*>> parts;
+        for (auto& q : runs) {
+          if (nsplit == 1 || q.size() < 128) {
+            parts.push_back(q);
+            continue;
+          }
+          for (int h = 0; h < nsplit; ++h) {
+            parts.emplace_back();
+            for (size_t i = h; i < q.size(); i += nsplit) parts.back().push_back(q[i]);
+          }
+        }
+        runs.swap(parts);
+      }
+      run_block_groups(ctx, *op, runs, per, qep);
+      continue;
+    }
     size_t free_b = 0, total_b = 0;
     TM_HIP(hipMemGetInfo(&free_b, &total_b));
     // blocks cached by the context's pool are reusable by this call
     int64_t budget = (int64_t)(0.85 * (double)(free_b + ctx->pool.cached()));
     size_t max_batch = (size_t)std::max<int64_t>(1, std::min<int64_t>(64, budget / std::max<int64_t>(per, 1)));
-    for (size_t i = 0; i < list.size();) {
-      // one block size per slot group (Lanczos: lockstep batches)
+    for (size_t i = 0; i < list.size();) {  // Lanczos: lockstep batches
       size_t j = i;
-      const int pb_i = block_width(list[i]->nev, false);
-      if (key.second >= 1) {
-        const bool qep = key.second == 2;
-        while (j < list.size() && (qep || block_width(list[j]->nev, false) == pb_i)) ++j;
-        std::vector<HostProblem*> queue(list.begin() + i, list.begin() + j);
-        BatchLanczos bl(ctx, *op, {}, true);
-        const int cap = static_cast<int>(std::min(max_batch, queue.size()));
-        bl.run_slots(queue, cap, cap, qep);
-      } else {
-        while (j < list.size() && j - i < max_batch) ++j;
-        std::vector<HostProblem*> chunk(list.begin() + i, list.begin() + j);
-        BatchLanczos bl(ctx, *op, chunk, false);
-        bl.run();
-      }
+      while (j < list.size() && j - i < max_batch) ++j;
+      std::vector<HostProblem*> chunk(list.begin() + i, list.begin() + j);
+      BatchLanczos bl(ctx, *op, chunk, false);
+      bl.run();
       i = j;
     }
   }
@@ -2606,27 +2750,6 @@ static void solve_problems_impl(Context* ctx, const tm_problem* problems, int n,
 // dispatcher thread.
 // ---------------------------------------------------------------------------
 namespace tme {
-
-// a context of its own (stream, pool) on the parent's device, for one slot group
-struct SubContext {
-  Context c;
-  explicit SubContext(int device) {
-    c.device = device;
-    TM_HIP(hipSetDevice(device));
-    TM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-  }
-  ~SubContext() {
-    (void)hipStreamSynchronize(c.stream);
-    c.deliver.reset();  // drains its copies
-    c.pool.trim();
-    for (auto& p : c.pending) {
-      (void)hipEventDestroy(p.a);
-      (void)hipEventDestroy(p.b);
-    }
-    for (auto ev : c.event_pool) (void)hipEventDestroy(ev);
-    (void)hipStreamDestroy(c.stream);
-  }
-};
 
 struct Server {
   struct Req {

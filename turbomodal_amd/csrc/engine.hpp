@@ -220,6 +220,10 @@ struct Context {
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
   std::unique_ptr<Deliverer> deliver;  // created on first use
+  // contexts of the slot groups that one solve call runs side by side (own
+  // stream, pool and deliverer), kept so their pools' cached workspaces serve
+  // the next call; their deleters synchronise and release them
+  std::vector<std::shared_ptr<Context>> groups;
   hipEvent_t get_event();
   void flush_timing();  // synchronises and folds pending events into stats
   Deliverer& deliverer() {

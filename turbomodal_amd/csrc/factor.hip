@@ -887,7 +887,7 @@ static void trisolve_levels(Context* ctx, const Plan& P, const SolveArgs& sa, in
 }
 
 void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs& bb, double2* X,
-                    long long ldx, const int* active, int p, long long colstride) {
+                    long long ldx, const int* active, int p, long long colstride, int nactive) {
   if (p < 1 || p > PMAX) throw ArgError("trisolve: 1 <= p <= 16 right-hand sides");
   SolveArgs sa;
   sa.sn = P.d_sn.as<DevSn>();
@@ -906,7 +906,8 @@ void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs&
   // algorithmic bytes: the factor once (L21 + inverted pivot triangles) and ~5 n-vectors per right-hand side
   const double vec_bytes = 5.0 * 3 * M.S.nf * 16.0;
   const double l_bytes = P.trisolve_bytes_c16 - vec_bytes;
-  KTimer t(ctx, "trisolve", (l_bytes + p * vec_bytes) * B, 8.0 * p * B * l_bytes / 20.0);
+  const int nb = nactive >= 0 ? std::min(nactive, B) : B;  // problems that do the work
+  KTimer t(ctx, "trisolve", (l_bytes + p * vec_bytes) * nb, 8.0 * p * nb * l_bytes / 20.0);
   if (p == 1) trisolve_levels<1>(ctx, P, sa, B);
   else if (p <= 4) trisolve_levels<4>(ctx, P, sa, B);
   else if (p <= 8) trisolve_levels<8>(ctx, P, sa, B);

@@ -156,8 +156,9 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
                   const OpArrays* ops_b = nullptr);
 // in-place solve A X = B for each problem's block X_b = X + b*ldx: p columns of
 // length n = 3 nf, column c at X_b + c*n.  bb.T must hold p * tvec_entries per problem.
+// nactive: how many problems `active` selects (-1: all B), for the byte count only.
 void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs& bb, double2* X,
-                    long long ldx, const int* active, int p = 1, long long colstride = -1);
+                    long long ldx, const int* active, int p = 1, long long colstride = -1, int nactive = -1);
 
 struct HostProblem;
 void create_operator(Context* ctx, Mesh& M, Operator& op);
