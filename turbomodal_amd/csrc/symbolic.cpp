@@ -29,7 +29,7 @@ namespace tme {
 
 namespace {
 
-constexpr int kLeafNodes = 40;
+constexpr int kLeafNodes = 32;
 
 struct NDTree {
   std::vector<std::vector<int>> nodes;
