@@ -2336,15 +2336,10 @@ class BatchLanczos {
         const double2* xs = Xb_.as<double2>() + (qep_ ? n_ : 0LL) + (long long)c0 * nvec_;
         const dim3 ga((S_.nf + 3) / 4, B_, (cm + 7) / 8);
         hipLaunchKernelGGL(k_spmv_abs, ga, dim3(256), 0, stream(), S_.nf, M_.d_rowptr.as<int>(),
-                           M_.d_colidx.as<int>(), kptr_.as<const double2*>(), xs, ldk_, nvec_, dnc2.as<int>(),
-                           Wb_.as<double2>(), ldw_);
-        hipLaunchKernelGGL(k_spmv_abs, ga, dim3(256), 0, stream(), S_.nf, M_.d_rowptr.as<int>(),
-                           M_.d_colidx.as<int>(), mk_ptrs_.as<const double2*>(), xs, ldk_, nvec_, dnc2.as<int>(),
-                           MWb_.as<double2>(), ldw_);
-        if (qep_)
-          hipLaunchKernelGGL(k_spmv_abs, ga, dim3(256), 0, stream(), S_.nf, M_.d_rowptr.as<int>(),
-                             M_.d_colidx.as<int>(), dptr_.as<const double2*>(), xs, ldk_, nvec_, dnc2.as<int>(),
-                             KWb_.as<double2>(), ldw_);
+                           M_.d_colidx.as<int>(), kptr_.as<const double2*>(), mk_ptrs_.as<const double2*>(),
+                           qep_ ? dptr_.as<const double2*>() : (const double2* const*)nullptr, xs, ldk_, nvec_,
+                           dnc2.as<int>(), Wb_.as<double2>(), MWb_.as<double2>(),
+                           qep_ ? KWb_.as<double2>() : (double2*)nullptr, ldw_);
         hipLaunchKernelGGL(k_floor_partial, dim3(nchunk_, B_, cm), dim3(256), 0, stream(), Wb_.as<double2>(),
                            qep_ ? KWb_.as<double2>() : (const double2*)nullptr, MWb_.as<double2>(), ldw_, n_, nvec_,
                            dl.as<double>() + c0, nkeep_, dnc2.as<int>(), fpart.as<double>() + (size_t)c0 * nchunk_,

@@ -690,43 +690,66 @@ __global__ void k_copy_cols(const double2* __restrict__ src, long long lds, cons
 }
 
 // Y_b[:, c] = |A_b| |X_b[:, c]| (entrywise moduli; real result in .x) for c < ncol_b
+// |A_q| |x| for up to three block arrays over the same columns (blocks3 optional):
+// Y_q[:, c] = |A_q| |X[:, c]| (real, in the .x part), c < ncol_b.  One wave per node
+// row, lanes = 8 block slots x 8 columns (columns 8 blockIdx.z ..), as k_spmv_pair:
+// each block row is read once for all the columns and |x| is formed once for all
+// the arrays.  Moduli as sqrt(re^2 + im^2): operator entries and vectors are far from
+// the overflow range, and the products only size a rounding floor.
 __global__ __launch_bounds__(256) void k_spmv_abs(int nf, const int* __restrict__ rowptr,
                                                   const int* __restrict__ colidx,
-                                                  const double2* const* __restrict__ blocks,
+                                                  const double2* const* __restrict__ blocks1,
+                                                  const double2* const* __restrict__ blocks2,
+                                                  const double2* const* __restrict__ blocks3,
                                                   const double2* __restrict__ X, long long ldx, long long cs,
-                                                  const int* __restrict__ ncol, double2* __restrict__ Y,
-                                                  long long ldy) {
-  // one wave per node row, lanes = 8 block slots x 8 columns (columns 8 blockIdx.z ..),
-  // as k_spmv_pair: the block row is read once for all the columns
+                                                  const int* __restrict__ ncol, double2* __restrict__ Y1,
+                                                  double2* __restrict__ Y2, double2* __restrict__ Y3, long long ldy) {
   const int b = blockIdx.y;
   const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (I >= nf) return;
   const int lane = threadIdx.x & 63, c = 8 * blockIdx.z + (lane & 7), sl = lane >> 3;
   const bool on = c < ncol[b];
-  const double2* Mb = blocks[b];
+  const double2* A[3] = {blocks1[b], blocks2[b], blocks3 ? blocks3[b] : nullptr};
   const double2* x = X + b * ldx + (long long)c * cs;
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
   const int s1 = rowptr[I + 1];
   if (on)
     for (int s = rowptr[I] + sl; s < s1; s += 8) {
-      const double2* blk = Mb + (long long)s * 9;
       const long long J3 = 3LL * colidx[s];
       double ax[3];
 #pragma unroll
-      for (int q = 0; q < 3; ++q) ax[q] = hypot(x[J3 + q].x, x[J3 + q].y);
+      for (int q = 0; q < 3; ++q) {
+        const double2 v = x[J3 + q];
+        ax[q] = sqrt(fma(v.x, v.x, v.y * v.y));
+      }
 #pragma unroll
-      for (int rr = 0; rr < 3; ++rr)
+      for (int m = 0; m < 3; ++m) {
+        if (!A[m]) continue;
+        const double2* blk = A[m] + (long long)s * 9;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) acc[rr] = fma(hypot(blk[3 * rr + q].x, blk[3 * rr + q].y), ax[q], acc[rr]);
+        for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const double2 e = blk[3 * rr + q];
+            acc[m][rr] = fma(sqrt(fma(e.x, e.x, e.y * e.y)), ax[q], acc[m][rr]);
+          }
+      }
     }
 #pragma unroll
-  for (int rr = 0; rr < 3; ++rr)
+  for (int m = 0; m < 3; ++m)
 #pragma unroll
-    for (int o = 8; o < 64; o <<= 1) acc[rr] += __shfl_xor(acc[rr], o, 64);
+    for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+      for (int o = 8; o < 64; o <<= 1) acc[m][rr] += __shfl_xor(acc[m][rr], o, 64);
   if (sl == 0 && on) {
-    double2* y = Y + b * ldy + (long long)c * cs + 3LL * I;
+    double2* Y[3] = {Y1, Y2, Y3};
 #pragma unroll
-    for (int rr = 0; rr < 3; ++rr) y[rr] = make_double2(acc[rr], 0.0);
+    for (int m = 0; m < 3; ++m) {
+      if (!A[m]) continue;
+      double2* y = Y[m] + b * ldy + (long long)c * cs + 3LL * I;
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr) y[rr] = make_double2(acc[m][rr], 0.0);
+    }
   }
 }
 

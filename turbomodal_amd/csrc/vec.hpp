@@ -89,8 +89,9 @@ __global__ void k_axpy_half(const double2* src, long long lds, const int* s0v, l
                             double alpha, double beta, const int* act);
 __global__ void k_head_dot(const double2* X, const double2* Y, long long ldb, long long stride, long long len,
                            const int* ncol, int ldc, double* part, int nchunk);
-__global__ void k_spmv_abs(int nf, const int* rowptr, const int* colidx, const double2* const* blocks,
-                           const double2* X, long long ldx, long long n, const int* ncol, double2* Y,
+__global__ void k_spmv_abs(int nf, const int* rowptr, const int* colidx, const double2* const* blocks1,
+                           const double2* const* blocks2, const double2* const* blocks3, const double2* X,
+                           long long ldx, long long cs, const int* ncol, double2* Y1, double2* Y2, double2* Y3,
                            long long ldy);
 __global__ void k_floor_partial(const double2* AK, const double2* AD, const double2* AM, long long ldb, long long n,
                                 long long cs, const double* lam,
