@@ -384,7 +384,60 @@ struct InvArgs {
   const int* active;  // per problem (null: all)
 };
 
-__global__ __launch_bounds__(64) void k_inv_diag(InvArgs ia, const int* __restrict__ tasks, int t0, int J) {
+// One level of the recursive inversion of a unit lower triangle held in A (strict
+// lower part; the unit diagonal is implicit and the diagonal entries hold 0): for
+// each of the nblk diagonal blocks of size 2h at 2h P, whose two h x h diagonal
+// blocks already hold their inverses X11, X22, the off-diagonal block L21 becomes
+//   X21 = -X22 (L21 X11)
+// in two passes (T = L21 X11 into L21's place, then X21), 256 threads, outputs
+// with the column fastest (row reads broadcast, column reads contiguous).
+template <int h, int nblk>
+__device__ __forceinline__ void inv_combine(double2 (*A)[64]) {
+  constexpr int nout = nblk * h * h, per = (nout + 255) / 256;
+  static_assert(nout % 256 == 0, "whole passes of 256 outputs");
+  const int tid = threadIdx.x;
+  double2 v[per];
+  // T[i][j] = L21[i][j] + sum_{k>j} L21[i][k] X11[k][j]
+#pragma unroll
+  for (int u = 0; u < per; ++u) {
+    const int e = tid + 256 * u;
+    const int j = e % h, i = (e / h) % h, o = 2 * h * (e / (h * h));
+    double2 t = A[o + h + i][o + j];
+    for (int k = j + 1; k < h; ++k) cmac(t, A[o + h + i][o + k], A[o + k][o + j]);
+    v[u] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < per; ++u) {
+    const int e = tid + 256 * u;
+    const int j = e % h, i = (e / h) % h, o = 2 * h * (e / (h * h));
+    A[o + h + i][o + j] = v[u];
+  }
+  __syncthreads();
+  // X21[i][j] = -(T[i][j] + sum_{k<i} X22[i][k] T[k][j])
+#pragma unroll
+  for (int u = 0; u < per; ++u) {
+    const int e = tid + 256 * u;
+    const int j = e % h, i = (e / h) % h, o = 2 * h * (e / (h * h));
+    double2 t = A[o + h + i][o + j];
+    for (int k = 0; k < i; ++k) cmac(t, A[o + h + i][o + h + k], A[o + h + k][o + j]);
+    v[u] = make_double2(-t.x, -t.y);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < per; ++u) {
+    const int e = tid + 256 * u;
+    const int j = e % h, i = (e / h) % h, o = 2 * h * (e / (h * h));
+    A[o + h + i][o + j] = v[u];
+  }
+  __syncthreads();
+}
+
+// Inverse of the unit lower 64-block J of a front's pivot triangle (nb <= 64
+// rows; the padding is an identity block), in place in LDS: the four 16 x 16
+// diagonal blocks by the row recurrence, one wave each (lanes 0..15 own the
+// columns), then two levels of inv_combine; written to the scratch X columns.
+__global__ __launch_bounds__(256) void k_inv_diag(InvArgs ia, const int* __restrict__ tasks, int t0, int J) {
   const int id = tasks[t0 + blockIdx.x];
   const int b = blockIdx.y;
   if (ia.active && !ia.active[b]) return;
@@ -393,23 +446,35 @@ __global__ __launch_bounds__(64) void k_inv_diag(InvArgs ia, const int* __restri
   const int j0 = 64 * J, nb = min(64, w - j0);
   const double2* Pn = ia.L + b * ia.PE + s.poff;
   double2* Sc = ia.S + b * ia.SE + s.soff;  // w x 128, column-major (ld = w): X | Z
-  __shared__ double2 A[64 * 64];
-  const int t = threadIdx.x;
-  for (int i = 0; i < nb; ++i) A[i * 64 + t] = (t < i) ? Pn[(j0 + i) + (long long)(j0 + t) * m] : make_double2(0.0, 0.0);
-  __syncthreads();
-  // row i of X: X[i][j] = -L[i][j] - sum_{j<k<i} L[i][k] X[k][j]; rows k < i already hold X
-  for (int i = 1; i < nb; ++i) {
-    double2 v = make_double2(0.0, 0.0);
-    if (t < i) {
-      v = make_double2(-A[i * 64 + t].x, -A[i * 64 + t].y);
-      for (int k = t + 1; k < i; ++k) v = csub(v, cmul(A[i * 64 + k], A[k * 64 + t]));
-    }
-    __syncthreads();
-    if (t < i) A[i * 64 + t] = v;
-    __syncthreads();
+  __shared__ double2 A[64][64];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < 64 * 64; idx += 256) {
+    const int i = idx / 64, t = idx % 64;
+    A[i][t] = (t < i && i < nb) ? Pn[(j0 + i) + (long long)(j0 + t) * m] : make_double2(0.0, 0.0);
   }
-  for (int i = 0; i < nb; ++i)
-    if (t < nb) Sc[(j0 + i) + (long long)t * w] = (t < i) ? A[i * 64 + t] : make_double2(t == i ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+  {
+    // row i of the 16-block o: X[i][t] = -L[i][t] - sum_{t<k<i} L[i][k] X[k][t]; rows
+    // k < i already hold X
+    const int o = 16 * (tid >> 6), t = tid & 63;
+    for (int i = 1; i < 16; ++i) {
+      double2 v = make_double2(0.0, 0.0);
+      if (t < i) {
+        v = make_double2(-A[o + i][o + t].x, -A[o + i][o + t].y);
+        for (int k = t + 1; k < i; ++k) v = csub(v, cmul(A[o + i][o + k], A[o + k][o + t]));
+      }
+      __syncthreads();
+      if (t < i) A[o + i][o + t] = v;
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  inv_combine<16, 2>(A);
+  inv_combine<32, 1>(A);
+  for (int idx = tid; idx < nb * nb; idx += 256) {
+    const int i = idx % nb, t = idx / nb;
+    Sc[(j0 + i) + (long long)t * w] = (t < i) ? A[i][t] : make_double2(t == i ? 1.0 : 0.0, 0.0);
+  }
 }
 
 // 64x64 tile product acc += A[rows, k] B[k, cols] over k in [k0, k1); A and B
@@ -848,7 +913,7 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
                            P.d_tile.as<TileTask>(), st.tile0, st.kb);
     }
     for (const InvStep& iv : L.inv) {
-      hipLaunchKernelGGL(k_inv_diag, dim3(iv.ndiag, B), dim3(64), 0, ctx->stream, ia, P.d_inv_diag.as<int>(),
+      hipLaunchKernelGGL(k_inv_diag, dim3(iv.ndiag, B), dim3(256), 0, ctx->stream, ia, P.d_inv_diag.as<int>(),
                          iv.diag0, iv.J);
       if (iv.ntile) {
         hipLaunchKernelGGL(k_inv_z, dim3(iv.ntile, B), dim3(256), 0, ctx->stream, ia,
