@@ -692,6 +692,10 @@ const double2* harmonic_mass(Context* ctx, Operator& op, int k) {
 // restarts, and each sweep costs more)
 constexpr int kNarrowNev = 6;
 inline int block_width(int nev, bool qep) { return (nev <= kNarrowNev && !qep) ? 8 : 16; }
+// Basis length of the block method: 8 blocks of 8; 16-wide blocks 8 of them (6
+// for the Coriolis pencil, whose vectors are twice as long).  With these the C4
+// sweep restarts almost never and the Coriolis problems about twice on average.
+inline int basis_length(int pb, bool qep) { return pb == 8 ? 64 : (qep ? 96 : 128); }
 
 struct HostProblem {
   tm_problem in;
@@ -1612,9 +1616,10 @@ class BatchLanczos {
 
   void block_setup() {
     pb_ = block_width(nevmax_, qep_);
-    mmax_ = pb_ == 8 ? 48 : 64;
+    mmax_ = basis_length(pb_, qep_);
+    if (mmax_ > kTsMaxRows || mmax_ % pb_) throw ArgError("basis length");
     nkeep_ = mmax_ / 2;
-    static_assert(kTsMaxRows >= 64, "tsmm coefficient rows");
+    static_assert(kTsMaxRows >= 128, "tsmm coefficient rows");
     ldv_ = (long long)mmax_ * nvec_;
     ldw_ = (long long)pb_ * nvec_;
     ldk_ = (long long)nkeep_ * nvec_;
@@ -2488,7 +2493,7 @@ static int64_t problem_footprint(const Operator& op, const tm_problem& p) {
   const int pcols = std::min(16, std::max(block ? pb : 1, hp.nev));
   int64_t bytes = 16 * (P.panel_entries + P.upool_entries + pcols * P.tvec_entries + P.inv_scratch_entries);
   if (block) {
-    const int mmax = pb == 8 ? 48 : 64;
+    const int mmax = basis_length(pb, hp.qep);
     const int64_t nv = hp.qep ? 2 * n : n;           // pencil vectors are 2n long
     bytes += 3 * 16 * (int64_t)mmax * nv;            // V, MV, KV
     bytes += 3 * 16 * (int64_t)pb * nv;              // W, MW, KW

@@ -7,7 +7,7 @@ namespace tme {
 
 constexpr int kDotChunk = 2048;  // rows per partial-dot block (8 per thread)
 constexpr int kTsChunk = 4096;   // rows per tall-skinny A^H B block
-constexpr int kTsMaxRows = 64;   // coefficient rows of k_tsmm (basis vectors per problem)
+constexpr int kTsMaxRows = 128;  // coefficient rows of k_tsmm (basis vectors per problem)
 
 // Device-side state of a batch of Lanczos runs.
 struct LzState {
