@@ -302,6 +302,16 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
       cr[p][q] = v4d{0.0, 0.0, 0.0, 0.0};
       ci[p][q] = v4d{0.0, 0.0, 0.0, 0.0};
     }
+  // 16 x 16 blocks of this wave whose result is written: not wholly above the
+  // diagonal, below row m, or right of column jlim (uniform over the wave)
+  bool need[2][2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int gi1 = t.i0 + r0 + 16 * p + 15, gi0 = t.i0 + r0 + 16 * p, gj0 = t.j0 + q0 + 16 * q;
+      need[p][q] = gi1 >= gj0 && gi0 < m && gj0 < jlim;
+    }
   // staging of one chunk: this thread's SCHUR_KC * TILE / 256 elements of each
   // panel, loaded one chunk ahead so the loads overlap the previous products
   constexpr int PER = TILE * SCHUR_KC / 256;
@@ -348,6 +358,7 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
       for (int p = 0; p < 2; ++p)
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
+          if (!need[p][q]) continue;
           cr[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p].x, bq[q].x, cr[p][q], 0, 0, 0);
           cr[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p].y, bq[q].y, cr[p][q], 0, 0, 0);
           ci[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p].y, bq[q].x, ci[p][q], 0, 0, 0);
