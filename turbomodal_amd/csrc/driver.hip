@@ -2699,21 +2699,17 @@ static void solve_problems_impl(Context* ctx, const tm_problem* problems, int n,
         runs.emplace_back(list.begin() + i, list.begin() + j);
         i = j;
       }
-      {  // runs of >= 128 problems in two interleaved parts (TMHIP_GROUP_SPLIT: 1..4 parts)
-        const char* sp = std::getenv("TMHIP_GROUP_SPLIT");
-        const int nsplit = sp ? std::max(1, std::min(4, std::atoi(sp))) : 2;
-        std::vector<std::vector<HostProblem*>> parts;
-        for (auto& q : runs) {
-          if (nsplit == 1 || q.size() < 128) {
-            parts.push_back(q);
-            continue;
-          }
-          for (int h = 0; h < nsplit; ++h) {
-            parts.emplace_back();
-            for (size_t i = h; i < q.size(); i += nsplit) parts.back().push_back(q[i]);
-          }
-        }
-        runs.swap(parts);
+      // while fewer than three groups: the largest run (16 problems or more) in two
+      // interleaved halves, so that groups overlap one another's host work
+      while (runs.size() < 3) {
+        size_t big = 0;
+        for (size_t g = 1; g < runs.size(); ++g)
+          if (runs[g].size() > runs[big].size()) big = g;
+        if (runs[big].size() < 16) break;
+        std::vector<HostProblem*> a, b;
+        for (size_t i = 0; i < runs[big].size(); ++i) (i % 2 ? b : a).push_back(runs[big][i]);
+        runs[big].swap(a);
+        runs.insert(runs.begin() + big + 1, std::move(b));
       }
       run_block_groups(ctx, *op, runs, per, qep);
       continue;
