@@ -1270,8 +1270,7 @@ class BatchLanczos {
     for (int q0 = 0; q0 < nemax; q0 += bm_.pcols)
       trisolve_batch(ctx_, M_, P_, bb, Y + (long long)q0 * n_, ldx, act_.as<int>(), std::min(bm_.pcols, nemax - q0),
                      -1, nact);
-    spmv_cols(kp, Y, ldx, dnr, nemax, KY);
-    spmv_cols(mk_ptrs_, Y, ldx, dnr, nemax, MY);
+    spmv_pair(kp, &mk_ptrs_, Y, ldx, dnr, nemax, KY, MY, false, false);  // K Y, M Y in one pass
     const size_t gsz = (size_t)B_ * nemax * nemax * nchunk_;
     DevBuf pk(pool(), sizeof(double2) * gsz), pm(pool(), sizeof(double2) * gsz);
     hipLaunchKernelGGL(k_gram_partial, dim3(nchunk_, B_, nemax * nemax), dim3(256), 0, stream(), Y,
