@@ -594,6 +594,7 @@ struct SolveArgs {
   const DevSn* sn;
   const int* struct_nodes;
   const int* relmap;
+  const int* invmap;  // per front and child: front node position -> child update node, or -1
   const double2* L;
   double2* T;
   long long PE, TE;
@@ -607,33 +608,45 @@ __device__ __forceinline__ double2* tcol(const SolveArgs& sa, int b, int c, long
   return sa.T + ((long long)b * sa.p + c) * sa.TE + off;
 }
 
+// the children's update vectors of front s, pulled through the inverse relmaps:
+// for each child (child0 first), its map and its update-vector offset
+struct ChildPull {
+  const int* inv[2];
+  long long toff[2];
+};
+__device__ __forceinline__ ChildPull child_pull(const SolveArgs& sa, const DevSn& s) {
+  ChildPull cp{{nullptr, nullptr}, {0, 0}};
+  int o = s.inv0;
+  for (int k = 0; k < 2; ++k) {
+    const int c = k == 0 ? s.child0 : s.child1;
+    if (c < 0) continue;
+    cp.inv[k] = sa.invmap + o;
+    o += s.ncol + s.nstr;
+    cp.toff[k] = sa.sn[c].toff;
+  }
+  return cp;
+}
+
+// pivot rows of front s: x += the children's update entries that land on them
+// (child0 then child1).  The update rows are written by k_fwd_upd.
 __global__ __launch_bounds__(256) void k_fwd_asm(SolveArgs sa, const int* __restrict__ level_sn, int s0) {
   const int b = blockIdx.y;
   if (sa.active && !sa.active[b]) return;
   const DevSn s = sa.sn[level_sn[s0 + blockIdx.x]];
-  const int w = 3 * s.ncol, r = 3 * s.nstr, p = sa.p;
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < r * p; idx += 256) *tcol(sa, b, idx / r, s.toff + idx % r) = make_double2(0.0, 0.0);
-  __syncthreads();
-  for (int k = 0; k < 2; ++k) {
-    const int c = k == 0 ? s.child0 : s.child1;
-    if (c < 0) continue;
-    const DevSn cs = sa.sn[c];
-    const int rc = 3 * cs.nstr;
-    const int* rm = sa.relmap + cs.rel0;
-    for (int idx = tid; idx < rc * p; idx += 256) {
-      const int q = idx % rc, col = idx / rc;
-      const int pos = 3 * rm[q / 3] + q % 3;
-      const double2 v = *tcol(sa, b, col, cs.toff + q);
-      if (pos < w) {
-        double2* x = sa.X + b * sa.ldx + (long long)col * sa.n + 3LL * s.col0 + pos;
-        *x = cadd(*x, v);
-      } else {
-        double2* t = tcol(sa, b, col, s.toff + pos - w);
-        *t = cadd(*t, v);
-      }
+  const int w = 3 * s.ncol, p = sa.p;
+  if (s.child0 < 0 && s.child1 < 0) return;
+  const ChildPull cp = child_pull(sa, s);
+  for (int idx = threadIdx.x; idx < w * p; idx += 256) {
+    const int i = idx % w, c = idx / w;
+    double2* x = sa.X + b * sa.ldx + (long long)c * sa.n + 3LL * s.col0 + i;
+    double2 v = *x;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (!cp.inv[k]) continue;
+      const int q = cp.inv[k][i / 3];
+      if (q >= 0) v = cadd(v, *tcol(sa, b, c, cp.toff[k] + 3 * q + i % 3));
     }
-    __syncthreads();
+    *x = v;
   }
 }
 
@@ -741,10 +754,18 @@ __global__ __launch_bounds__(256) void k_fwd_upd(SolveArgs sa, const RowTask* __
     for (int c = 0; c < PM; ++c)
       if (c < p) sa.X[b * sa.ldx + (long long)c * sa.n + 3LL * s.col0 + i] = *tcol(sa, b, c, s.toff + r + i);
   } else if (upd) {
+    // the update entry: the children's entries that land on row i, less L21 y
+    const ChildPull cp = child_pull(sa, s);
+    int q[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) q[k] = cp.inv[k] ? cp.inv[k][i / 3] : -1;
     for (int c = 0; c < PM; ++c)
       if (c < p) {
-        double2* tq = tcol(sa, b, c, s.toff + i - w);
-        *tq = csub(*tq, acc[c]);
+        double2 v = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+          if (q[k] >= 0) v = cadd(v, *tcol(sa, b, c, cp.toff[k] + 3 * q[k] + i % 3));
+        *tcol(sa, b, c, s.toff + i - w) = csub(v, acc[c]);
       }
   }
 }
@@ -969,6 +990,7 @@ void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs&
   sa.sn = P.d_sn.as<DevSn>();
   sa.struct_nodes = M.d_struct.as<int>();
   sa.relmap = M.d_relmap.as<int>();
+  sa.invmap = P.d_invmap.as<int>();
   sa.L = bb.L;
   sa.T = bb.T;
   sa.PE = P.panel_entries;
