@@ -118,13 +118,15 @@ __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const Asm
       rc[k] = 3 * cs.nstr;
     }
   }
-  for (int j = j0; j < j1; ++j) {
+  // wave v takes columns j0 + v, j0 + v + 4, ...: four columns in flight per block
+  const int lane = tid & 63;
+  for (int j = j0 + (tid >> 6); j < j1; j += 4) {
     int qj[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) qj[k] = inv[k] ? inv[k][j / 3] : -1;
     const int ioff = j < w ? 0 : w;  // row i of column j at col[i - ioff]
     double2* col = j < w ? Pn + (long long)j * m : Un + (long long)(j - w) * r;
-    for (int i = j + tid; i < m; i += 256) {
+    for (int i = j + lane; i < m; i += 64) {
       double2 v = make_double2(0.0, 0.0);
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
