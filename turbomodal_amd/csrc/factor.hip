@@ -199,10 +199,28 @@ __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __re
     }
     __syncthreads();
   }
+  // the inverse of the unit lower block, column c by forward substitution, kept
+  // transposed in the block's strictly upper triangle (no other kernel reads it)
+  // for the panel solves of this step
+  __shared__ double2 Li[NBMAX][NBMAX + 1];
+  if (tid < jb) {
+    const int c = tid;
+    Li[c][c] = make_double2(1.0, 0.0);
+    for (int i = c + 1; i < jb; ++i) {
+      double2 v = make_double2(0.0, 0.0);
+      for (int k = c; k < i; ++k) v = csub(v, cmul(A[i][k], Li[k][c]));
+      Li[i][c] = v;
+    }
+  }
+  __syncthreads();
   for (int idx = tid; idx < jb * jb; idx += 256) {
     int i = idx % jb, j = idx / jb;
-    if (i > j) Pn[(kb + i) + (long long)(kb + j) * m] = A[i][j];
-    else if (i == j) Pn[(kb + i) + (long long)(kb + j) * m] = make_double2(A[i][i].x, 0.0);
+    if (i > j) {
+      Pn[(kb + i) + (long long)(kb + j) * m] = A[i][j];
+      Pn[(kb + j) + (long long)(kb + i) * m] = Li[i][j];
+    } else if (i == j) {
+      Pn[(kb + i) + (long long)(kb + j) * m] = make_double2(A[i][i].x, 0.0);
+    }
   }
 }
 
@@ -216,26 +234,16 @@ __global__ __launch_bounds__(256) void k_panel_trsm(FactorArgs fa, const RowTask
   const int jb = min(NBMAX, w - kb);
   double2* Pn = fa.L + b * fa.PE + s.poff;
   // Li = inverse of the unit lower diagonal block (zero above the diagonal and
-  // outside jb x jb), so that x = b L11^-H is a product with static indices
-  __shared__ double2 Lk[NBMAX][NBMAX + 1];
+  // outside jb x jb), so that x = b L11^-H is a product with static indices;
+  // k_diag_ldl left it transposed in the block's strictly upper triangle
   __shared__ double2 Li[NBMAX][NBMAX + 1];
   __shared__ double dk[NBMAX];
   const int tid = threadIdx.x;
   for (int idx = tid; idx < NBMAX * NBMAX; idx += 256) {
-    const int i = idx % NBMAX, j = idx / NBMAX;
-    Lk[i][j] = (i > j && i < jb) ? Pn[(kb + i) + (long long)(kb + j) * m] : make_double2(0.0, 0.0);
-    Li[i][j] = make_double2(0.0, 0.0);
+    const int i = idx / NBMAX, j = idx % NBMAX;  // row i of Li is a contiguous piece of column kb + i
+    Li[i][j] = (i > j && i < jb) ? Pn[(kb + j) + (long long)(kb + i) * m]
+                                 : make_double2(i == j && i < jb ? 1.0 : 0.0, 0.0);
     if (i == j && i < jb) dk[i] = Pn[(kb + i) + (long long)(kb + i) * m].x;
-  }
-  __syncthreads();
-  if (tid < jb) {  // column c of the inverse by forward substitution
-    const int c = tid;
-    Li[c][c] = make_double2(1.0, 0.0);
-    for (int i = c + 1; i < jb; ++i) {
-      double2 v = make_double2(0.0, 0.0);
-      for (int k = c; k < i; ++k) v = csub(v, cmul(Lk[i][k], Li[k][c]));
-      Li[i][c] = v;
-    }
   }
   __syncthreads();
   const int i = t.r0 + tid;
