@@ -562,9 +562,14 @@ __global__ __launch_bounds__(256) void k_spmv_pair(int nf, int B, const int* __r
 #pragma unroll
     for (int rr = 0; rr < 3; ++rr) a1[j][rr] = a2[j][rr] = make_double2(0.0, 0.0);
   const int s1 = rowptr[I + 1];
-  if (cg < nc)
-    for (int s = rowptr[I] + sl; s < s1; s += 8) {
-      const long long J3 = 3LL * colidx[s];
+  if (cg < nc) {
+    int s = rowptr[I] + sl;
+    // the next pass's block column is loaded one pass ahead, so only the first
+    // pass waits for an index before its x gathers
+    int Jn = s < s1 ? colidx[s] : 0;
+    for (; s < s1; s += 8) {
+      const long long J3 = 3LL * Jn;
+      if (s + 8 < s1) Jn = colidx[s + 8];
       double2 xv[CPL][3];
 #pragma unroll
       for (int j = 0; j < CPL; ++j) {
@@ -594,6 +599,7 @@ __global__ __launch_bounds__(256) void k_spmv_pair(int nf, int B, const int* __r
           }
       }
     }
+  }
 #pragma unroll
   for (int j = 0; j < CPL; ++j)
 #pragma unroll
