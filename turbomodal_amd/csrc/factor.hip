@@ -700,11 +700,24 @@ __global__ __launch_bounds__(64) void k_fwd_inv(SolveArgs sa, const RowTask* __r
 #pragma unroll
           for (int c = 0; c < PM; ++c) cmac(acc[c], l, xs[jj][c]);
         }
-      } else {
-        const int jend = min(nj, i - j0 + 1);
-        for (int jj = 0; jj < jend; ++jj) {
+      } else {  // the diagonal chunk: the same pass, entries at or above the diagonal not loaded
+        const double2* lp = Pn + i + (long long)j0 * m;
+        int jj = 0;
+        for (; jj + 4 <= nj; jj += 4) {
+          double2 l[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int j = j0 + jj + u;
+            l[u] = j < i ? lp[(long long)(jj + u) * m] : make_double2(j == i ? 1.0 : 0.0, 0.0);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], xs[jj + u][c]);
+        }
+        for (; jj < nj; ++jj) {
           const int j = j0 + jj;
-          const double2 l = j == i ? make_double2(1.0, 0.0) : Pn[i + (long long)j * m];
+          const double2 l = j < i ? lp[(long long)jj * m] : make_double2(j == i ? 1.0 : 0.0, 0.0);
 #pragma unroll
           for (int c = 0; c < PM; ++c) cmac(acc[c], l, xs[jj][c]);
         }
