@@ -1604,7 +1604,7 @@ class BatchLanczos {
   DevBuf dptr_;
   int pb_ = 0, mmax_ = 0, nkeep_ = 0, nts_ = 0;
   long long ldv_ = 0, ldw_ = 0, ldk_ = 0;
-  DevBuf Vb_, MVb_, KVb_, Wb_, MWb_, KWb_, Xb_, KXb_, MXb_, tsp_, tsc_;
+  DevBuf Vb_, MVb_, KVb_, Wb_, MWb_, KWb_, Xb_, KXb_, MXb_, tsp_, tsc_, tsp2_, tsc2_;
   std::vector<DevBuf> kblk_;
   std::vector<const double2*> kptr_host_;
   DevBuf kptr_;
@@ -1626,8 +1626,8 @@ class BatchLanczos {
     for (DevBuf* b : {&Vb_, &MVb_, &KVb_}) b->alloc(pool(), sizeof(double2) * ldv_ * B_);
     for (DevBuf* b : {&Wb_, &MWb_, &KWb_}) b->alloc(pool(), sizeof(double2) * ldw_ * B_);
     for (DevBuf* b : {&Xb_, &KXb_, &MXb_}) b->alloc(pool(), sizeof(double2) * ldk_ * B_);
-    tsp_.alloc(pool(), sizeof(double2) * (size_t)B_ * mmax_ * 16 * nts_);
-    tsc_.alloc(pool(), sizeof(double2) * (size_t)B_ * mmax_ * 16);
+    for (DevBuf* b : {&tsp_, &tsp2_}) b->alloc(pool(), sizeof(double2) * (size_t)B_ * mmax_ * 16 * nts_);
+    for (DevBuf* b : {&tsc_, &tsc2_}) b->alloc(pool(), sizeof(double2) * (size_t)B_ * mmax_ * 16);
     Hm_.assign(B_, std::vector<std::complex<double>>((size_t)mmax_ * mmax_));
     Gm_.assign(B_, std::vector<std::complex<double>>((size_t)mmax_ * mmax_));
     prev_res_.assign(B_, std::vector<double>());
@@ -1671,14 +1671,36 @@ class BatchLanczos {
   // of columns nvec_ apart, returned on the host as [b][a][c] (rows of 16)
   std::vector<double2> tsgemm(const double2* A, long long lda, const DevBuf& ka, int kamax, const double2* Bm,
                               long long ldb, int kb, const DevBuf& act, long long len = -1) {
+    return tsgemm2(A, lda, ka, kamax, Bm, nullptr, ldb, kb, act, len).first;
+  }
+  // C_b = A_b^H B_b and, when B2 is given, C2_b = A_b^H B2_b from one pass over A
+  std::pair<std::vector<double2>, std::vector<double2>> tsgemm2(const double2* A, long long lda, const DevBuf& ka,
+                                                                int kamax, const double2* Bm, const double2* B2,
+                                                                long long ldb, int kb, const DevBuf& act,
+                                                                long long len = -1) {
     if (len < 0) len = nvec_;
-    KTimer t(ctx_, "gram", 0, 8.0 * len * kamax * kb * B_);
-    hipLaunchKernelGGL(k_tsgemm_partial, dim3(nts_, B_, (kamax + 15) / 16), dim3(256), 0, stream(), A, lda, Bm, ldb,
-                       len, nvec_, ka.as<int>(), kb, act.as<int>(), mmax_, tsp_.as<double2>(), nts_);
+    KTimer t(ctx_, "gram", 0, 8.0 * len * kamax * kb * B_ * (B2 ? 2 : 1));
+    const dim3 g(nts_, B_, (kamax + 15) / 16);
+    if (B2)
+      hipLaunchKernelGGL(k_tsgemm_partial<true>, g, dim3(256), 0, stream(), A, lda, Bm, B2, ldb, len, nvec_,
+                         ka.as<int>(), kb, act.as<int>(), mmax_, tsp_.as<double2>(), tsp2_.as<double2>(), nts_);
+    else
+      hipLaunchKernelGGL(k_tsgemm_partial<false>, g, dim3(256), 0, stream(), A, lda, Bm, (const double2*)nullptr,
+                         ldb, len, nvec_, ka.as<int>(), kb, act.as<int>(), mmax_, tsp_.as<double2>(),
+                         (double2*)nullptr, nts_);
     hipLaunchKernelGGL(k_tsgemm_reduce, dim3((kamax * 16 + 255) / 256, B_), dim3(256), 0, stream(),
                        tsp_.as<double2>(), ka.as<int>(), kb, act.as<int>(), mmax_, nts_, tsc_.as<double2>());
-    std::vector<double2> h((size_t)B_ * mmax_ * 16);
-    TM_HIP(hipMemcpyAsync(h.data(), tsc_.p, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, stream()));
+    if (B2)
+      hipLaunchKernelGGL(k_tsgemm_reduce, dim3((kamax * 16 + 255) / 256, B_), dim3(256), 0, stream(),
+                         tsp2_.as<double2>(), ka.as<int>(), kb, act.as<int>(), mmax_, nts_, tsc2_.as<double2>());
+    std::pair<std::vector<double2>, std::vector<double2>> h;
+    h.first.resize((size_t)B_ * mmax_ * 16);
+    TM_HIP(hipMemcpyAsync(h.first.data(), tsc_.p, sizeof(double2) * h.first.size(), hipMemcpyDeviceToHost, stream()));
+    if (B2) {
+      h.second.resize((size_t)B_ * mmax_ * 16);
+      TM_HIP(hipMemcpyAsync(h.second.data(), tsc2_.p, sizeof(double2) * h.second.size(), hipMemcpyDeviceToHost,
+                            stream()));
+    }
     TM_HIP(hipStreamSynchronize(stream()));
     return h;
   }
@@ -1833,9 +1855,11 @@ class BatchLanczos {
       }
     if (nvnmax > 0) {
       DevBuf dnvn = ints(nvn);
-      std::vector<double2> hk = tsgemm(Vb_.as<double2>(), ldv_, dnvn, nvnmax, KWb_.as<double2>(), ldw_, pb_, dok);
-      std::vector<double2> hm = tsgemm(Vb_.as<double2>(), ldv_, dnvn, nvnmax, (qep_ ? Wb_ : MWb_).as<double2>(),
-                                       ldw_, pb_, dok);
+      // V^H (K W) and V^H (M W) from one pass over V
+      auto hkm = tsgemm2(Vb_.as<double2>(), ldv_, dnvn, nvnmax, KWb_.as<double2>(),
+                         (qep_ ? Wb_ : MWb_).as<double2>(), ldw_, pb_, dok);
+      const std::vector<double2>& hk = hkm.first;
+      const std::vector<double2>& hm = hkm.second;
       using cd = std::complex<double>;
       for (int b = 0; b < B_; ++b) {
         if (!ok[b]) continue;
@@ -2425,10 +2449,10 @@ class BatchLanczos {
     DevBuf dr = ints(restart), dnk = ints(nk);
     for (int c0 = 0; c0 < nkeep_; c0 += 16) {
       const int kb = std::min(16, nkeep_ - c0);
-      std::vector<double2> hk = tsgemm(Vb_.as<double2>(), ldv_, dnk, nkeep_, KVb_.as<double2>() + (long long)c0 * nvec_,
-                                       ldv_, kb, dr);
-      std::vector<double2> hm = tsgemm(Vb_.as<double2>(), ldv_, dnk, nkeep_, MVb_.as<double2>() + (long long)c0 * nvec_,
-                                       ldv_, kb, dr);
+      auto hkm = tsgemm2(Vb_.as<double2>(), ldv_, dnk, nkeep_, KVb_.as<double2>() + (long long)c0 * nvec_,
+                         MVb_.as<double2>() + (long long)c0 * nvec_, ldv_, kb, dr);
+      const std::vector<double2>& hk = hkm.first;
+      const std::vector<double2>& hm = hkm.second;
       for (int b = 0; b < B_; ++b)
         if (restart[b])
           for (int a = 0; a < nk[b]; ++a)

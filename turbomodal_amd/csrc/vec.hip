@@ -433,12 +433,16 @@ __global__ void k_flag_cols(const int* __restrict__ ncol, int q, int B, int* __r
 
 // Tall-skinny C = A^H B, partial over row chunks.  A_b: n x ka (columns ldA apart,
 // problem stride lda_b), B_b: n x kb.  grid (nchunk, problems, ceil(ka/16)); each
-// block owns 16 columns of A and all kb columns of B for one chunk of rows.
+// block owns 16 columns of A and all kb columns of B for one chunk of rows.  With
+// TWO, also C2 = A^H B2 from the same staged rows of A.
+template <bool TWO>
 __global__ __launch_bounds__(256) void k_tsgemm_partial(const double2* __restrict__ A, long long lda_b,
-                                                        const double2* __restrict__ Bm, long long ldb_b,
-                                                        long long n, long long cs, const int* __restrict__ ka_b, int kb,
+                                                        const double2* __restrict__ Bm, const double2* __restrict__ Bm2,
+                                                        long long ldb_b, long long n, long long cs,
+                                                        const int* __restrict__ ka_b, int kb,
                                                         const int* __restrict__ act, int kamax,
-                                                        double2* __restrict__ part, int nchunk) {
+                                                        double2* __restrict__ part, double2* __restrict__ part2,
+                                                        int nchunk) {
   const int b = blockIdx.y;
   if (act && !act[b]) return;
   const int ka = ka_b[b];
@@ -446,31 +450,43 @@ __global__ __launch_bounds__(256) void k_tsgemm_partial(const double2* __restric
   if (a0 >= ka) return;
   const int na = min(16, ka - a0);
   __shared__ double2 As[64][17];
-  __shared__ double2 Bs[64][17];
+  __shared__ double2 Bs[TWO ? 2 : 1][64][17];
   const int tid = threadIdx.x;
   const int oa = tid / 16, oc = tid % 16;  // output (a0 + oa, oc)
-  double re = 0.0, im = 0.0;
+  double2 u1 = make_double2(0.0, 0.0), u2 = make_double2(0.0, 0.0);
   const long long r0 = (long long)blockIdx.x * kTsChunk;
   const long long r1 = r0 + kTsChunk < n ? r0 + kTsChunk : n;
   const double2* Ab = A + b * lda_b;
   const double2* Bb = Bm + b * ldb_b;
+  const double2* Bb2 = TWO ? Bm2 + b * ldb_b : nullptr;
   for (long long rr = r0; rr < r1; rr += 64) {
     const int nr = (int)(r1 - rr < 64 ? r1 - rr : 64);
     __syncthreads();
     for (int idx = tid; idx < 64 * 16; idx += 256) {
       const int r = idx % 64, c = idx / 64;
       As[r][c] = (r < nr && c < na) ? Ab[(long long)(a0 + c) * cs + rr + r] : make_double2(0.0, 0.0);
-      Bs[r][c] = (r < nr && c < kb) ? Bb[(long long)c * cs + rr + r] : make_double2(0.0, 0.0);
+      Bs[0][r][c] = (r < nr && c < kb) ? Bb[(long long)c * cs + rr + r] : make_double2(0.0, 0.0);
+      if (TWO) Bs[TWO ? 1 : 0][r][c] = (r < nr && c < kb) ? Bb2[(long long)c * cs + rr + r] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    double2 u = make_double2(re, im);
-    for (int r = 0; r < nr; ++r) cmacc(u, As[r][oa], Bs[r][oc]);
-    re = u.x;
-    im = u.y;
+    for (int r = 0; r < nr; ++r) {
+      const double2 av = As[r][oa];
+      cmacc(u1, av, Bs[0][r][oc]);
+      if (TWO) cmacc(u2, av, Bs[TWO ? 1 : 0][r][oc]);
+    }
   }
-  if (oa < na && oc < kb)
-    part[(((long long)b * kamax + a0 + oa) * 16 + oc) * nchunk + blockIdx.x] = make_double2(re, im);
+  if (oa < na && oc < kb) {
+    const long long o = (((long long)b * kamax + a0 + oa) * 16 + oc) * nchunk + blockIdx.x;
+    part[o] = u1;
+    if (TWO) part2[o] = u2;
+  }
 }
+template __global__ void k_tsgemm_partial<false>(const double2*, long long, const double2*, const double2*, long long,
+                                                 long long, long long, const int*, int, const int*, int, double2*,
+                                                 double2*, int);
+template __global__ void k_tsgemm_partial<true>(const double2*, long long, const double2*, const double2*, long long,
+                                                long long, long long, const int*, int, const int*, int, double2*,
+                                                double2*, int);
 
 // C[b][a][c] = sum over chunks (fixed order)
 __global__ void k_tsgemm_reduce(const double2* __restrict__ part, const int* __restrict__ ka_b, int kb,

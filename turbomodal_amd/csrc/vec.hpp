@@ -68,8 +68,10 @@ __global__ void k_copy_col(const double2* src, long long lds, long long n, int q
 __global__ void k_flag_cols(const int* ncol, int q, int B, int* act);
 
 // (A_b^H B_b) partial sums over row chunks of n rows; columns cs apart
-__global__ void k_tsgemm_partial(const double2* A, long long lda_b, const double2* Bm, long long ldb_b, long long n,
-                                 long long cs, const int* ka_b, int kb, const int* act, int kamax, double2* part, int nchunk);
+template <bool TWO>
+__global__ void k_tsgemm_partial(const double2* A, long long lda_b, const double2* Bm, const double2* Bm2,
+                                 long long ldb_b, long long n, long long cs, const int* ka_b, int kb, const int* act,
+                                 int kamax, double2* part, double2* part2, int nchunk);
 __global__ void k_tsgemm_reduce(const double2* part, const int* ka_b, int kb, const int* act, int kamax, int nchunk,
                                 double2* C);
 __global__ void k_tsmm(const double2* A, long long lda_b, long long n, long long cs, const double2* C, const int* ka_b, int kc,
