@@ -202,14 +202,29 @@ __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __re
   // the inverse of the unit lower block, column c by forward substitution, kept
   // transposed in the block's strictly upper triangle (no other kernel reads it)
   // for the panel solves of this step
+  // Eight lanes per column (a group of eight consecutive lanes of one wave): each
+  // row's dot product split over the group and summed by shuffles within it.  A
+  // group's LDS writes precede its next reads (one wave, in order; the waits keep
+  // the compiler from moving them).
   __shared__ double2 Li[NBMAX][NBMAX + 1];
-  if (tid < jb) {
-    const int c = tid;
-    Li[c][c] = make_double2(1.0, 0.0);
-    for (int i = c + 1; i < jb; ++i) {
-      double2 v = make_double2(0.0, 0.0);
-      for (int k = c; k < i; ++k) v = csub(v, cmul(A[i][k], Li[k][c]));
-      Li[i][c] = v;
+  {
+    const int c = tid >> 3, l8 = tid & 7;
+    if (c < jb) {
+      if (l8 == 0) Li[c][c] = make_double2(1.0, 0.0);
+      __builtin_amdgcn_s_waitcnt(0xc07f);
+      __builtin_amdgcn_wave_barrier();
+      for (int i = c + 1; i < jb; ++i) {
+        double2 v = make_double2(0.0, 0.0);
+        for (int k = c + l8; k < i; k += 8) v = csub(v, cmul(A[i][k], Li[k][c]));
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) {
+          v.x += __shfl_xor(v.x, o, 64);
+          v.y += __shfl_xor(v.y, o, 64);
+        }
+        if (l8 == 0) Li[i][c] = v;
+        __builtin_amdgcn_s_waitcnt(0xc07f);
+        __builtin_amdgcn_wave_barrier();
+      }
     }
   }
   __syncthreads();
@@ -588,8 +603,9 @@ __global__ __launch_bounds__(256) void k_inv_store(InvArgs ia, const int* __rest
   const int j0 = 64 * J, nj = min(64, w - j0);
   double2* Pn = ia.L + b * ia.PE + s.poff;
   const double2* Sc = ia.S + b * ia.SE + s.soff;
-  for (int c = 0; c < nj; ++c)
-    for (int i = j0 + c + 1 + threadIdx.x; i < w; i += 256)
+  // wave v copies columns v, v + 4, ...: four columns in flight per block
+  for (int c = threadIdx.x >> 6; c < nj; c += 4)
+    for (int i = j0 + c + 1 + (threadIdx.x & 63); i < w; i += 64)
       Pn[i + (long long)(j0 + c) * m] = Sc[i + (long long)c * w];
 }
 
