@@ -157,6 +157,33 @@ def test_batch_composition_does_not_change_results(ctx, method):
     em.close()
 
 
+def test_slot_groups_side_by_side_equal_serial(ctx):
+    # a call's slot groups (block sizes; a large run in two) run side by side on their own
+    # streams, or one after another with TMHIP_SERIAL_GROUPS=1: bit-identical results
+    import os
+    from turbomodal_amd import capi
+    X, T, S, N, _ = _case("c1")
+    m = oracle_model(X, T, S, N)
+    em = engine_mesh(ctx, m)
+    op = capi.EngineOperator(em, 200e9, 0.3, 7800.0, True)
+    probs = [dict(op=op, rpm=r, harmonic=k, num_modes=5) for r in (0.0, 5000.0, 10000.0) for k in range(N // 2 + 1)]
+    probs += [dict(op=op, rpm=7500.0, harmonic=k, num_modes=10) for k in range(N // 2 + 1)]
+    old = os.environ.pop("TMHIP_SERIAL_GROUPS", None)
+    try:
+        side = capi.solve_batch(ctx, probs)
+        os.environ["TMHIP_SERIAL_GROUPS"] = "1"
+        serial = capi.solve_batch(ctx, probs)
+    finally:
+        os.environ.pop("TMHIP_SERIAL_GROUPS", None)
+        if old is not None:
+            os.environ["TMHIP_SERIAL_GROUPS"] = old
+    for a, b in zip(side, serial):
+        assert np.array_equal(a.frequencies, b.frequencies)
+        assert np.array_equal(a.mode_shapes, b.mode_shapes)
+    op.close()
+    em.close()
+
+
 def test_rpm_sweep_matches_independent_solves(ctx):
     # one operator at unit omega serves every rpm: compare against oracles built per rpm
     from turbomodal_amd import capi
