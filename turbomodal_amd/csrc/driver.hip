@@ -2362,7 +2362,7 @@ class BatchLanczos {
         // |K||x|, |M||x| (and |D||x|) into the (consumed) W, MW (KW) buffers, pb
         // columns at a time; x is the b half for the Coriolis problem
         const double2* xs = Xb_.as<double2>() + (qep_ ? n_ : 0LL) + (long long)c0 * nvec_;
-        const dim3 ga((S_.nf + 3) / 4, B_, (cm + 7) / 8);
+        const dim3 ga((unsigned)(((S_.nf + 3) / 4) * (long long)B_), (cm + 7) / 8);  // (row quad, problem) x columns
         hipLaunchKernelGGL(k_spmv_abs, ga, dim3(256), 0, stream(), S_.nf, M_.d_rowptr.as<int>(),
                            M_.d_colidx.as<int>(), kptr_.as<const double2*>(), mk_ptrs_.as<const double2*>(),
                            qep_ ? dptr_.as<const double2*>() : (const double2* const*)nullptr, xs, ldk_, nvec_,

@@ -714,10 +714,12 @@ __global__ void k_copy_cols(const double2* __restrict__ src, long long lds, cons
 // Y_b[:, c] = |A_b| |X_b[:, c]| (entrywise moduli; real result in .x) for c < ncol_b
 // |A_q| |x| for up to three block arrays over the same columns (blocks3 optional):
 // Y_q[:, c] = |A_q| |X[:, c]| (real, in the .x part), c < ncol_b.  One wave per node
-// row, lanes = 8 block slots x 8 columns (columns 8 blockIdx.z ..), as k_spmv_pair:
+// row, lanes = 8 block slots x 8 columns (columns 8 blockIdx.y ..), as k_spmv_pair:
 // each block row is read once for all the columns and |x| is formed once for all
-// the arrays.  Moduli as sqrt(re^2 + im^2): operator entries and vectors are far from
-// the overflow range, and the products only size a rounding floor.
+// the arrays; (row quad, problem) tasks problem-fastest, each XCD a contiguous run.
+// Moduli as the hardware square root of re^2 + im^2 (no IEEE refinement): operator
+// entries and vectors are far from the overflow range, and the products only size a
+// rounding floor.
 __global__ __launch_bounds__(256) void k_spmv_abs(int nf, const int* __restrict__ rowptr,
                                                   const int* __restrict__ colidx,
                                                   const double2* const* __restrict__ blocks1,
@@ -726,10 +728,11 @@ __global__ __launch_bounds__(256) void k_spmv_abs(int nf, const int* __restrict_
                                                   const double2* __restrict__ X, long long ldx, long long cs,
                                                   const int* __restrict__ ncol, double2* __restrict__ Y1,
                                                   double2* __restrict__ Y2, double2* __restrict__ Y3, long long ldy) {
-  const int b = blockIdx.y;
-  const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int B = gridDim.x / ((nf + 3) / 4);
+  const int t = xcd_task(blockIdx.x, gridDim.x);
+  const int b = t % B, I = (t / B) * 4 + (threadIdx.x >> 6);
   if (I >= nf) return;
-  const int lane = threadIdx.x & 63, c = 8 * blockIdx.z + (lane & 7), sl = lane >> 3;
+  const int lane = threadIdx.x & 63, c = 8 * blockIdx.y + (lane & 7), sl = lane >> 3;
   const bool on = c < ncol[b];
   const double2* A[3] = {blocks1[b], blocks2[b], blocks3 ? blocks3[b] : nullptr};
   const double2* x = X + b * ldx + (long long)c * cs;
@@ -742,7 +745,7 @@ __global__ __launch_bounds__(256) void k_spmv_abs(int nf, const int* __restrict_
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         const double2 v = x[J3 + q];
-        ax[q] = sqrt(fma(v.x, v.x, v.y * v.y));
+        ax[q] = __builtin_amdgcn_sqrt(fma(v.x, v.x, v.y * v.y));
       }
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
@@ -753,7 +756,7 @@ __global__ __launch_bounds__(256) void k_spmv_abs(int nf, const int* __restrict_
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
             const double2 e = blk[3 * rr + q];
-            acc[m][rr] = fma(sqrt(fma(e.x, e.x, e.y * e.y)), ax[q], acc[m][rr]);
+            acc[m][rr] = fma(__builtin_amdgcn_sqrt(fma(e.x, e.x, e.y * e.y)), ax[q], acc[m][rr]);
           }
       }
     }
