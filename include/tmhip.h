@@ -209,6 +209,15 @@ void tm_server_destroy(tm_server* s);
 int64_t tm_problem_footprint(const tm_operator* op, int32_t num_modes,
                              int32_t precision, int32_t harmonic, int32_t method);
 int64_t tm_device_free_bytes(tm_context* ctx);
+/* Fraction of the device memory that tm_solve_batch keeps out of its batch
+ * sizing (default 0.15): batches hold at most (1 - reserve) x (free memory +
+ * the context's cached blocks) / tm_problem_footprint problems, and at least
+ * one.  The device counterpart of solve_at_rpm's memory_reserve_fraction
+ * (src/python_bindings.cpp:360), which caps concurrent harmonics against
+ * (1 - reserve) x host RAM (src/cyclic_solver.cpp:1257-1272).  Values outside
+ * [0, 1] return TM_ERR_ARG.  Applies to later tm_solve_batch calls on ctx;
+ * results do not depend on it, only how many problems share a batch. */
+int tm_set_memory_reserve(tm_context* ctx, double reserve_fraction);
 
 /* Diagnostics used by the parity tests: the assembled free-space matrices
  * K(k) (at rpm) and M(k) in COO form over free DOFs in the reference's free

@@ -184,6 +184,34 @@ def test_slot_groups_side_by_side_equal_serial(ctx):
     em.close()
 
 
+def test_memory_reserve_sizes_batches_not_results(ctx):
+    # memory_reserve_fraction (src/python_bindings.cpp:360) keeps device memory out of batch
+    # sizing; a reserve of 1 leaves one problem per batch (the reference's max(1, budget / per),
+    # src/cyclic_solver.cpp:1263), and the results stay bit-identical to full batches
+    from turbomodal_amd import capi
+    X, T, S, N, rpm = _case("c1")
+    m = oracle_model(X, T, S, N)
+    em = engine_mesh(ctx, m)
+    op = capi.EngineOperator(em, 200e9, 0.3, 7800.0, True)
+    probs = [dict(op=op, rpm=r, harmonic=k, num_modes=nm, method=meth)
+             for r, k, nm in ((0.0, 0, 10), (rpm, 7, 5), (9000.0, 18, 3), (rpm, 2, 5))
+             for meth in (capi.TM_METHOD_BLOCK, capi.TM_METHOD_LANCZOS)]
+    for bad in (-0.1, 1.5, float("nan")):
+        with pytest.raises(ValueError):
+            ctx.set_memory_reserve(bad)
+    try:
+        full = capi.solve_batch(ctx, probs)
+        ctx.set_memory_reserve(1.0)
+        single = capi.solve_batch(ctx, probs)
+    finally:
+        ctx.set_memory_reserve(0.15)
+    for a, b in zip(full, single):
+        assert np.array_equal(a.frequencies, b.frequencies)
+        assert np.array_equal(a.mode_shapes, b.mode_shapes)
+    op.close()
+    em.close()
+
+
 def test_rpm_sweep_matches_independent_solves(ctx):
     # one operator at unit omega serves every rpm: compare against oracles built per rpm
     from turbomodal_amd import capi

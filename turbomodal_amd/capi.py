@@ -34,7 +34,7 @@ EXPORTS = (
     "tm_mesh_get_info", "tm_operator_create", "tm_operator_destroy", "tm_solve_batch",
     "tm_problem_footprint", "tm_device_free_bytes", "tm_operator_export",
     "tm_operator_static_u", "tm_server_create", "tm_server_submit", "tm_server_wait",
-    "tm_server_destroy",
+    "tm_server_destroy", "tm_set_memory_reserve",
 )
 
 
@@ -100,6 +100,7 @@ def lib():
     L.tm_problem_footprint.argtypes = [vp, i32, i32, i32, i32]
     L.tm_problem_footprint.restype = i64
     L.tm_device_free_bytes.argtypes = [vp]
+    L.tm_set_memory_reserve.argtypes = [vp, dbl]
     L.tm_device_free_bytes.restype = i64
     L.tm_operator_export.argtypes = [vp, dbl, i32, i32, P(i64), P(i32), P(i32), P(dbl)]
     L.tm_operator_static_u.argtypes = [vp, P(dbl)]
@@ -168,6 +169,11 @@ class Context:
 
     def free_bytes(self) -> int:
         return int(self.L.tm_device_free_bytes(self.h))
+
+    def set_memory_reserve(self, fraction: float):
+        """Share of device memory kept out of solve-batch sizing (solve_at_rpm's
+        memory_reserve_fraction, src/python_bindings.cpp:360)."""
+        self.check(self.L.tm_set_memory_reserve(self.h, float(fraction)))
 
 
 class EngineMesh:

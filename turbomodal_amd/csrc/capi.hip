@@ -301,6 +301,15 @@ int64_t tm_problem_footprint(const tm_operator* op, int32_t num_modes, int32_t p
   }
 }
 
+int tm_set_memory_reserve(tm_context* ctx, double reserve_fraction) {
+  if (!ctx) return TM_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (!(reserve_fraction >= 0.0 && reserve_fraction <= 1.0))
+      throw ArgError("memory_reserve_fraction must lie in [0, 1]");
+    ctx->batch_fraction = 1.0 - reserve_fraction;
+  });
+}
+
 int64_t tm_device_free_bytes(tm_context* ctx) {
   if (!ctx) return -1;
   size_t f = 0, t = 0;

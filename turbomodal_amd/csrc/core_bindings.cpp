@@ -506,17 +506,17 @@ class CyclicSymmetrySolver {
                                         bool coriolis, double min_freq, py::object progress_cb,
                                         bool allow_condensation, double memory_reserve_fraction) {
     (void)max_threads;
-    (void)allow_condensation;
-    (void)memory_reserve_fraction;
-    auto res = solve_points({rpm}, num_modes, hi, coriolis, min_freq, progress_cb);
+    (void)allow_condensation;  // the reference's Guyan branch is not part of this engine
+    auto res = solve_points({rpm}, num_modes, hi, coriolis, min_freq, progress_cb, memory_reserve_fraction);
     return res.empty() ? std::vector<ModalResult>{} : res[0];
   }
 
   std::vector<std::vector<ModalResult>> solve_rpm_sweep(const std::vector<double>& rpms, int num_modes,
                                                         const std::vector<int>& hi, int max_threads,
-                                                        bool coriolis, double min_freq) {
+                                                        bool coriolis, double min_freq,
+                                                        double memory_reserve_fraction) {
     (void)max_threads;
-    return solve_points(rpms, num_modes, hi, coriolis, min_freq, py::none());
+    return solve_points(rpms, num_modes, hi, coriolis, min_freq, py::none(), memory_reserve_fraction);
   }
 
   // engine statistics of the cached mesh (symbolic analysis)
@@ -637,9 +637,12 @@ class CyclicSymmetrySolver {
 
   std::vector<std::vector<ModalResult>> solve_points(const std::vector<double>& rpms, int num_modes,
                                                      const std::vector<int>& hi, bool coriolis, double min_freq,
-                                                     py::object progress_cb) {
+                                                     py::object progress_cb, double memory_reserve_fraction) {
     ensure_engine();
     tm_context* ctx = Engine::get().ctx();
+    // device memory kept out of batch sizing, as the reference keeps host RAM
+    // out of its concurrency cap (src/cyclic_solver.cpp:1257-1272)
+    Engine::check(ctx, tm_set_memory_reserve(ctx, memory_reserve_fraction));
     const int max_k = mesh_.num_sectors / 2;
     std::vector<int> ks;
     if (hi.empty())
@@ -984,7 +987,8 @@ PYBIND11_MODULE(_core, m) {
            py::arg("allow_condensation") = false, py::arg("memory_reserve_fraction") = 0.2)
       .def("solve_rpm_sweep", &CyclicSymmetrySolver::solve_rpm_sweep, py::arg("rpm_values"),
            py::arg("num_modes_per_harmonic"), py::arg("harmonic_indices") = std::vector<int>{},
-           py::arg("max_threads") = 0, py::arg("include_coriolis") = false, py::arg("min_frequency") = 0.0)
+           py::arg("max_threads") = 0, py::arg("include_coriolis") = false, py::arg("min_frequency") = 0.0,
+           py::arg("memory_reserve_fraction") = 0.2)
       .def("engine_info", &CyclicSymmetrySolver::engine_info)
       .def_property(
           "engine_method",

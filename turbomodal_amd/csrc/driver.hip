@@ -2616,7 +2616,7 @@ static void run_block_groups(Context* ctx, Operator& op, std::vector<std::vector
     for (auto& q : runs) {
       TM_HIP(hipMemGetInfo(&free_b, &total_b));
       // blocks cached by the context's pool are reusable by this call
-      const int64_t budget = (int64_t)(0.85 * (double)(free_b + ctx->pool.cached()));
+      const int64_t budget = (int64_t)(ctx->batch_fraction * (double)(free_b + ctx->pool.cached()));
       const int cap = (int)std::max<int64_t>(1, std::min<int64_t>({64, (int64_t)q.size(), budget / per}));
       BatchLanczos bl(ctx, op, {}, true);
       bl.run_slots(q, cap, cap, qep);
@@ -2633,7 +2633,7 @@ static void run_block_groups(Context* ctx, Operator& op, std::vector<std::vector
     cached += (int64_t)gc[g]->pool.cached();
   }
   TM_HIP(hipMemGetInfo(&free_b, &total_b));
-  const double budget = 0.85 * (double)((int64_t)free_b + cached);
+  const double budget = ctx->batch_fraction * (double)((int64_t)free_b + cached);
   std::vector<std::exception_ptr> err(runs.size());
   std::vector<std::thread> th;
   for (size_t g = 0; g < runs.size(); ++g)
@@ -2740,7 +2740,7 @@ static void solve_problems_impl(Context* ctx, const tm_problem* problems, int n,
     size_t free_b = 0, total_b = 0;
     TM_HIP(hipMemGetInfo(&free_b, &total_b));
     // blocks cached by the context's pool are reusable by this call
-    int64_t budget = (int64_t)(0.85 * (double)(free_b + ctx->pool.cached()));
+    int64_t budget = (int64_t)(ctx->batch_fraction * (double)(free_b + ctx->pool.cached()));
     size_t max_batch = (size_t)std::max<int64_t>(1, std::min<int64_t>(64, budget / std::max<int64_t>(per, 1)));
     for (size_t i = 0; i < list.size();) {  // Lanczos: lockstep batches
       size_t j = i;

@@ -214,6 +214,9 @@ struct Context {
   DevPool pool;  // workspaces of the solve batches
   std::string last_error;
   bool profiling = false;
+  // share of the free device memory (plus this context's cached blocks) that a
+  // solve call sizes its batches against: 1 - memory reserve fraction
+  double batch_fraction = 0.85;
   std::map<std::string, KernelStat> stats;
   // pending timed launches: (name, start, stop, bytes, flops)
   struct Pending { std::string name; hipEvent_t a, b; double bytes, flops; };

@@ -246,7 +246,7 @@ def rpm_sweep(
     t0 = time.perf_counter()
     solver = _make_solver(mesh, material, fluid, hub_constraint, boundary_conditions)
     rows = solver.solve_rpm_sweep([float(r) for r in rpm_arr], num_modes, hi, max_threads, include_coriolis,
-                                  min_frequency)
+                                  min_frequency, memory_reserve_fraction=memory_reserve_fraction)
     out = [_enrich(_validate_modal_results(row)) for row in rows]
     if verbose >= PROGRESS:
         total = time.perf_counter() - t0
