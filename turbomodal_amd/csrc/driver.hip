@@ -650,6 +650,12 @@ void create_operator(Context* ctx, Mesh& M, Operator& op) {
   }
   launch_assemble(ctx, M, 2, false, ke, m_abs, m_raw, &s_ab, op.dKc);
   launch_assemble(ctx, M, 2, true, ke, m_abs, m_raw, &s_ab, op.dKp);
+  op.real_slot.alloc(nblk);
+  hipLaunchKernelGGL(k_real_slots, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, st, (long long)nblk,
+                     M.d_slot_row.as<int>(), M.d_colidx.as<int>(), M.d_tslot.as<int>(),
+                     M.d_pinned.as<unsigned char>(), op.Kp.as<double>(), op.dKp.as<double>(), op.Mp.as<double>(),
+                     op.real_slot.as<unsigned char>());
+  TM_HIP(hipGetLastError());
   TM_HIP(hipStreamSynchronize(st));
 }
 
@@ -1270,7 +1276,8 @@ class BatchLanczos {
     for (int q0 = 0; q0 < nemax; q0 += bm_.pcols)
       trisolve_batch(ctx_, M_, P_, bb, Y + (long long)q0 * n_, ldx, act_.as<int>(), std::min(bm_.pcols, nemax - q0),
                      -1, nact);
-    spmv_pair(kp, &mk_ptrs_, Y, ldx, dnr, nemax, KY, MY, false, false);  // K Y, M Y in one pass
+    if (!qep_ && &kp == &kptr_) spmv_km(Y, ldx, dnr, nemax, KY, MY);  // K Y, M Y in one pass
+    else spmv_pair(kp, &mk_ptrs_, Y, ldx, dnr, nemax, KY, MY, false, false);
     const size_t gsz = (size_t)B_ * nemax * nemax * nchunk_;
     DevBuf pk(pool(), sizeof(double2) * gsz), pm(pool(), sizeof(double2) * gsz);
     hipLaunchKernelGGL(k_gram_partial, dim3(nchunk_, B_, nemax * nemax), dim3(256), 0, stream(), Y,
@@ -1608,6 +1615,8 @@ class BatchLanczos {
   std::vector<DevBuf> kblk_;
   std::vector<const double2*> kptr_host_;
   DevBuf kptr_;
+  std::vector<SplitKM> sk_host_;  // per slot: the split K/M product's operands
+  DevBuf sk_;
   std::vector<int> iter_;  // slot is iterating
   std::vector<int> nv_, y0_;
   std::vector<std::vector<std::complex<double>>> Hm_, Gm_;  // mmax x mmax per problem
@@ -1639,9 +1648,12 @@ class BatchLanczos {
 
   // FP64 K(k) blocks of the problems in the listed slots (residuals, Rayleigh-Ritz)
   void build_stiffness_blocks(const std::vector<int>& slots) {
+    if (sk_host_.size() != (size_t)B_) sk_host_.assign(B_, SplitKM{});
     for (int b : slots) {
       HostProblem* hp = probs_[b];
       const Operator& sop = *static_cast<const Operator*>(hp->in.op);
+      sk_host_[b] = SplitKM{sop.Kc.as<double>(), sop.dKc.as<double>(), sop.Mc.as<double>(),
+                            sop.real_slot.as<unsigned char>(), hp->s};
       if (!kblk_[b].p) kblk_[b].alloc(pool(), sizeof(double2) * 9 * op_.nblk);
       hipLaunchKernelGGL(k_build_phase_blocks, dim3((9 * (long long)op_.nblk + 255) / 256), dim3(256), 0, stream(),
                          (long long)op_.nblk, M_.d_slot_row.as<int>(), M_.d_colidx.as<int>(), M_.d_tslot.as<int>(),
@@ -1662,6 +1674,7 @@ class BatchLanczos {
     }
     TM_HIP(hipGetLastError());
     kptr_ = upload(pool(), kptr_host_, stream());
+    sk_ = upload(pool(), sk_host_, stream());
     if (qep_) dptr_ = upload(pool(), dptr_host_, stream());
   }
 
@@ -1745,6 +1758,27 @@ class BatchLanczos {
     }
   }
 
+  // Y1 = K(k) X and Y2 = M(k) X of the Hermitian pencil in one pass, real slots
+  // from the operators' shared arrays (k_spmv_km)
+  void spmv_km(const double2* X, long long ld, const DevBuf& ncol, int cmax, double2* Y1, double2* Y2) {
+    KTimer t(ctx_, "spmv_m", 0, 0);
+    const long long cstride = cs_ > 0 ? cs_ : n_;
+    const long long ntask = (long long)((S_.nf + 3) / 4) * B_;
+    if (ntask > std::numeric_limits<int>::max()) throw ArgError("spmv: too many (row quad, problem) tasks for one launch");
+    for (int c0 = 0; c0 < cmax; c0 += 16) {
+      if (cmax - c0 > 8)
+        hipLaunchKernelGGL(k_spmv_km<2>, dim3((unsigned)ntask), dim3(256), 0, stream(), S_.nf, B_,
+                           M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), sk_.as<SplitKM>(),
+                           kptr_.as<const double2*>(), mk_ptrs_.as<const double2*>(), X, ld, cstride,
+                           ncol.as<int>(), c0, Y1, Y2, ld);
+      else
+        hipLaunchKernelGGL(k_spmv_km<1>, dim3((unsigned)ntask), dim3(256), 0, stream(), S_.nf, B_,
+                           M_.d_rowptr.as<int>(), M_.d_colidx.as<int>(), sk_.as<SplitKM>(),
+                           kptr_.as<const double2*>(), mk_ptrs_.as<const double2*>(), X, ld, cstride,
+                           ncol.as<int>(), c0, Y1, Y2, ld);
+    }
+  }
+
   // The Rayleigh-Ritz pencil (A, B) with B positive definite: (K, M) with
   // eigenvalues lambda = omega^2, or for the Coriolis problem on z = [a; b]
   // (a = omega x, b = x) the definite pencil (Bc, L2), Bc = [[0, M], [M, -D]],
@@ -1765,7 +1799,7 @@ class BatchLanczos {
   // and L2 z = [M a; K b] need only K b and D b besides.
   void apply_AB(const double2* X, long long ld, const DevBuf& ncol, int cmax, double2* AX, double2* BX) {
     if (!qep_) {
-      spmv_pair(kptr_, &mk_ptrs_, X, ld, ncol, cmax, AX, BX, false, false);
+      spmv_km(X, ld, ncol, cmax, AX, BX);
       return;
     }
     spmv_cols(mk_ptrs_, X, ld, ncol, 2 * cmax, BX, false, n_, 2);  // [M a; M b]

@@ -274,9 +274,9 @@ __global__ void k_build_phase_blocks(long long nslot, const int* __restrict__ sl
   const long long e1 = idx, e2 = tt * 9 + 3 * cj + ci;
   double c = Xc[e1], p = Xp[e1], q = Xp[e2];
   if (dXc) {
-    c += s * dXc[e1];
-    p += s * dXp[e1];
-    q += s * dXp[e2];
+    c = fma(s, dXc[e1], c);
+    p = fma(s, dXp[e1], p);
+    q = fma(s, dXp[e2], q);
   }
   // symmetric X: Xc + P Xp + conj(P) Xp^T; skew X: Xc + P Xp - conj(P) Xp^T
   double2 v = skew ? make_double2(c + pre * (p - q), pim * (p + q)) : make_double2(c + pre * (p + q), pim * (p - q));
@@ -647,6 +647,138 @@ template __global__ void k_spmv_pair<1>(int, int, const int*, const int*, const 
 template __global__ void k_spmv_pair<2>(int, int, const int*, const int*, const double2* const*,
                                         const double2* const*, const double2*, long long, long long, const int*, int,
                                         int, double2*, double2*, long long, int, int);
+
+// A slot is real when the phase parts of K and M vanish at it and at its
+// transpose slot (Kp, dKp, Mp all zero) and its row and column nodes have no
+// pinned DOF: its K(k) block is then Kc + s dKc and its M(k) block Mc for every
+// harmonic, with zero imaginary parts (k_build_phase_blocks gives exactly these).
+__global__ void k_real_slots(long long nslot, const int* __restrict__ slot_row, const int* __restrict__ colidx,
+                             const int* __restrict__ tslot, const unsigned char* __restrict__ pinned,
+                             const double* __restrict__ Kp, const double* __restrict__ dKp,
+                             const double* __restrict__ Mp, unsigned char* __restrict__ real) {
+  const long long sl = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (sl >= nslot) return;
+  const int I = slot_row[sl], J = colidx[sl];
+  const long long t = tslot[sl];
+  bool r = true;
+  for (int c = 0; c < 3; ++c) r = r && !pinned[3 * I + c] && !pinned[3 * J + c];
+  for (int e = 0; e < 9; ++e)
+    r = r && Kp[sl * 9 + e] == 0.0 && Kp[t * 9 + e] == 0.0 && dKp[sl * 9 + e] == 0.0 && dKp[t * 9 + e] == 0.0 &&
+        Mp[sl * 9 + e] == 0.0 && Mp[t * 9 + e] == 0.0;
+  real[sl] = r ? 1 : 0;
+}
+
+// Y1 = K(k) X and Y2 = M(k) X for the Hermitian pencil, as k_spmv_pair, with the
+// real slots read from the operator's shared real arrays (Kc + s dKc, Mc; 216 B
+// per slot, the same for every problem of the operator and so found in L2 by the
+// problems after the first) instead of the problem's complex blocks (288 B per
+// slot and problem).  A real slot's products are the complex path's with the zero
+// imaginary terms left out: bit-identical sums.
+template <int CPL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_spmv_km(int nf, int B, const int* __restrict__ rowptr,
+                                                 const int* __restrict__ colidx, const SplitKM* __restrict__ sk,
+                                                 const double2* const* __restrict__ kblocks,
+                                                 const double2* const* __restrict__ mblocks,
+                                                 const double2* __restrict__ X, long long ldx, long long n,
+                                                 const int* __restrict__ ncol, int c0, double2* __restrict__ Y1,
+                                                 double2* __restrict__ Y2, long long ldy) {
+  const int t = xcd_task(blockIdx.x, gridDim.x);
+  const int b = t % B, I = (t / B) * 4 + (threadIdx.x >> 6);
+  const int nc = min(8 * CPL, ncol[b] - c0);
+  if (nc <= 0 || I >= nf) return;  // whole waves leave together
+  const int lane = threadIdx.x & 63, cg = lane & 7, sl = lane >> 3;
+  const SplitKM q = sk[b];
+  const double2* A1 = kblocks[b];
+  const double2* A2 = mblocks[b];
+  const double2* x = X + b * ldx + (long long)(c0 + cg) * n;
+  double2 a1[CPL][3], a2[CPL][3];
+#pragma unroll
+  for (int j = 0; j < CPL; ++j)
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) a1[j][rr] = a2[j][rr] = make_double2(0.0, 0.0);
+  const int s1 = rowptr[I + 1];
+  if (cg < nc) {
+    int s = rowptr[I] + sl;
+    int Jn = s < s1 ? colidx[s] : 0, Rn = s < s1 ? q.real[s] : 0;
+    for (; s < s1; s += 8) {
+      const long long J3 = 3LL * Jn;
+      const int re = Rn;
+      if (s + 8 < s1) {
+        Jn = colidx[s + 8];
+        Rn = q.real[s + 8];
+      }
+      double2 xv[CPL][3];
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) {
+        const bool on = cg + 8 * j < nc;
+        const double2* xj = x + (long long)(8 * j) * n + J3;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) xv[j][k] = on ? xj[k] : make_double2(0.0, 0.0);
+      }
+      if (re) {
+        const double* kc = q.kc + (long long)s * 9;
+        const double* kd = q.dkc + (long long)s * 9;
+        const double* mc = q.mc + (long long)s * 9;
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const double kv = fma(q.s, kd[3 * rr + k], kc[3 * rr + k]);
+            const double mv = mc[3 * rr + k];
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+              a1[j][rr].x = fma(kv, xv[j][k].x, a1[j][rr].x);
+              a1[j][rr].y = fma(kv, xv[j][k].y, a1[j][rr].y);
+              a2[j][rr].x = fma(mv, xv[j][k].x, a2[j][rr].x);
+              a2[j][rr].y = fma(mv, xv[j][k].y, a2[j][rr].y);
+            }
+          }
+      } else {
+        const double2* e1 = A1 + (long long)s * 9;
+        const double2* e2 = A2 + (long long)s * 9;
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const double2 u = e1[3 * rr + k], v = e2[3 * rr + k];
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+              cmac(a1[j][rr], u, xv[j][k]);
+              cmac(a2[j][rr], v, xv[j][k]);
+            }
+          }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < CPL; ++j)
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+      for (int o = 8; o < 64; o <<= 1) {
+        a1[j][rr].x += __shfl_xor(a1[j][rr].x, o, 64);
+        a1[j][rr].y += __shfl_xor(a1[j][rr].y, o, 64);
+        a2[j][rr].x += __shfl_xor(a2[j][rr].x, o, 64);
+        a2[j][rr].y += __shfl_xor(a2[j][rr].y, o, 64);
+      }
+  if (sl != 0) return;
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+    if (cg + 8 * j >= nc) break;
+    const long long o = b * ldy + (long long)(c0 + cg + 8 * j) * n + 3LL * I;
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) {
+      Y1[o + rr] = a1[j][rr];
+      Y2[o + rr] = a2[j][rr];
+    }
+  }
+}
+template __global__ void k_spmv_km<1>(int, int, const int*, const int*, const SplitKM*, const double2* const*,
+                                      const double2* const*, const double2*, long long, long long, const int*, int,
+                                      double2*, double2*, long long);
+template __global__ void k_spmv_km<2>(int, int, const int*, const int*, const SplitKM*, const double2* const*,
+                                      const double2* const*, const double2*, long long, long long, const int*, int,
+                                      double2*, double2*, long long);
 
 // Coriolis pencil: from BX = [M a; M b] (per column), AX = [M b; M a] (the
 // -D b term is added after), c < ncol_b, n rows per half, columns cs apart

@@ -84,6 +84,23 @@ __global__ void k_spmv_pair(int nf, int B, const int* rowptr, const int* colidx,
                             const double2* const* blocks2, const double2* X, long long ldx, long long n,
                             const int* ncol, int cmul, int c0, double2* Y1, double2* Y2, long long ldy, int acc1,
                             int acc2);
+// Per-problem operands of k_spmv_km: the problem's operator's real slot arrays
+// (K base and its omega^2 derivative, M), its real-slot flags, and omega^2
+struct SplitKM {
+  const double* kc;
+  const double* dkc;
+  const double* mc;
+  const unsigned char* real;
+  double s;
+};
+__global__ void k_real_slots(long long nslot, const int* slot_row, const int* colidx, const int* tslot,
+                             const unsigned char* pinned, const double* Kp, const double* dKp, const double* Mp,
+                             unsigned char* real);
+template <int CPL>
+__global__ void k_spmv_km(int nf, int B, const int* rowptr, const int* colidx, const SplitKM* sk,
+                          const double2* const* kblocks, const double2* const* mblocks, const double2* X,
+                          long long ldx, long long n, const int* ncol, int c0, double2* Y1, double2* Y2,
+                          long long ldy);
 __global__ void k_qep_swap_halves(const double2* BX, double2* AX, long long ld, long long n, long long cs,
                                   const int* ncol);
 __global__ void k_axpy_half(const double2* src, long long lds, const int* s0v, long long soff, double2* dst,
