@@ -745,6 +745,86 @@ __global__ __launch_bounds__(64) void k_fwd_inv(SolveArgs sa, const RowTask* __r
       if (c < p) *tcol(sa, b, c, s.toff + 3 * s.nstr + i) = acc[c];
 }
 
+// k_fwd_inv with the column chunks of a task dealt round-robin to four waves
+// (their partial sums added in wave order): for the upper tree levels, whose
+// few tasks with long rows leave most SIMDs idle under one wave per task
+template <int PM>
+__global__ __launch_bounds__(256) void k_fwd_inv4(SolveArgs sa, const RowTask* __restrict__ tasks, int t0) {
+  const int b = blockIdx.y;
+  if (sa.active && !sa.active[b]) return;
+  const RowTask t = tasks[t0 + blockIdx.x];
+  const DevSn s = sa.sn[t.sn];
+  const int w = 3 * s.ncol, m = w + 3 * s.nstr, p = sa.p;
+  const double2* Pn = sa.L + b * sa.PE + s.poff;
+  const double2* x = sa.X + b * sa.ldx + 3LL * s.col0;
+  __shared__ double2 xs[4][64][PM];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = t.r0 + lane;
+  double2 acc[PM];
+  for (int c = 0; c < PM; ++c) acc[c] = make_double2(0.0, 0.0);
+  const int nch = (t.r1 + 63) / 64;
+  for (int q = 0; q < nch; q += 4) {  // the same number of rounds for every wave
+    const int j0 = 64 * (q + wv);
+    const int nj = q + wv < nch ? min(64, t.r1 - j0) : 0;
+    __syncthreads();
+    for (int idx = lane; idx < 64 * PM; idx += 64) {
+      const int jj = idx % 64, c = idx / 64;
+      xs[wv][jj][c] = (jj < nj && c < p) ? x[(long long)c * sa.n + j0 + jj] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    if (i < t.r1 && nj > 0) {
+      const double2* lp = Pn + i + (long long)j0 * m;
+      if (j0 + nj <= t.r0) {  // strictly below this task's diagonal
+        int jj = 0;
+        for (; jj + 4 <= nj; jj += 4) {
+          double2 l[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) l[u] = lp[(long long)(jj + u) * m];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], xs[wv][jj + u][c]);
+        }
+        for (; jj < nj; ++jj) {
+          const double2 l = lp[(long long)jj * m];
+#pragma unroll
+          for (int c = 0; c < PM; ++c) cmac(acc[c], l, xs[wv][jj][c]);
+        }
+      } else {  // the diagonal chunk: entries at or above the diagonal not loaded
+        int jj = 0;
+        for (; jj + 4 <= nj; jj += 4) {
+          double2 l[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int j = j0 + jj + u;
+            l[u] = j < i ? lp[(long long)(jj + u) * m] : make_double2(j == i ? 1.0 : 0.0, 0.0);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], xs[wv][jj + u][c]);
+        }
+        for (; jj < nj; ++jj) {
+          const int j = j0 + jj;
+          const double2 l = j < i ? lp[(long long)jj * m] : make_double2(j == i ? 1.0 : 0.0, 0.0);
+#pragma unroll
+          for (int c = 0; c < PM; ++c) cmac(acc[c], l, xs[wv][jj][c]);
+        }
+      }
+    }
+  }
+  __syncthreads();  // every wave is done with its tile: the tiles now hold the partial sums
+  if (wv > 0)
+    for (int c = 0; c < PM; ++c) xs[wv][lane][c] = acc[c];
+  __syncthreads();
+  if (wv == 0 && i < t.r1) {
+    for (int v = 1; v < 4; ++v)
+      for (int c = 0; c < PM; ++c) acc[c] = cadd(acc[c], xs[v][lane][c]);
+    for (int c = 0; c < PM; ++c)
+      if (c < p) *tcol(sa, b, c, s.toff + 3 * s.nstr + i) = acc[c];
+  }
+}
+
 // rows < w: write the pivot solution back into x; rows >= w: t -= L21 y
 template <int PM>
 __global__ __launch_bounds__(256) void k_fwd_upd(SolveArgs sa, const RowTask* __restrict__ tasks, int t0) {
@@ -999,13 +1079,20 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
   TM_HIP(hipGetLastError());
 }
 
+constexpr int kFwdInvSplit = 96;  // tasks per level at or below which k_fwd_inv4 runs
+
 template <int PM>
 static void trisolve_levels(Context* ctx, const Plan& P, const SolveArgs& sa, int B) {
   const int* lsn = P.d_level_sn.as<int>();
   for (const LevelPlan& L : P.levels) {
     hipLaunchKernelGGL(k_fwd_asm, dim3(L.nsn, B), dim3(256), 0, ctx->stream, sa, lsn, L.sn0);
-    if (L.nfinv)
+    // levels with few pivot-row tasks (the upper tree) split each task's columns
+    // over four waves; the choice depends on the level only, never on the batch
+    if (L.nfinv > kFwdInvSplit)
       hipLaunchKernelGGL(k_fwd_inv<PM>, dim3(L.nfinv, B), dim3(64), 0, ctx->stream, sa, P.d_finv.as<RowTask>(),
+                         L.finv0);
+    else if (L.nfinv)
+      hipLaunchKernelGGL(k_fwd_inv4<PM>, dim3(L.nfinv, B), dim3(256), 0, ctx->stream, sa, P.d_finv.as<RowTask>(),
                          L.finv0);
     if (L.nfup)
       hipLaunchKernelGGL(k_fwd_upd<PM>, dim3(L.nfup, B), dim3(256), 0, ctx->stream, sa, P.d_fup.as<RowTask>(),
