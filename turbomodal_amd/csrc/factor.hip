@@ -889,6 +889,85 @@ __global__ __launch_bounds__(256) void k_fwd_upd(SolveArgs sa, const RowTask* __
   }
 }
 
+// k_fwd_upd with the pivot columns dealt in 64-column chunks round-robin to NG
+// groups of 256 threads (their partial sums added in group order), for the upper
+// tree levels, whose few long-row tasks leave most SIMDs idle
+template <int PM, int NG>
+__global__ __launch_bounds__(256 * NG) void k_fwd_upd_split(SolveArgs sa, const RowTask* __restrict__ tasks, int t0) {
+  const int b = blockIdx.y;
+  if (sa.active && !sa.active[b]) return;
+  const RowTask t = tasks[t0 + blockIdx.x];
+  const DevSn s = sa.sn[t.sn];
+  const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r, p = sa.p;
+  const double2* Pn = sa.L + b * sa.PE + s.poff;
+  constexpr int CH = 64;
+  constexpr int SH = NG * CH * PM > 256 * PM ? NG * CH * PM : 256 * PM;
+  __shared__ double2 sh[SH];  // NG tiles of CH x PM solution entries, then the partial sums
+  const int g = threadIdx.x >> 8, tr = threadIdx.x & 255;
+  double2* ys = sh + g * CH * PM;
+  const int i = t.r0 + tr;
+  const bool upd = i < t.r1 && i >= w;
+  double2 acc[PM];
+  for (int c = 0; c < PM; ++c) acc[c] = make_double2(0.0, 0.0);
+  if (t.r1 > w) {
+    const int nch = (w + CH - 1) / CH;
+    for (int q = 0; q < nch; q += NG) {  // the same number of rounds for every group
+      const int j0 = CH * (q + g);
+      const int nj = q + g < nch ? min(CH, w - j0) : 0;
+      __syncthreads();
+      for (int idx = tr; idx < CH * PM; idx += 256) {
+        const int jj = idx % CH, c = idx / CH;
+        ys[jj * PM + c] = (jj < nj && c < p) ? *tcol(sa, b, c, s.toff + r + j0 + jj) : make_double2(0.0, 0.0);
+      }
+      __syncthreads();
+      if (upd && nj > 0) {
+        const double2* lp = Pn + i + (long long)j0 * m;
+        int jj = 0;
+        for (; jj + 8 <= nj; jj += 8) {
+          double2 l[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) l[u] = lp[(long long)(jj + u) * m];
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], ys[(jj + u) * PM + c]);
+        }
+        for (; jj < nj; ++jj) {
+          const double2 l = lp[(long long)jj * m];
+#pragma unroll
+          for (int c = 0; c < PM; ++c) cmac(acc[c], l, ys[jj * PM + c]);
+        }
+      }
+    }
+  }
+  for (int v = 1; v < NG; ++v) {  // group v's partial sums into group 0, in group order
+    __syncthreads();
+    if (g == v)
+      for (int c = 0; c < PM; ++c) sh[tr * PM + c] = acc[c];
+    __syncthreads();
+    if (g == 0)
+      for (int c = 0; c < PM; ++c) acc[c] = cadd(acc[c], sh[tr * PM + c]);
+  }
+  if (g != 0) return;
+  if (i < t.r1 && i < w) {
+    for (int c = 0; c < PM; ++c)
+      if (c < p) sa.X[b * sa.ldx + (long long)c * sa.n + 3LL * s.col0 + i] = *tcol(sa, b, c, s.toff + r + i);
+  } else if (upd) {
+    const ChildPull cp = child_pull(sa, s);
+    int q[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) q[k] = cp.inv[k] ? cp.inv[k][i / 3] : -1;
+    for (int c = 0; c < PM; ++c)
+      if (c < p) {
+        double2 v = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+          if (q[k] >= 0) v = cadd(v, *tcol(sa, b, c, cp.toff[k] + 3 * q[k] + i % 3));
+        *tcol(sa, b, c, s.toff + i - w) = csub(v, acc[c]);
+      }
+  }
+}
+
 // backward: the solved ancestor values x[struct(q)] of every front of the level,
 // gathered once into the front's update-vector slots (free after the forward
 // sweep), so the column dot products below read them contiguously
@@ -1080,6 +1159,10 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
 }
 
 constexpr int kFwdInvSplit = 96;  // tasks per level at or below which k_fwd_inv4 runs
+// k_fwd_upd_split runs on levels of kFwdUpdSplitLo..kFwdUpdSplitHi tasks (measured on
+// the bladed sector: 10 tasks 287 -> 151 us, 17 tasks 355 -> 306 us, but 59 to 121
+// tasks slower, up to 383 -> 617 us, and 3 tasks 6 -> 10 us)
+constexpr int kFwdUpdSplitLo = 8, kFwdUpdSplitHi = 24;
 
 template <int PM>
 static void trisolve_levels(Context* ctx, const Plan& P, const SolveArgs& sa, int B) {
@@ -1094,9 +1177,13 @@ static void trisolve_levels(Context* ctx, const Plan& P, const SolveArgs& sa, in
     else if (L.nfinv)
       hipLaunchKernelGGL(k_fwd_inv4<PM>, dim3(L.nfinv, B), dim3(256), 0, ctx->stream, sa, P.d_finv.as<RowTask>(),
                          L.finv0);
-    if (L.nfup)
+    constexpr int NG = PM <= 8 ? 4 : 2;
+    if (L.nfup > kFwdUpdSplitHi || (L.nfup && L.nfup < kFwdUpdSplitLo))
       hipLaunchKernelGGL(k_fwd_upd<PM>, dim3(L.nfup, B), dim3(256), 0, ctx->stream, sa, P.d_fup.as<RowTask>(),
                          L.fup0);
+    else if (L.nfup)
+      hipLaunchKernelGGL((k_fwd_upd_split<PM, NG>), dim3(L.nfup, B), dim3(256 * NG), 0, ctx->stream, sa,
+                         P.d_fup.as<RowTask>(), L.fup0);
   }
   for (int h = static_cast<int>(P.levels.size()) - 1; h >= 0; --h) {
     const LevelPlan& L = P.levels[h];
