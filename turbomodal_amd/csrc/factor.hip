@@ -1158,7 +1158,9 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
   TM_HIP(hipGetLastError());
 }
 
-constexpr int kFwdInvSplit = 96;  // tasks per level at or below which k_fwd_inv4 runs
+// tasks per level at or below which k_fwd_inv4 runs (measured on the bladed sector:
+// 11 to 52 tasks 242-169 -> 99-147 us, but 61 to 76 tasks slower, 46 -> 102 us at 76)
+constexpr int kFwdInvSplit = 56;
 // k_fwd_upd_split runs on levels of kFwdUpdSplitLo..kFwdUpdSplitHi tasks (measured on
 // the bladed sector: 10 tasks 287 -> 151 us, 17 tasks 355 -> 306 us, but 59 to 121
 // tasks slower, up to 383 -> 617 us, and 3 tasks 6 -> 10 us)
