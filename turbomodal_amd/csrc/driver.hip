@@ -655,6 +655,10 @@ void create_operator(Context* ctx, Mesh& M, Operator& op) {
                      M.d_slot_row.as<int>(), M.d_colidx.as<int>(), M.d_tslot.as<int>(),
                      M.d_pinned.as<unsigned char>(), op.Kp.as<double>(), op.dKp.as<double>(), op.Mp.as<double>(),
                      op.real_slot.as<unsigned char>());
+  op.real_pack.alloc(sizeof(double2) * 14 * nblk);
+  hipLaunchKernelGGL(k_pack_real_slots, dim3((unsigned)((14 * nblk + 255) / 256)), dim3(256), 0, st,
+                     (long long)nblk, op.Kc.as<double>(), op.dKc.as<double>(), op.Mc.as<double>(),
+                     op.real_pack.as<double2>());
   TM_HIP(hipGetLastError());
   TM_HIP(hipStreamSynchronize(st));
 }
@@ -1652,8 +1656,7 @@ class BatchLanczos {
     for (int b : slots) {
       HostProblem* hp = probs_[b];
       const Operator& sop = *static_cast<const Operator*>(hp->in.op);
-      sk_host_[b] = SplitKM{sop.Kc.as<double>(), sop.dKc.as<double>(), sop.Mc.as<double>(),
-                            sop.real_slot.as<unsigned char>(), hp->s};
+      sk_host_[b] = SplitKM{sop.real_pack.as<double2>(), sop.real_slot.as<unsigned char>(), hp->s};
       if (!kblk_[b].p) kblk_[b].alloc(pool(), sizeof(double2) * 9 * op_.nblk);
       hipLaunchKernelGGL(k_build_phase_blocks, dim3((9 * (long long)op_.nblk + 255) / 256), dim3(256), 0, stream(),
                          (long long)op_.nblk, M_.d_slot_row.as<int>(), M_.d_colidx.as<int>(), M_.d_tslot.as<int>(),

@@ -125,6 +125,7 @@ struct Operator {
   size_t nblk = 0;
   DevBuf Kc, Kp, dKc, dKp, Mc, Mp, Gc, Gp;
   DevBuf real_slot;  // per slot: K(k), M(k) blocks real and unpinned for every harmonic (k_real_slots)
+  DevBuf real_pack;  // per slot: Kc, dKc, Mc packed for the split K/M product (k_pack_real_slots)
   std::vector<double> u_static;       // 3*nn at unit omega
   // harmonic -> complex M(k) blocks, built on the stream of their first user;
   // `ready` marks the end of the build for the other streams

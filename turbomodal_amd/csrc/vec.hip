@@ -668,10 +668,25 @@ __global__ void k_real_slots(long long nslot, const int* __restrict__ slot_row, 
   real[sl] = r ? 1 : 0;
 }
 
+// the real slot arrays of an operator packed for k_spmv_km: per slot (Kc_e, dKc_e)
+// for its 9 entries, then Mc_0..8 and a zero (224 B, one 16-byte-aligned record)
+__global__ void k_pack_real_slots(long long nslot, const double* __restrict__ Kc, const double* __restrict__ dKc,
+                                  const double* __restrict__ Mc, double2* __restrict__ pack) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= 14 * nslot) return;
+  const long long sl = idx / 14;
+  const int u = (int)(idx - 14 * sl);
+  double2 v;
+  if (u < 9) v = make_double2(Kc[sl * 9 + u], dKc[sl * 9 + u]);
+  else if (u < 13) v = make_double2(Mc[sl * 9 + 2 * (u - 9)], Mc[sl * 9 + 2 * (u - 9) + 1]);
+  else v = make_double2(Mc[sl * 9 + 8], 0.0);
+  pack[idx] = v;
+}
+
 // Y1 = K(k) X and Y2 = M(k) X for the Hermitian pencil, as k_spmv_pair, with the
-// real slots read from the operator's shared real arrays (Kc + s dKc, Mc; 216 B
-// per slot, the same for every problem of the operator and so found in L2 by the
-// problems after the first) instead of the problem's complex blocks (288 B per
+// real slots read from the operator's packed real arrays (Kc + s dKc, Mc; 224 B
+// per slot in 14 16-byte loads, the same for every problem of the operator and so
+// found in L2 by the problems after the first) instead of the problem's complex blocks (288 B per
 // slot and problem).  A real slot's products are the complex path's with the zero
 // imaginary terms left out: bit-identical sums.
 template <int CPL>
@@ -716,15 +731,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         for (int k = 0; k < 3; ++k) xv[j][k] = on ? xj[k] : make_double2(0.0, 0.0);
       }
       if (re) {
-        const double* kc = q.kc + (long long)s * 9;
-        const double* kd = q.dkc + (long long)s * 9;
-        const double* mc = q.mc + (long long)s * 9;
+        const double2* pk = q.pack + (long long)s * 14;
+        double mcv[10];
+#pragma unroll
+        for (int u = 0; u < 5; ++u) {
+          const double2 m2 = pk[9 + u];
+          mcv[2 * u] = m2.x;
+          mcv[2 * u + 1] = m2.y;
+        }
 #pragma unroll
         for (int rr = 0; rr < 3; ++rr)
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
-            const double kv = fma(q.s, kd[3 * rr + k], kc[3 * rr + k]);
-            const double mv = mc[3 * rr + k];
+            const double2 kk = pk[3 * rr + k];
+            const double kv = fma(q.s, kk.y, kk.x);
+            const double mv = mcv[3 * rr + k];
 #pragma unroll
             for (int j = 0; j < CPL; ++j) {
               a1[j][rr].x = fma(kv, xv[j][k].x, a1[j][rr].x);

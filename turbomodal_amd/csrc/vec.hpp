@@ -87,12 +87,12 @@ __global__ void k_spmv_pair(int nf, int B, const int* rowptr, const int* colidx,
 // Per-problem operands of k_spmv_km: the problem's operator's real slot arrays
 // (K base and its omega^2 derivative, M), its real-slot flags, and omega^2
 struct SplitKM {
-  const double* kc;
-  const double* dkc;
-  const double* mc;
+  const double2* pack;  // per slot 14 double2: (Kc, dKc) of its 9 entries, then Mc and a zero
   const unsigned char* real;
   double s;
 };
+__global__ void k_pack_real_slots(long long nslot, const double* Kc, const double* dKc, const double* Mc,
+                                  double2* pack);
 __global__ void k_real_slots(long long nslot, const int* slot_row, const int* colidx, const int* tslot,
                              const unsigned char* pinned, const double* Kp, const double* dKp, const double* Mp,
                              unsigned char* real);
