@@ -2404,7 +2404,8 @@ class BatchLanczos {
                            M_.d_colidx.as<int>(), kptr_.as<const double2*>(), mk_ptrs_.as<const double2*>(),
                            qep_ ? dptr_.as<const double2*>() : (const double2* const*)nullptr, xs, ldk_, nvec_,
                            dnc2.as<int>(), Wb_.as<double2>(), MWb_.as<double2>(),
-                           qep_ ? KWb_.as<double2>() : (double2*)nullptr, ldw_);
+                           qep_ ? KWb_.as<double2>() : (double2*)nullptr, ldw_,
+                           qep_ || !sk_.p ? (const SplitKM*)nullptr : sk_.as<SplitKM>());
         hipLaunchKernelGGL(k_floor_partial, dim3(nchunk_, B_, cm), dim3(256), 0, stream(), Wb_.as<double2>(),
                            qep_ ? KWb_.as<double2>() : (const double2*)nullptr, MWb_.as<double2>(), ldw_, n_, nvec_,
                            dl.as<double>() + c0, nkeep_, dnc2.as<int>(), fpart.as<double>() + (size_t)c0 * nchunk_,

@@ -870,7 +870,8 @@ __global__ void k_copy_cols(const double2* __restrict__ src, long long lds, cons
 // row, lanes = 8 block slots x 8 columns (columns 8 blockIdx.y ..), as k_spmv_pair:
 // each block row is read once for all the columns and |x| is formed once for all
 // the arrays; (row quad, problem) tasks problem-fastest, each XCD a contiguous run.
-// Moduli as the hardware square root of re^2 + im^2 (no IEEE refinement): operator
+// With sk (the Hermitian pencil: no third array), real slots read the packed real
+// arrays as k_spmv_km does.  Moduli as the hardware square root of re^2 + im^2 (no IEEE refinement): operator
 // entries and vectors are far from the overflow range, and the products only size a
 // rounding floor.
 __global__ __launch_bounds__(256) void k_spmv_abs(int nf, const int* __restrict__ rowptr,
@@ -880,7 +881,8 @@ __global__ __launch_bounds__(256) void k_spmv_abs(int nf, const int* __restrict_
                                                   const double2* const* __restrict__ blocks3,
                                                   const double2* __restrict__ X, long long ldx, long long cs,
                                                   const int* __restrict__ ncol, double2* __restrict__ Y1,
-                                                  double2* __restrict__ Y2, double2* __restrict__ Y3, long long ldy) {
+                                                  double2* __restrict__ Y2, double2* __restrict__ Y3, long long ldy,
+                                                  const SplitKM* __restrict__ sk) {
   const int B = gridDim.x / ((nf + 3) / 4);
   const int t = xcd_task(blockIdx.x, gridDim.x);
   const int b = t % B, I = (t / B) * 4 + (threadIdx.x >> 6);
@@ -899,6 +901,26 @@ __global__ __launch_bounds__(256) void k_spmv_abs(int nf, const int* __restrict_
       for (int q = 0; q < 3; ++q) {
         const double2 v = x[J3 + q];
         ax[q] = __builtin_amdgcn_sqrt(fma(v.x, v.x, v.y * v.y));
+      }
+      if (sk && sk[b].real[s]) {  // real K and M entries (pair k_spmv_km): their moduli are |entry|
+        const double2* pk = sk[b].pack + (long long)s * 14;
+        const double sb = sk[b].s;
+        double mcv[10];
+#pragma unroll
+        for (int u = 0; u < 5; ++u) {
+          const double2 m2 = pk[9 + u];
+          mcv[2 * u] = m2.x;
+          mcv[2 * u + 1] = m2.y;
+        }
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const double2 kk = pk[3 * rr + q];
+            acc[0][rr] = fma(fabs(fma(sb, kk.y, kk.x)), ax[q], acc[0][rr]);
+            acc[1][rr] = fma(fabs(mcv[3 * rr + q]), ax[q], acc[1][rr]);
+          }
+        continue;
       }
 #pragma unroll
       for (int m = 0; m < 3; ++m) {

@@ -111,7 +111,7 @@ __global__ void k_head_dot(const double2* X, const double2* Y, long long ldb, lo
 __global__ void k_spmv_abs(int nf, const int* rowptr, const int* colidx, const double2* const* blocks1,
                            const double2* const* blocks2, const double2* const* blocks3, const double2* X,
                            long long ldx, long long cs, const int* ncol, double2* Y1, double2* Y2, double2* Y3,
-                           long long ldy);
+                           long long ldy, const SplitKM* sk);
 __global__ void k_floor_partial(const double2* AK, const double2* AD, const double2* AM, long long ldb, long long n,
                                 long long cs, const double* lam,
                                 int lam_ld, const int* ncol, double* part, int nchunk);
