@@ -54,40 +54,97 @@ def workload_spec(workload):
     return np.array([0.0]), 5
 
 
-def measured_traffic(args):
-    """HBM bytes per triangular-sweep launch of this workload, as the FETCH_SIZE and
-    WRITE_SIZE counters read it (rocprofv3 --pmc, scaled by a plain copy kernel);
-    recorded with its commands in profiles/ by a separate profiled run, since the
-    counters cannot be read inside the timed run."""
+def source_hash():
+    """SHA-1 of the engine's sources (turbomodal_amd/csrc, sorted by name): a traffic
+    figure measured on other sources is not reported."""
+    import hashlib
+    h = hashlib.sha1()
+    d = os.path.join(ROOT, "turbomodal_amd", "csrc")
+    for name in sorted(os.listdir(d)):
+        with open(os.path.join(d, name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()
+
+
+def measured_traffic(args, key):
+    """HBM bytes per launch of the factorization ('factor') or the triangular-sweep pair
+    ('trisolve') of this workload, as the FETCH_SIZE and WRITE_SIZE counters read them
+    (rocprofv3 --pmc, scaled by a plain copy kernel), recorded with their commands in
+    profiles/ by separate profiled runs (the counters cannot be read inside the timed
+    run).  None unless the profile was taken on these very sources."""
     if args.workload != "c4" or args.method != "block" or args.coriolis:
         return None
     path = os.path.join(ROOT, "profiles", "traffic_c4_rpm_sweep_fp64_block.md")
+    vals = {}
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":")[1])
+            if ":" in line and not line.startswith(("|", " ", "#")):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return None
+    if vals.get("source_hash") != source_hash():
+        return None
+    v = vals.get(f"{key}_traffic_bytes_per_launch")
+    return float(v) if v else None
 
 
-def cpu_baseline(sm, rpm, num_modes, ks=(0, 1)):
-    """The oracle (numpy/scipy restatement of the reference algorithm, the
-    reference's own nev/ncv/tolerance policy) on a bounded sample, one core."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
+def _oracle_solve(args):
+    """One harmonic of the oracle (worker of the all-core baseline; the operators come
+    from the parent through fork)."""
+    from threadpoolctl import threadpool_limits
+    from oracle import cyclic
+    k, num_modes, rpm = args
+    with threadpool_limits(limits=1):  # one BLAS thread per process (the pool exists since import)
+        t = time.perf_counter()
+        cyclic.solve_harmonic(_BASE["m"], _BASE["ops"], k, num_modes, rpm, "reference")
+        return time.perf_counter() - t
+
+
+_BASE = {}
+
+
+def cpu_baseline(sm, mat, rpm, num_modes):
+    """The oracle (numpy/scipy restatement of the reference algorithm with the reference's
+    own nev/ncv/tolerance policy, one thread per process) on one rpm point of the workload:
+    the operator build, then all N/2 + 1 harmonics over the host cores (one process per
+    harmonic, as the reference runs one thread per harmonic, src/cyclic_solver.cpp:1229-1231),
+    and two harmonics on one core."""
+    import multiprocessing as mp
+    os.environ["OMP_NUM_THREADS"] = "1"
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"
     from oracle import cyclic
     t0 = time.perf_counter()
     m = cyclic.build_model(sm.nodes, sm.elements, sm.node_sets, sm.num_sectors)
-    ops = cyclic.sector_operators(m, 200e9, 0.3, 7800.0, rpm)
+    ops = cyclic.sector_operators(m, mat["E"], mat["nu"], mat["rho"], rpm)
     t_ops = time.perf_counter() - t0
-    sols = {}
-    t1 = time.perf_counter()
-    for k in ks:
-        sols[k] = cyclic.solve_harmonic(m, ops, k, num_modes, rpm, "reference")
-    t_solve = (time.perf_counter() - t1) / len(ks)
+    _BASE.update(m=m, ops=ops)
     n_h = sm.num_sectors // 2 + 1
-    rate = n_h / (t_ops + n_h * t_solve)
-    return rate, t_ops, t_solve, sols, m, ops
+    workers = max(1, min(16, os.cpu_count() or 1, n_h))   # the box's CPU share for one GPU
+    t1 = time.perf_counter()
+    with mp.get_context("fork").Pool(workers) as pool:
+        per = pool.map(_oracle_solve, [(k, num_modes, rpm) for k in range(n_h)], chunksize=1)
+    t_all = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    for k in (0, 1):
+        _oracle_solve((k, num_modes, rpm))
+    t_one = (time.perf_counter() - t2) / 2
+    return {"t_ops": t_ops, "t_all": t_all, "workers": workers, "t_one": t_one, "n_h": n_h,
+            "per_harmonic_s": per}
+
+
+def host_description():
+    """Host CPU of this run: logical CPUs and the model name (lscpu)."""
+    import subprocess
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
 
 
 def run_c5(args):
@@ -207,6 +264,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64"])
     ap.add_argument("--method", default="block", choices=["block", "lanczos"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 one-at-a-time measurement")
     ap.add_argument("--serial-groups", action="store_true",
                     help="run the block-size slot groups of a sweep one after another instead of side by side "
                          "(every sweep; the kernels then run alone on the GPU, as in the roofline sweep)")
@@ -231,6 +289,7 @@ def main():
         return
 
     tm, sm, mesh, mat, solver = build_solver(args.workload, args.precision)
+    from turbomodal_amd import meshgen
     solver.engine_method = args.method
     rpms, num_modes = workload_spec(args.workload)
     n_problems = len(rpms) * (sm.num_sectors // 2 + 1)
@@ -281,30 +340,40 @@ def main():
         core.engine_set_profiling(False)
         del os.environ["TMHIP_SERIAL_GROUPS"]
 
-    def tri_rate(st):
-        tr = st.get("trisolve", {})
+    def rate(st, key):
+        tr = st.get(key, {})
         if not tr.get("launches"):
             return None
         avg = tr["total_ms"] / tr["launches"]
-        per = tr["alg_bytes"] / tr["launches"]
-        return avg, per, per / (avg * 1e-3) / 1e9
+        return avg, tr["alg_bytes"] / tr["launches"], tr["alg_flops"] / tr["launches"]
 
+    measured_on = ("the timed steps (groups serial)" if args.serial_groups else
+                   "one sweep after the timed steps with the block-size groups serial")
+    roofline = roofline_tri = None
+    fr = rate(kstats, "factor")
+    if fr:
+        avg_ms, _, flops_per = fr
+        ach = flops_per / (avg_ms * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "kernel": "factorization (batched supernodal LDL^H: front assembly, diagonal "
+                                              "blocks, panel solves, Schur tiles on v_mfma_f64_16x16x4, pivot-block "
+                                              "inversion)",
+                    "achieved": round(ach, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": measured_traffic(args, "factor"),
+                    "avg_launch_ms": round(avg_ms, 3), "alg_flops_per_launch": flops_per,
+                    "measured_on": measured_on}
+    tr = rate(kstats, "trisolve")
+    if tr:
+        avg_ms, bytes_per, _ = tr
+        ach = bytes_per / (avg_ms * 1e-3) / 1e9
+        roofline_tri = {"bound": "hbm", "kernel": "trisolve (L, D, L^H sweep pair)", "achieved": round(ach, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                        "traffic": measured_traffic(args, "trisolve"), "avg_launch_ms": round(avg_ms, 3),
+                        "alg_bytes_per_launch": bytes_per, "measured_on": measured_on}
+        if not args.serial_groups and rate(stats, "trisolve"):
+            a2, b2, _ = rate(stats, "trisolve")
+            roofline_tri["frac_timed_region_groups_concurrent"] = round(b2 / (a2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     tri = stats.get("trisolve", {})
-    fac = kstats.get("factor", {})
-    roofline = None
-    if tri_rate(kstats):
-        avg_ms, bytes_per, ach = tri_rate(kstats)
-        roofline = {"bound": "hbm", "kernel": "trisolve (L, D, L^H sweep pair)", "achieved": round(ach, 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                    "traffic": measured_traffic(args), "avg_launch_ms": round(avg_ms, 3),
-                    "alg_bytes_per_launch": bytes_per,
-                    "measured_on": ("the timed steps (groups serial)" if args.serial_groups else
-                                    "one sweep after the timed steps with the block-size groups serial")}
-        if not args.serial_groups and tri_rate(stats):
-            roofline["frac_timed_region_groups_concurrent"] = round(tri_rate(stats)[2] / HBM_PEAK_GBS, 4)
-    factor_rate = None
-    if fac.get("launches"):
-        factor_rate = fac["alg_flops"] / (fac["total_ms"] * 1e-3) / 1e12
+    all_res = np.concatenate([np.asarray(r.residuals) for row in results for r in row])
 
     out = {
         "metric": METRIC,
@@ -318,7 +387,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c128 (complex f64)",
-        "data": "synthetic (seeded Kuhn TET10 lattice meshes, steel)",
+        "data": "synthetic (seeded Kuhn TET10 lattice meshes, %s)" % ("titanium" if args.workload == "c3" else "steel"),
         "config": {"workload": {"c4": "rpm sweep bladed sector 20 rpm x 19 ND", "c2": "bladed sector 19 ND @10000 rpm",
                                 "c1": "flat disk 19 ND @0 rpm",
                                 "c3": "doubled bladed sector (Ti) 19 ND @12000 rpm"}[args.workload]
@@ -331,8 +400,11 @@ def main():
         "p99_latency_ms": round(1e3 * float(np.percentile(lat, 99)), 1),
         "solved": n_ok,
         "max_residual": max_res,
+        "residuals": {"pairs": int(all_res.size), "above_1e-8": int(np.sum(all_res > 1e-8)),
+                      "max": float(all_res.max())},
         "roofline": roofline,
-        "factor_tflops": None if factor_rate is None else round(factor_rate, 2),
+        "roofline_trisolve": roofline_tri,
+        "source_hash": source_hash(),
         "kernels": {k: {"launches": v["launches"], "ms": round(v["total_ms"], 2)} for k, v in stats.items()},
         "sweeps_per_step": tri.get("launches", 0) / args.steps,
         "restarts": {"max": int(max(r.restarts for row in results for r in row)),
@@ -342,31 +414,62 @@ def main():
         "cpu_baseline": None,
     }
 
-    if args.workload == "c3" or args.coriolis:  # hours at 330k DOF; the oracle sample is the Hermitian path
+    # accuracy of the timed results against the oracle's answers on a spread sample
+    # (rpm 0, 7895, 15000 x ND 0, 9, 18; tests/golden/make_golden.py)
+    if args.workload == "c4" and args.method == "block" and not args.coriolis:
+        with np.load(os.path.join(ROOT, "tests", "golden", "c4_sample.npz"), allow_pickle=False) as z:
+            gold = {k: z[k] for k in z.files}
+        sub = gold["subset"]
+        errs, macs = [], []
+        for ir in (0, 10, 19):
+            row = {r.harmonic_index: r for r in results[ir]}
+            for k in (0, 9, 18):
+                r = row[k]
+                f_ref = gold[f"r{ir}_k{k}_freq"]
+                errs.append(float(np.max(np.abs(np.asarray(r.frequencies) / f_ref - 1))))
+                sh = np.asarray(r.mode_shapes)[sub]
+                g = gold[f"r{ir}_k{k}_shape"]
+                for q in range(len(f_ref)):
+                    a, b = sh[:, q], g[:, q]
+                    macs.append(float(abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)))
+        out["accuracy_vs_oracle"] = {"problems": len(errs), "sample": "rpm {0, 7895, 15000} x ND {0, 9, 18}",
+                                     "max_freq_rel_err": max(errs), "min_mac_on_dof_subset": min(macs)}
+
+    # the plain FP64 path: single-vector Lanczos, one problem per call, over one rpm point
+    if not args.no_plain and args.workload in ("c4", "c2"):
+        rpm_p = float(rpms[len(rpms) * 2 // 3])
+        solver.engine_method = "lanczos"
+        n_h = sm.num_sectors // 2 + 1
+        tp = time.perf_counter()
+        for k in range(n_h):
+            solver.solve_at_rpm(rpm_p, num_modes, [k], include_coriolis=args.coriolis)
+        plain = n_h / (time.perf_counter() - tp)
+        solver.engine_method = args.method
+        out["plain_fp64_one_at_a_time"] = {"value": round(plain, 3), "unit": "ND solves/s", "problems": n_h,
+                                           "sample": f"ND 0-{n_h - 1} at {rpm_p:.0f} rpm, one solve_at_rpm call each",
+                                           "eigensolver": "lanczos (single vector)"}
+        out["speedup_vs_plain_fp64"] = round(value / plain, 2)
+        print(f"[bench] plain FP64 one at a time {plain:.2f} ND solves/s", file=sys.stderr, flush=True)
+
+    if args.workload == "c3" or args.coriolis:  # the oracle at 330k DOF / on the QEP takes minutes per harmonic
         args.no_cpu_baseline = True
     if not args.no_cpu_baseline:
         rpm_s = float(rpms[len(rpms) * 2 // 3])
-        print(f"[bench] cpu baseline sample at {rpm_s:.0f} rpm", file=sys.stderr, flush=True)
-        rate, t_ops, t_solve, sols, m, ops = cpu_baseline(sm, rpm_s, num_modes)
+        print(f"[bench] cpu baseline at {rpm_s:.0f} rpm", file=sys.stderr, flush=True)
+        mat = meshgen.TITANIUM if args.workload == "c3" else meshgen.STEEL
+        cb = cpu_baseline(sm, mat, rpm_s, num_modes)
+        n_h = cb["n_h"]
+        rate_all = n_h / (cb["t_ops"] + cb["t_all"])
+        rate_one = n_h / (cb["t_ops"] + n_h * cb["t_one"])
         out["cpu_baseline"] = {
-            "value": round(rate, 4), "unit": "ND solves/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle restatement (scipy SuperLU + numpy Lanczos, reference policy) on the same mesh at "
-                       f"{rpm_s:.0f} rpm: operator build {t_ops:.1f}s + ND 0,1 solves {t_solve:.1f}s each; rate = "
-                       f"19 ND / (build + 19 solves)")}
-        # accuracy of the GPU results on the sampled problems
-        ir = list(rpms).index(rpm_s)
-        errs, macs = [], []
-        from turbomodal_amd.mac import compute_mac
-        for r in results[ir]:
-            if r.harmonic_index in sols:
-                ref = sols[r.harmonic_index]
-                nm = min(len(ref.frequencies), len(r.frequencies))
-                errs.append(float(np.max(np.abs(np.asarray(r.frequencies)[:nm] / ref.frequencies[:nm] - 1))))
-                sh = np.asarray(r.mode_shapes)
-                for q in range(nm):
-                    macs.append(float(compute_mac(sh[:, q], ref.mode_shapes[:, q])[0, 0]))
-        out["accuracy_vs_oracle"] = {"problems": len(errs), "max_freq_rel_err": max(errs) if errs else None,
-                                     "min_mac": min(macs) if macs else None}
+            "value": round(rate_all, 4), "unit": "ND solves/s", "cores": cb["workers"], "kind": "port",
+            "sample": (f"oracle restatement (scipy SuperLU + numpy Lanczos, the reference's policy) on the same mesh, "
+                       f"one rpm point ({rpm_s:.0f} rpm): operator build {cb['t_ops']:.1f}s, then ND 0-{n_h - 1} over "
+                       f"{cb['workers']} processes (one thread each) in {cb['t_all']:.1f}s; rate = {n_h} ND / "
+                       f"(build + solves)"),
+            "one_core": {"value": round(rate_one, 4), "unit": "ND solves/s", "cores": 1,
+                         "sample": f"ND 0, 1 at {cb['t_one']:.1f}s each; rate = {n_h} ND / (build + {n_h} x that)"},
+            "host": host_description()}
     print(json.dumps(out), flush=True)
 
 

@@ -52,9 +52,11 @@ extern "C" {
 #define TM_NODE_LEFT 1
 #define TM_NODE_RIGHT 2
 
-/* precision of the shift-invert factor (north-star mixed precision) */
+/* precision of the shift-invert factor */
 #define TM_PREC_FP64 0       /* complex FP64 factor, FP64 everywhere */
-#define TM_PREC_MIXED 1      /* complex FP32 factor, FP64 refinement + Rayleigh-Ritz */
+#define TM_PREC_MIXED 1      /* complex FP32 factor with FP64 refinement: not provided by this
+                                engine; tm_solve_batch and tm_problem_footprint reject it
+                                (TM_ERR_ARG / -1) */
 
 /* eigen-solver convergence policy */
 #define TM_CONV_TIGHT 0      /* relative Ritz residual <= tol_rel (default 1e-12) */
@@ -174,10 +176,24 @@ typedef struct tm_result {
   int32_t restarts;          /* out: implicit restarts used */
   int32_t status;            /* out: TM_OK or failure (harmonic dropped by caller) */
   double seconds;            /* out: wall time of the batch this problem ran in */
+  int32_t factor_retries;    /* out: solves repeated alone after a failure (tm_solve_batch) */
+  int32_t pivot_flags;       /* out: status of the last factorization of the problem: bit 0
+                                a zero or non-finite pivot, bit 1 a near-singular pivot
+                                (|d_j| <= 1e-10 |a_jj| and <= 1e-3 |shift M_jj|: the shift
+                                sits on an eigenvalue); either fails that attempt */
+  double shift_used;         /* out: shift of the factorization behind the result: the
+                                problem's shift, moved by 1 % of max(|shift|, 1) per retry
+                                of a problem whose factor was flagged; 0 for the Coriolis
+                                pencil (it factors K_eff(k)) */
 } tm_result;
 
 /* Solve a batch: problems sharing an operator run together (one symbolic
- * analysis, batched numeric factorizations and Lanczos).  Blocking. */
+ * analysis, batched numeric factorizations and Lanczos).  Blocking.
+ * A problem that fails is solved again alone, as CyclicSymmetrySolver::solve_at_rpm
+ * retries failed harmonics (src/cyclic_solver.cpp:1334-1342); where its LDL^H met a
+ * zero, non-finite or near-singular pivot (pivot_flags != 0), the retry moves the
+ * shift (up to three times), where the reference falls back to LU on a zero pivot
+ * (src/hermitian_lanczos.cpp:27-55).  Still failing: status != TM_OK. */
 int tm_solve_batch(tm_context* ctx, const tm_problem* problems, int32_t n,
                    tm_result* results);
 
@@ -224,11 +240,31 @@ int tm_set_memory_reserve(tm_context* ctx, double reserve_fraction);
  * ordering (interior DOFs, then left DOFs, node-ascending; constrained DOFs
  * removed).  Call with rows == NULL to get the entry count. */
 int tm_operator_export(tm_operator* op, double rpm, int32_t harmonic,
-                       int32_t which /* 0 = K_eff(k), 1 = M(k), 2 = G(k) */,
+                       int32_t which /* 0 = K_eff(k), 1 = M(k), 2 = G(k) = G_c + P G_p - conj(P) G_p^T,
+                                        the skew projection behind D(k) = i 2 Omega G(k) */,
                        int64_t* nnz, int32_t* rows, int32_t* cols, double* vals_c128);
 /* Static prestress displacement at unit omega (3*n_nodes), the reference's
  * static_centrifugal(1.0). */
 int tm_operator_static_u(tm_operator* op, double* u_full);
+
+/* Grounded springs of ELASTIC_SUPPORT constraint groups: spring_full holds 3*n_nodes
+ * stiffnesses (N/m, >= 0) added to the diagonal of the sector stiffness K before the
+ * cyclic projection, as CyclicSymmetrySolver::build_spring_stiffness and solve_at_rpm
+ * step 1b (src/cyclic_solver.cpp:193-212, 747-751); a right-boundary node's spring acts
+ * on its left partner through the rotation R(alpha).  The static prestress solve keeps
+ * the base K, as the reference's static_centrifugal does (src/cyclic_solver.cpp:456-558).
+ * Springs accumulate over calls; call before solving with the operator. */
+int tm_operator_add_springs(tm_operator* op, const double* spring_full);
+
+/* Inertia of K(k) - shift M(k) at rpm (Sylvester): the number of negative pivots of
+ * its LDL^H, which is the number of eigenvalues lambda < shift of
+ * K(k) phi = lambda M(k) phi over the free DOFs (a completeness check of a solve:
+ * with shift just above the largest eigenvalue returned, it equals n_unique).  Pinned
+ * DOFs count as positive.  min_pivot_ratio (may be NULL): the smallest |d_j| / |a_jj|;
+ * pivot_status (may be NULL): bits as tm_result.pivot_flags.  No counterpart in the
+ * reference (a diagnostic of this engine). */
+int tm_operator_inertia(tm_operator* op, double rpm, int32_t harmonic, double shift,
+                        int64_t* n_negative, double* min_pivot_ratio, int32_t* pivot_status);
 
 #ifdef __cplusplus
 }

@@ -396,18 +396,24 @@ class SectorOperators:
     omega: float = 0.0
 
 
-def sector_operators(m: CyclicModel, E, nu, rho, rpm):
-    """Steps 1-3 of solve_at_rpm (src/cyclic_solver.cpp:735-818)."""
+def sector_operators(m: CyclicModel, E, nu, rho, rpm, springs=None):
+    """Steps 1-3 of solve_at_rpm (src/cyclic_solver.cpp:735-818).  springs: optional
+    3 n_nodes grounded spring stiffnesses (ELASTIC_SUPPORT, build_spring_stiffness,
+    src/cyclic_solver.cpp:193-212) added to K before the rotating terms (step 1b,
+    747-751); the static prestress solve keeps the base K (static_centrifugal uses
+    assembler_.K(), 456-470)."""
     omega = rpm * 2.0 * PI / 60.0
     K, M = fem.assemble_KM(m.nodes, m.elements, E, nu, rho)
     K_eff = K
+    if springs is not None and np.any(np.asarray(springs) > 0):
+        K_eff = (K + sp.diags(np.maximum(np.asarray(springs, float), 0.0))).tocsc()
     u = None
     Gc = Gp = None
     if abs(omega) > 0.0:
         ax = np.eye(3)[m.rotation_axis]
         Kw, G = fem.assemble_rotating(m.nodes, m.elements, rho, omega, ax)
         Gc, Gp = project_phase_split(m, G)
-        K_eff = K - Kw
+        K_eff = K_eff - Kw
         if len(m.constrained) > 0:
             F = fem.assemble_centrifugal(m.nodes, m.elements, rho, omega, ax)
             u = static_centrifugal(m, K, F)
@@ -518,35 +524,55 @@ def select_qep(omega, vectors, num_modes, n):
     return freqs, whirl, shapes, np.array([omega[i] for i in sel])
 
 
-def solve_qep(Kk, Mk, Dk, num_modes):
+def lancaster_inverse(Kk, Dk):
+    """L1^-1 as a LinearOperator through one sparse factor of K(k):
+    L1 [a; b] = [c; d] gives K a = d and D a + K b = c, so a = K^-1 d and
+    b = K^-1 (c - D a)."""
+    n = Kk.shape[0]
+    lu_ = spla.splu(sp.csc_matrix(Kk), permc_spec="MMD_AT_PLUS_A")
+    Dc = sp.csr_matrix(Dk)
+
+    def mv(z):
+        z = np.asarray(z).reshape(-1)
+        a = lu_.solve(np.ascontiguousarray(z[n:], dtype=complex))
+        b = lu_.solve(np.ascontiguousarray(z[:n] - Dc @ a, dtype=complex))
+        return np.concatenate([a, b])
+
+    return spla.LinearOperator((2 * n, 2 * n), matvec=mv, dtype=complex)
+
+
+def solve_qep(Kk, Mk, Dk, num_modes, dense_max=400):
     """The forward/backward gyroscopic modes the reference's selection keeps: the
     eigenvalues w of the Lancaster pencil with the smallest |w| of each sign.
-    Dense generalised eigh of (L1, L2) for 2n <= 400 (the reference's dense path
-    covers 2n <= 200, src/modal_solver.cpp:224-226, and returns every eigenvalue);
-    above that, scipy eigsh with shift-invert at 0, which finds exactly the
-    eigenvalues of smallest magnitude.  (The reference's Lanczos for large
-    systems uses sigma = sqrt(||K||_F / ||M||_F) with ascending-lambda selection,
-    whose result depends on its iteration history: parity unpinned there.)"""
+    Dense generalised eigh of (L1, L2) for 2n <= dense_max (the reference's dense
+    path covers 2n <= 200, src/modal_solver.cpp:224-226, and returns every
+    eigenvalue); above that, scipy eigsh with shift-invert at 0 (L1^-1 applied
+    through a factor of K(k), lancaster_inverse), which finds exactly the
+    eigenvalues of smallest magnitude, to ARPACK's tolerance tol=1e-13.  (The
+    reference's Lanczos for large systems uses sigma = sqrt(||K||_F / ||M||_F) with
+    ascending-lambda selection, whose result depends on its iteration history:
+    parity unpinned there.)"""
     n = Kk.shape[0]
     L1, L2 = lancaster_pencil(Kk, Mk, Dk)
-    if 2 * n <= 400:
+    if 2 * n <= dense_max:
         A = L1.toarray(); B = L2.toarray()
         A = 0.5 * (A + A.conj().T); B = 0.5 * (B + B.conj().T)
         w, Z = sla.eigh(A, B)
     else:
         nev = min(2 * num_modes + 6, 2 * n - 2)
-        w, Z = spla.eigsh(L1, k=nev, M=L2, sigma=0.0, which="LM")
+        w, Z = spla.eigsh(L1, k=nev, M=L2, sigma=0.0, which="LM", OPinv=lancaster_inverse(Kk, Dk),
+                          tol=1e-13)
     return select_qep(w, Z, num_modes, n) + (L1, L2)
 
 
-def solve_harmonic_qep(m: CyclicModel, ops: SectorOperators, k, num_modes, rpm):
+def solve_harmonic_qep(m: CyclicModel, ops: SectorOperators, k, num_modes, rpm, dense_max=400):
     """QEP branch of solve_harmonic for k > 0 with Coriolis (src/cyclic_solver.cpp:986-1005,
     1058-1096): no duplication; truncated to num_modes."""
     N = m.num_sectors
     Kk = harmonic_matrix(ops.Kc, ops.Kp, k, N)
     Mk = harmonic_matrix(ops.Mc, ops.Mp, k, N)
     Dk = coriolis_matrix(ops, k, N)
-    freqs, whirl, heads, om, L1, L2 = solve_qep(Kk, Mk, Dk, num_modes)
+    freqs, whirl, heads, om, L1, L2 = solve_qep(Kk, Mk, Dk, num_modes, dense_max)
     freqs, whirl, heads, om = freqs[:num_modes], whirl[:num_modes], heads[:, :num_modes], om[:num_modes]
     shapes = expand_modes(m, heads, k)
     r = HarmonicResult(k, rpm, freqs, shapes, whirl, True, om, heads, Kk, Mk)

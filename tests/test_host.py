@@ -176,3 +176,18 @@ def test_grid_matching_equals_greedy_scan(seed):
     assert [tuple(p) for p in mesh.matched_pairs] == [tuple(p) for p in pairs]
     assert list(mesh.left_boundary) == l2 and list(mesh.right_boundary) == r2
     assert sorted(mesh.free_boundary) == free
+
+
+def test_reference_package_name_resolves_to_the_engine():
+    # `import turbomodal as tm` code keeps working (python/turbomodal/__init__.py:1-60)
+    import turbomodal as tm
+    import turbomodal_amd
+    from turbomodal import _core
+    from turbomodal.io import load_mesh
+    from turbomodal.solver import rpm_sweep
+    assert tm.solve is turbomodal_amd.solve and rpm_sweep is turbomodal_amd.rpm_sweep
+    assert _core.CyclicSymmetrySolver is turbomodal_amd.CyclicSymmetrySolver
+    m = load_mesh(os.path.join(GOLDEN, "wedge_sector.msh"), 24)
+    assert m.num_sectors == 24 and len(m.matched_pairs) > 0
+    with pytest.raises(ValueError):
+        load_mesh(os.path.join(GOLDEN, "wedge_sector.msh"), 24, file_format="nastran")

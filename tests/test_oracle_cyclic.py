@@ -57,7 +57,20 @@ def test_phase_split_equals_direct_projection(wedge):
             assert np.abs(split.imag).max() > 1e-6 * np.abs(split).max()
 
 
-def test_full_annulus_brute_force(wedge):
+def _wedge_springs(X, m):
+    """Grounded springs of unequal x, y, z stiffness on every third node plus every
+    left and right boundary node (ELASTIC_SUPPORT, N/m)."""
+    k = np.zeros(3 * len(X))
+    nodes = sorted(set(range(0, len(X), 3)) | set(m.left) | set(m.right))
+    for i in nodes:
+        k[3 * i:3 * i + 3] = (3e8, 1e8, 2e8)
+    return k
+
+
+@pytest.mark.parametrize("with_springs", [False, True])
+def test_full_annulus_brute_force(wedge, with_springs):
+    # springs: each sector copy carries the sector's springs rotated with it; the cyclic
+    # reduction adds a right node's spring to its left partner through R(alpha)
     X, T, S, N, m = wedge
     nn = len(X)
     r2l = {r: l for l, r in m.pairs}
@@ -79,11 +92,21 @@ def test_full_annulus_brute_force(wedge):
             coords[gid[s, i]] = R @ X[i]
     els = np.concatenate([gid[s][T] for s in range(N)])
     K, M = fem.assemble_KM(coords, els, 200e9, 0.3, 7800.0)
+    K = K.toarray()
+    springs = _wedge_springs(X, m) if with_springs else None
+    if with_springs:
+        for s in range(N):
+            a = 2 * math.pi * s / N
+            R = np.array([[math.cos(a), -math.sin(a), 0], [math.sin(a), math.cos(a), 0], [0, 0, 1]])
+            for i in range(nn):
+                g = gid[s, i]
+                K[3 * g:3 * g + 3, 3 * g:3 * g + 3] += R @ np.diag(springs[3 * i:3 * i + 3]) @ R.T
     hub = {gid[s, i] for s in range(N) for i in S["hub_constraint"]}
     free = np.array([d for d in range(3 * cnt) if d // 3 not in hub])
-    w = sla.eigh(K.toarray()[np.ix_(free, free)], M.toarray()[np.ix_(free, free)], eigvals_only=True)
+    w = sla.eigh(K[np.ix_(free, free)], M.toarray()[np.ix_(free, free)], eigvals_only=True)
     f_full = np.sqrt(w) / (2 * math.pi)
-    res = cyclic.solve_at_rpm(m, 200e9, 0.3, 7800.0, 0.0, 5, policy="tight")
+    ops = cyclic.sector_operators(m, 200e9, 0.3, 7800.0, 0.0, springs=springs)
+    res = cyclic.solve_at_rpm(m, 200e9, 0.3, 7800.0, 0.0, 5, policy="tight", ops=ops)
     fc = []
     for r in res:
         mult = 1 if r.harmonic_index in (0, N // 2) else 2
@@ -157,3 +180,17 @@ def test_static_centrifugal_residual():
     rim = np.argmax(np.hypot(X[:, 0], X[:, 1]))
     ur = u[3 * rim:3 * rim + 2] @ X[rim, :2] / np.hypot(*X[rim, :2])
     assert ur > 0
+
+
+def test_full_size_fixture_is_the_oracle():
+    # tests/golden/c4_sample.npz holds the oracle's answers for the GPU tests: one of its
+    # entries (rpm 0, ND 18 on the 45k-DOF bladed sector) recomputed here
+    from common import load_golden, subset_mac
+    gold = load_golden("c4_sample.npz")
+    X, T, S, N = generated("bladed_c2")
+    m = oracle_model(X, T, S, N)
+    ops = cyclic.sector_operators(m, 200e9, 0.3, 7800.0, 0.0)
+    r = cyclic.solve_harmonic(m, ops, 18, 10, 0.0, "tight")
+    assert np.allclose(r.frequencies, gold["r0_k18_freq"], rtol=1e-10)
+    for q in range(10):
+        assert subset_mac(r.mode_shapes[gold["subset"], q], gold["r0_k18_shape"][:, q]) >= 1 - 1e-9

@@ -124,3 +124,16 @@ def test_full_annulus_gyroscopic_brute_force(wedge):
             assert np.min(np.abs(w_full - w)) <= 1e-8 * abs(w), (k, w)
             if k < N // 2:
                 assert np.min(np.abs(w_full + w)) <= 1e-8 * abs(w), (k, -w)
+
+
+def test_sparse_path_equals_dense(wedge):
+    # the large-system path (eigsh shift-invert at 0 with L1^-1 through a factor of K(k))
+    # returns the dense path's pairs
+    X, T, S, N, m, ops = wedge
+    for k in (1, 7):
+        d = cyclic.solve_harmonic_qep(m, ops, k, 6, RPM)
+        s = cyclic.solve_harmonic_qep(m, ops, k, 6, RPM, dense_max=0)
+        assert np.allclose(s.eigenvalues, d.eigenvalues, rtol=1e-10)
+        assert np.array_equal(s.whirl_direction, d.whirl_direction)
+        for q in range(6):
+            assert cyclic.compute_mac(s.mode_shapes[:, q], d.mode_shapes[:, q])[0, 0] >= 1 - 1e-9

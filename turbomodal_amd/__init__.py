@@ -24,6 +24,7 @@ from turbomodal_amd.solver import (  # noqa: F401
     BoundaryCondition,
     EnrichedModalResult,
     campbell_data,
+    compute_modal_mass,
     load_results,
     rpm_sweep,
     save_results,

@@ -34,7 +34,7 @@ EXPORTS = (
     "tm_mesh_get_info", "tm_operator_create", "tm_operator_destroy", "tm_solve_batch",
     "tm_problem_footprint", "tm_device_free_bytes", "tm_operator_export",
     "tm_operator_static_u", "tm_server_create", "tm_server_submit", "tm_server_wait",
-    "tm_server_destroy", "tm_set_memory_reserve",
+    "tm_server_destroy", "tm_set_memory_reserve", "tm_operator_add_springs", "tm_operator_inertia",
 )
 
 
@@ -65,7 +65,8 @@ class Result(C.Structure):
                 ("residuals", C.POINTER(C.c_double)), ("modal_mass", C.POINTER(C.c_double)),
                 ("n_modes", C.c_int32),
                 ("n_unique", C.c_int32), ("converged", C.c_int32), ("restarts", C.c_int32),
-                ("status", C.c_int32), ("seconds", C.c_double)]
+                ("status", C.c_int32), ("seconds", C.c_double), ("factor_retries", C.c_int32),
+                ("pivot_flags", C.c_int32), ("shift_used", C.c_double)]
 
 
 _lib = None
@@ -109,6 +110,8 @@ def lib():
     L.tm_server_wait.argtypes = [vp, C.c_uint64, dbl, P(i32)]
     L.tm_server_destroy.argtypes = [vp]
     L.tm_server_destroy.restype = None
+    L.tm_operator_add_springs.argtypes = [vp, P(dbl)]
+    L.tm_operator_inertia.argtypes = [vp, dbl, i32, dbl, P(i64), P(dbl), P(i32)]
     _lib = L
     return L
 
@@ -237,6 +240,20 @@ class EngineOperator:
         self.mesh.ctx.check(self.mesh.ctx.L.tm_operator_static_u(self.h, _ptr(u, C.c_double)))
         return u
 
+    def add_springs(self, spring_full):
+        """Grounded springs (3 n_nodes stiffnesses, N/m) on the diagonal of K (ELASTIC_SUPPORT)."""
+        k = np.ascontiguousarray(spring_full, dtype=np.float64)
+        if k.shape != (3 * self.mesh.n_nodes,):
+            raise ValueError("spring_full must hold 3 * n_nodes stiffnesses")
+        self.mesh.ctx.check(self.mesh.ctx.L.tm_operator_add_springs(self.h, _ptr(k, C.c_double)))
+
+    def inertia(self, rpm: float, harmonic: int, shift: float):
+        """(negative pivots of the LDL^H of K(k) - shift M(k), smallest |d| / |a_jj|, status bits)."""
+        neg, ratio, st = C.c_int64(), C.c_double(), C.c_int32()
+        self.mesh.ctx.check(self.mesh.ctx.L.tm_operator_inertia(self.h, float(rpm), int(harmonic), float(shift),
+                                                                C.byref(neg), C.byref(ratio), C.byref(st)))
+        return int(neg.value), float(ratio.value), int(st.value)
+
     def footprint(self, num_modes, harmonic, precision=TM_PREC_FP64, method=None):
         method = TM_METHOD_BLOCK if method is None else method
         return int(self.mesh.ctx.L.tm_problem_footprint(self.h, num_modes, precision, harmonic, method))
@@ -256,6 +273,9 @@ class SolveOutput:
     status: int
     seconds: float
     modal_mass: np.ndarray = None
+    factor_retries: int = 0
+    pivot_flags: int = 0
+    shift_used: float = 0.0
 
 
 def _prepare(p, want_shapes, prob, res):
@@ -301,7 +321,7 @@ def _output(p, r, keep):
     return SolveOutput(int(p["harmonic"]), float(p.get("rpm", 0.0)), f[:nm].copy(), shapes,
                        w[:nm].copy(), ev[: r.n_unique].copy(), rs[: r.n_unique].copy(),
                        bool(r.converged), int(r.restarts), int(r.status), float(r.seconds),
-                       mm[:nm].copy())
+                       mm[:nm].copy(), int(r.factor_retries), int(r.pivot_flags), float(r.shift_used))
 
 
 def solve_batch(ctx: Context, problems, want_shapes=True):

@@ -54,6 +54,15 @@ void upload_mesh(Mesh& M, hipStream_t s) {
       for (int q = S.rowptr[I]; q < S.rowptr[I + 1]; ++q) row[q] = I;
     M.d_slot_row = upload(row, s);
   }
+  {
+    std::vector<int> diag(S.nf, -1);
+    for (int I = 0; I < S.nf; ++I)
+      for (int q = S.rowptr[I]; q < S.rowptr[I + 1]; ++q)
+        if (S.colidx[q] == I) diag[I] = q;
+    for (int I = 0; I < S.nf; ++I)
+      if (diag[I] < 0) throw std::runtime_error("pattern without a diagonal block");
+    M.d_diag_slot = upload(diag, s);
+  }
   M.d_kc_ptr = upload(S.kc_ptr, s);
   M.d_kc_src = upload(S.kc_src, s);
   M.d_kp_ptr = upload(S.kp_ptr, s);
@@ -245,6 +254,9 @@ int tm_solve_batch(tm_context* ctx, const tm_problem* problems, int32_t n, tm_re
   if (n <= 0) return TM_OK;
   return guarded(ctx, [&] {
     if (ctx->server) throw std::runtime_error("the context is owned by a server; submit to it instead");
+    for (int i = 0; i < n; ++i)
+      if (problems[i].precision != TM_PREC_FP64)
+        throw ArgError("precision: only TM_PREC_FP64 is provided (no complex-FP32 factor path)");
     TM_HIP(hipSetDevice(ctx->device));
     for (int i = 0; i < n; ++i)
       if (results) {
@@ -254,6 +266,9 @@ int tm_solve_batch(tm_context* ctx, const tm_problem* problems, int32_t n, tm_re
         results[i].restarts = 0;
         results[i].status = TM_ERR_RUNTIME;
         results[i].seconds = 0;
+        results[i].factor_retries = 0;
+        results[i].pivot_flags = 0;
+        results[i].shift_used = problems[i].shift;
       }
     solve_problems(ctx, problems, n, results);
   });
@@ -292,8 +307,7 @@ void tm_server_destroy(tm_server* s) {
 
 int64_t tm_problem_footprint(const tm_operator* op, int32_t num_modes, int32_t precision,
                              int32_t harmonic, int32_t method) {
-  if (!op) return -1;
-  (void)precision;
+  if (!op || precision != TM_PREC_FP64) return -1;
   try {
     return problem_footprint(*op, num_modes, harmonic, method);
   } catch (...) {
@@ -321,6 +335,31 @@ int tm_operator_static_u(tm_operator* op, double* u_full) {
   if (!op || !u_full) return TM_ERR_ARG;
   std::copy(op->u_static.begin(), op->u_static.end(), u_full);
   return TM_OK;
+}
+
+int tm_operator_add_springs(tm_operator* op, const double* spring_full) {
+  if (!op || !spring_full) return TM_ERR_ARG;
+  Context* ctx = op->mesh->ctx;
+  return guarded(ctx, [&] {
+    TM_HIP(hipSetDevice(ctx->device));
+    add_springs(ctx, *op, spring_full);
+  });
+}
+
+int tm_operator_inertia(tm_operator* op, double rpm, int32_t harmonic, double shift, int64_t* n_negative,
+                        double* min_pivot_ratio, int32_t* pivot_status) {
+  if (!op || !n_negative) return TM_ERR_ARG;
+  Context* ctx = op->mesh->ctx;
+  return guarded(ctx, [&] {
+    if (harmonic < 0 || harmonic > op->mesh->S.N / 2) throw ArgError("harmonic index out of range 0..N/2");
+    if (!std::isfinite(shift) || !std::isfinite(rpm)) throw ArgError("rpm and shift must be finite");
+    TM_HIP(hipSetDevice(ctx->device));
+    double r = 0;
+    int st = 0;
+    operator_inertia(ctx, *op, rpm, harmonic, shift, n_negative, &r, &st);
+    if (min_pivot_ratio) *min_pivot_ratio = r;
+    if (pivot_status) *pivot_status = st;
+  });
 }
 
 int tm_operator_export(tm_operator* op, double rpm, int32_t harmonic, int32_t which, int64_t* nnz,
@@ -368,8 +407,13 @@ int tm_operator_export(tm_operator* op, double rpm, int32_t harmonic, int32_t wh
             if (rows) {
               rows[cnt] = r;
               cols[cnt] = c;
-              vals[2 * cnt] = cv + pre * (pv + qv);
-              vals[2 * cnt + 1] = pim * (pv - qv);
+              if (which == 2) {  // G is skew: G(k) = G_c + P G_p - conj(P) G_p^T
+                vals[2 * cnt] = cv + pre * (pv - qv);
+                vals[2 * cnt + 1] = pim * (pv + qv);
+              } else {           // K, M symmetric: X_c + P X_p + conj(P) X_p^T
+                vals[2 * cnt] = cv + pre * (pv + qv);
+                vals[2 * cnt + 1] = pim * (pv - qv);
+              }
             }
             ++cnt;
           }

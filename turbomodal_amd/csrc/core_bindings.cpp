@@ -385,6 +385,9 @@ struct StationaryFrameResult {
 // ---------------------------------------------------------------------------
 // engine handles
 // ---------------------------------------------------------------------------
+// the context's memory reserve outside solve calls (tm_set_memory_reserve's default)
+constexpr double kDefaultReserve = 0.15;
+
 class Engine {
  public:
   static Engine& get() {
@@ -598,10 +601,23 @@ class CyclicSymmetrySolver {
   void check_fluid() const {
     if (fluid_.type != FluidConfig::Type::NONE && fluid_.fluid_density > 0.0)
       throw std::runtime_error("fluid coupling is not part of the MI355X cyclic solve (dry structures only)");
-    for (const auto& cg : constraints_)
-      if (cg.type == BCType::ELASTIC_SUPPORT)
-        for (double k : cg.spring_stiffness)
-          if (k > 0.0) throw std::runtime_error("ELASTIC_SUPPORT springs are not supported by the MI355X engine");
+  }
+
+  // build_spring_stiffness (src/cyclic_solver.cpp:193-212): per DOF, the sum of the
+  // positive spring stiffnesses of the ELASTIC_SUPPORT groups that hold its node
+  std::vector<double> spring_stiffness_full(bool& any) const {
+    std::vector<double> k(3 * (size_t)mesh_.num_nodes(), 0.0);
+    any = false;
+    for (const auto& cg : constraints_) {
+      if (cg.type != BCType::ELASTIC_SUPPORT) continue;
+      for (int node : cg.node_ids)
+        for (int c = 0; c < 3; ++c)
+          if (cg.spring_stiffness[c] > 0.0) {
+            k.at(3 * (size_t)node + c) += cg.spring_stiffness[c];
+            any = true;
+          }
+    }
+    return k;
   }
 
   void ensure_engine() {
@@ -633,6 +649,13 @@ class CyclicSymmetrySolver {
       Engine::check(ctx, tm_operator_create(ctx, emesh_, mat_.E, mat_.nu, mat_.rho, has_constraints_ ? 1 : 0, &op));
     }
     op_ = op;
+    bool springs = false;
+    const std::vector<double> k = spring_stiffness_full(springs);
+    if (springs) {
+      py::gil_scoped_release rel;
+      std::lock_guard<std::mutex> g(Engine::get().mutex());
+      Engine::check(ctx, tm_operator_add_springs(op_, k.data()));
+    }
   }
 
   std::vector<std::vector<ModalResult>> solve_points(const std::vector<double>& rpms, int num_modes,
@@ -640,9 +663,8 @@ class CyclicSymmetrySolver {
                                                      py::object progress_cb, double memory_reserve_fraction) {
     ensure_engine();
     tm_context* ctx = Engine::get().ctx();
-    // device memory kept out of batch sizing, as the reference keeps host RAM
-    // out of its concurrency cap (src/cyclic_solver.cpp:1257-1272)
-    Engine::check(ctx, tm_set_memory_reserve(ctx, memory_reserve_fraction));
+    if (!(memory_reserve_fraction >= 0.0 && memory_reserve_fraction <= 1.0))
+      throw py::value_error("memory_reserve_fraction must lie in [0, 1]");
     const int max_k = mesh_.num_sectors / 2;
     std::vector<int> ks;
     if (hi.empty())
@@ -695,7 +717,13 @@ class CyclicSymmetrySolver {
     {
       py::gil_scoped_release rel;
       std::lock_guard<std::mutex> g(Engine::get().mutex());
-      Engine::check(ctx, tm_solve_batch(ctx, probs.data(), static_cast<int32_t>(np), res.data()));
+      // device memory kept out of batch sizing, as the reference keeps host RAM out of
+      // its concurrency cap (src/cyclic_solver.cpp:1257-1272); set under the engine lock
+      // for this call only, so concurrent callers do not see each other's value
+      Engine::check(ctx, tm_set_memory_reserve(ctx, memory_reserve_fraction));
+      const int rc = tm_solve_batch(ctx, probs.data(), static_cast<int32_t>(np), res.data());
+      tm_set_memory_reserve(ctx, kDefaultReserve);
+      Engine::check(ctx, rc);
     }
     const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::vector<std::vector<ModalResult>> out(rpms.size());

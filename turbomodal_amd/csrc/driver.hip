@@ -569,7 +569,7 @@ void herm_gen_eig(int n, const std::vector<double>& A, const std::vector<double>
 // operator: assembly at unit omega + static prestress
 // ---------------------------------------------------------------------------
 struct BatchMem {
-  DevBuf L, U, T, S, status, params;
+  DevBuf L, U, T, S, status, pivinfo, params;
   int pcols = 1;
   BatchBufs bufs(const Plan&, int B) {
     BatchBufs bb;
@@ -578,6 +578,7 @@ struct BatchMem {
     bb.T = T.as<double2>();
     bb.S = S.as<double2>();
     bb.status = status.as<int>();
+    bb.pivinfo = pivinfo.as<int>();
     bb.params = params.as<ProbParam>();
     bb.B = B;
     return bb;
@@ -589,6 +590,7 @@ struct BatchMem {
     T.alloc(pool, sizeof(double2) * P.tvec_entries * B * p);
     S.alloc(pool, sizeof(double2) * P.inv_scratch_entries * B);
     status.alloc(pool, sizeof(int) * B);
+    pivinfo.alloc(pool, sizeof(int) * 2 * B);
     params.alloc(pool, sizeof(ProbParam) * B);
   }
 };
@@ -643,7 +645,7 @@ void create_operator(Context* ctx, Mesh& M, Operator& op) {
     int status = 0;
     TM_HIP(hipMemcpyAsync(&status, bm.status.p, sizeof(int), hipMemcpyDeviceToHost, st));
     TM_HIP(hipStreamSynchronize(st));
-    if (status) throw NumericError("zero pivot in the static centrifugal solve");
+    if (status & 1) throw NumericError("zero pivot in the static centrifugal solve");
     launch_expand_real(ctx, M, x.as<double2>(), ufull);
     TM_HIP(hipMemcpyAsync(op.u_static.data(), ufull.p, ufull.bytes, hipMemcpyDeviceToHost, st));
     launch_element_sigma(ctx, M, op.E, op.nu, ufull, s_ab);
@@ -661,6 +663,103 @@ void create_operator(Context* ctx, Mesh& M, Operator& op) {
                      op.real_pack.as<double2>());
   TM_HIP(hipGetLastError());
   TM_HIP(hipStreamSynchronize(st));
+}
+
+// Kc[diag_slot[f]] += add[f] (9 doubles per free node)
+__global__ void k_add_diag_blocks(int nf, const int* __restrict__ diag_slot, const double* __restrict__ add,
+                                  double* __restrict__ Kc) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= 9LL * nf) return;
+  const int f = (int)(i / 9), q = (int)(i % 9);
+  Kc[(long long)diag_slot[f] * 9 + q] += add[i];
+}
+
+// Grounded springs (ELASTIC_SUPPORT) on the diagonal blocks of K_c: a node of the
+// interior or the left boundary adds diag(k) to its own block; a right-boundary node
+// acts on its left partner, u_right = R(alpha) u_left with R rotating the in-plane pair
+// (c1, c2) (src/cyclic_solver.cpp:291-327), so it adds R^T diag(k) R, the same for every
+// harmonic (|P| = 1).  Pinned components are left out (removed DOFs in the reference).
+void add_springs(Context* ctx, Operator& op, const double* kfull) {
+  Mesh& M = *op.mesh;
+  const Symbolic& S = M.S;
+  const double alpha = 2.0 * M_PI / S.N, ca = std::cos(alpha), sa = std::sin(alpha);
+  std::vector<double> add(9 * (size_t)S.nf, 0.0);
+  bool any = false;
+  for (int i = 0; i < S.nn; ++i) {
+    const double* kv = kfull + 3 * (size_t)i;
+    for (int c = 0; c < 3; ++c)
+      if (!(kv[c] >= 0.0) || !std::isfinite(kv[c])) throw ArgError("spring stiffness must be finite and >= 0");
+    if (kv[0] == 0.0 && kv[1] == 0.0 && kv[2] == 0.0) continue;
+    const int f = S.free_of_full[i];
+    if (f < 0) continue;
+    double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (S.node_class[i] == TM_NODE_RIGHT) {
+      const int a = S.c1, b = S.c2, z = S.cax;
+      const double k1 = kv[a], k2 = kv[b];
+      blk[3 * a + a] = ca * ca * k1 + sa * sa * k2;
+      blk[3 * a + b] = blk[3 * b + a] = sa * ca * (k2 - k1);
+      blk[3 * b + b] = sa * sa * k1 + ca * ca * k2;
+      blk[3 * z + z] = kv[z];
+    } else {
+      for (int c = 0; c < 3; ++c) blk[4 * c] = kv[c];
+    }
+    for (int ci = 0; ci < 3; ++ci)
+      for (int cj = 0; cj < 3; ++cj) {
+        if (S.pinned[3 * (size_t)f + ci] || S.pinned[3 * (size_t)f + cj]) continue;
+        add[9 * (size_t)f + 3 * ci + cj] += blk[3 * ci + cj];
+        any = any || blk[3 * ci + cj] != 0.0;
+      }
+  }
+  if (!any) return;
+  hipStream_t st = ctx->stream;
+  DevBuf d = upload(add, st);
+  hipLaunchKernelGGL(k_add_diag_blocks, dim3((unsigned)((9LL * S.nf + 255) / 256)), dim3(256), 0, st, S.nf,
+                     M.d_diag_slot.as<int>(), d.as<double>(), op.Kc.as<double>());
+  // the split K/M product reads K_c from the packed real slots
+  hipLaunchKernelGGL(k_pack_real_slots, dim3((unsigned)((14 * op.nblk + 255) / 256)), dim3(256), 0, st,
+                     (long long)op.nblk, op.Kc.as<double>(), op.dKc.as<double>(), op.Mc.as<double>(),
+                     op.real_pack.as<double2>());
+  TM_HIP(hipGetLastError());
+  TM_HIP(hipStreamSynchronize(st));
+}
+
+__global__ void k_inertia_start(ProbParam* params, ProbParam pp, int* status) {
+  if (threadIdx.x == 0) {
+    params[0] = pp;
+    status[0] = 0;
+  }
+}
+
+// Factors K(k) - shift M(k) at rpm (one problem) into bb and reads back the number of
+// negative pivots, the smallest relative pivot and the factor's status bits.  Launches
+// and a device-to-host copy only: bb's memory is the caller's.
+static void inertia_into(Context* ctx, Operator& op, double rpm, int k, double shift, const BatchBufs& bb,
+                         int64_t* neg, double* ratio, int* status) {
+  Mesh& M = *op.mesh;
+  const Symbolic& S = M.S;
+  hipStream_t st = ctx->stream;
+  const double w = rpm * 2.0 * M_PI / 60.0, alpha = 2.0 * M_PI / S.N;
+  const ProbParam pp{std::cos(k * alpha), std::sin(k * alpha), w * w, shift};
+  hipLaunchKernelGGL(k_inertia_start, dim3(1), dim3(64), 0, st, const_cast<ProbParam*>(bb.params), pp, bb.status);
+  factor_batch(ctx, M, M.plan, op_arrays(op), bb);
+  int info[3] = {0, 0, 0};
+  TM_HIP(hipMemcpyAsync(info, bb.status, sizeof(int), hipMemcpyDeviceToHost, st));
+  TM_HIP(hipMemcpyAsync(info + 1, bb.pivinfo, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+  TM_HIP(hipStreamSynchronize(st));
+  float r;
+  std::memcpy(&r, &info[2], sizeof(float));
+  *neg = info[1];
+  *ratio = r;
+  *status = info[0];
+}
+
+// Negative pivots of the LDL^H of K(k) - shift M(k) at rpm, workspace from the
+// context's pool.
+void operator_inertia(Context* ctx, Operator& op, double rpm, int k, double shift, int64_t* neg, double* ratio,
+                      int* status) {
+  BatchMem bm;
+  bm.alloc(&ctx->pool, op.mesh->plan, 1);
+  inertia_into(ctx, op, rpm, k, shift, bm.bufs(op.mesh->plan, 1), neg, ratio, status);
 }
 
 // M(k) blocks of the operator, cached per harmonic.  Operators are shared by
@@ -726,7 +825,16 @@ struct HostProblem {
   int steps = 0;                             // block steps taken (seeds its restarts)
   std::chrono::steady_clock::time_point t_admit;
   std::string message;
+  // status bits of its shift-invert factorization: bit 0 a zero or non-finite pivot,
+  // bit 1 a near-singular one (the shift on an eigenvalue); either fails the problem,
+  // and its retry moves the shift (retry_failed)
+  int factor_flags = 0;
 };
+
+static const char* factor_message(int flags) {
+  return (flags & 1) ? "zero or non-finite pivot in the shift-invert factorization"
+                     : "near-singular pivot in the shift-invert factorization (the shift is on an eigenvalue)";
+}
 
 static void fill_policy(const Symbolic& S, HostProblem& hp) {
   const int N = S.N, max_k = N / 2;
@@ -882,11 +990,13 @@ class BatchLanczos {
     std::vector<int> status(B_);
     TM_HIP(hipMemcpyAsync(status.data(), bm_.status.p, sizeof(int) * B_, hipMemcpyDeviceToHost, stream()));
     TM_HIP(hipStreamSynchronize(stream()));
-    for (int b = 0; b < B_; ++b)
+    for (int b = 0; b < B_; ++b) {
+      probs_[b]->factor_flags = status[b];
       if (status[b]) {
         probs_[b]->failed = true;
-        probs_[b]->message = "zero pivot in the shift-invert factorization";
+        probs_[b]->message = factor_message(status[b]);
       }
+    }
   }
 
   void set_host_state(const std::vector<int>& jcur, const std::vector<int>& live,
@@ -1293,6 +1403,8 @@ class BatchLanczos {
     TM_HIP(hipMemcpyAsync(hm.data(), pm.p, sizeof(double2) * hm.size(), hipMemcpyDeviceToHost, stream()));
     TM_HIP(hipStreamSynchronize(stream()));
     std::vector<double2> zc((size_t)B_ * nemax * nemax, make_double2(0.0, 0.0));
+    std::vector<int> keep = nr;  // problems whose X becomes Y Z
+    bool skipped = false;
     for (int b = 0; b < B_; ++b) {
       const int ne = nr[b];
       if (!ne) continue;
@@ -1308,7 +1420,15 @@ class BatchLanczos {
           A[2 * (p * ne + q)] = ar; A[2 * (p * ne + q) + 1] = ai;
           Bm[2 * (p * ne + q)] = br; Bm[2 * (p * ne + q) + 1] = bi;
         }
-      herm_gen_eig(ne, A, Bm, mu, Z);
+      try {
+        herm_gen_eig(ne, A, Bm, mu, Z);
+      } catch (const NumericError&) {
+        // Y = (K - sigma M)^-1 M X lost rank: a shift on one of the wanted eigenvalues
+        // turns every column towards that eigenvector.  X stays as it is.
+        keep[b] = 0;
+        skipped = true;
+        continue;
+      }
       HostProblem* hp = probs_[b];
       for (int q = 0; q < ne; ++q) hp->theta_sorted_lambda[q] = mu[q];
       for (int k = 0; k < ne; ++k)
@@ -1316,8 +1436,9 @@ class BatchLanczos {
           zc[((size_t)b * nemax + k) * nemax + q] = make_double2(Z[2 * (k * ne + q)], Z[2 * (k * ne + q) + 1]);
     }
     DevBuf dz = upload(pool(), zc, stream());
+    DevBuf dkeep = skipped ? upload(pool(), keep, stream()) : DevBuf();
     hipLaunchKernelGGL(k_combine_c, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), Y, ldx, n_,
-                       dz.as<double2>(), nemax, (long long)nemax * nemax, dnr.as<int>(), X, ldx);
+                       dz.as<double2>(), nemax, (long long)nemax * nemax, (skipped ? dkeep : dnr).as<int>(), X, ldx);
   }
 
   // Ritz vectors, residuals, refinement and results of the selected problems
@@ -2067,12 +2188,14 @@ class BatchLanczos {
       std::vector<int> status(B_);
       TM_HIP(hipMemcpyAsync(status.data(), bm_.status.p, sizeof(int) * B_, hipMemcpyDeviceToHost, stream()));
       TM_HIP(hipStreamSynchronize(stream()));
-      for (int b : slots)
+      for (int b : slots) {
+        probs_[b]->factor_flags = status[b];
         if (status[b]) {
           probs_[b]->failed = true;
-          probs_[b]->message = "zero pivot in the shift-invert factorization";
+          probs_[b]->message = factor_message(status[b]);
           mask[b] = 0;
         }
+      }
     }
     HostTimer h(ctx_, "iterate");
     random_block(mask, std::vector<int>(B_, 0));
@@ -2697,13 +2820,50 @@ static void run_block_groups(Context* ctx, Operator& op, std::vector<std::vector
     if (e) std::rethrow_exception(e);
 }
 
-static void solve_problems_impl(Context* ctx, const tm_problem* problems, int n, tm_result* results);
+static std::vector<int> solve_problems_impl(Context* ctx, const tm_problem* problems, int n, tm_result* results);
+
+// Failed problems are solved again, each alone, as the reference retries its failed
+// harmonics one at a time (src/cyclic_solver.cpp:1334-1342).  Where the factorization of
+// a failed problem met a zero, non-finite or near-singular pivot (the reference falls
+// back from LDL^H to LU on a zero pivot, src/hermitian_lanczos.cpp:27-55), the retry moves
+// the shift down by 1 % of max(|sigma|, 1), up to three times: the block method's Ritz values come from
+// K(k) and M(k) and do not depend on the shift, and the Lanczos method maps its Ritz
+// values back with the shift it used (tm_result.shift_used).  Other failures get one
+// retry.  The Coriolis pencil factors K_eff(k) itself, with no shift to move: a flagged
+// factor there stays failed.
+static void retry_failed(Context* ctx, const tm_problem* problems, int n, tm_result* results,
+                         std::vector<int> flags) {
+  std::vector<tm_problem> cur(problems, problems + n);
+  for (int attempt = 1; attempt <= 3; ++attempt)
+    for (int i = 0; i < n; ++i) {
+      if (results[i].status == TM_OK) continue;
+      const Operator& op = *static_cast<const Operator*>(cur[i].op);
+      if (flags[i] && is_qep(op.mesh->S, cur[i])) continue;
+      if (!flags[i] && attempt > 1) continue;
+      if (flags[i]) cur[i].shift -= 0.01 * std::max(std::fabs(cur[i].shift), 1.0);
+      flags[i] = solve_problems_impl(ctx, &cur[i], 1, &results[i])[0];
+      results[i].factor_retries = attempt;
+      results[i].pivot_flags = flags[i];
+      if (results[i].status == TM_OK)
+        results[i].shift_used = is_qep(op.mesh->S, cur[i]) ? 0.0 : cur[i].shift;
+    }
+}
 
 // Mode shapes reach the caller's buffers through the context's deliverer; every
 // copy has landed when this returns, also when it throws.
 void solve_problems(Context* ctx, const tm_problem* problems, int n, tm_result* results) {
   try {
-    solve_problems_impl(ctx, problems, n, results);
+    std::vector<int> flags = solve_problems_impl(ctx, problems, n, results);
+    if (results) {
+      for (int i = 0; i < n; ++i) {
+        const Operator& op = *static_cast<const Operator*>(problems[i].op);
+        results[i].factor_retries = 0;
+        results[i].pivot_flags = flags[i];
+        results[i].shift_used = is_qep(op.mesh->S, problems[i]) ? 0.0 : problems[i].shift;
+      }
+      if (ctx->deliver) ctx->deliver->drain();
+      retry_failed(ctx, problems, n, results, flags);
+    }
   } catch (...) {
     if (ctx->deliver) ctx->deliver->drain_quiet();
     throw;
@@ -2711,7 +2871,7 @@ void solve_problems(Context* ctx, const tm_problem* problems, int n, tm_result* 
   if (ctx->deliver) ctx->deliver->drain();
 }
 
-static void solve_problems_impl(Context* ctx, const tm_problem* problems, int n, tm_result* results) {
+static std::vector<int> solve_problems_impl(Context* ctx, const tm_problem* problems, int n, tm_result* results) {
   std::vector<HostProblem> hps(n);
   // batches: problems of one operator and one method, ordered by nev so a
   // batch shares its block size and basis length
@@ -2789,6 +2949,9 @@ static void solve_problems_impl(Context* ctx, const tm_problem* problems, int n,
       i = j;
     }
   }
+  std::vector<int> flags(n);
+  for (int i = 0; i < n; ++i) flags[i] = hps[i].factor_flags;
+  return flags;
 }
 
 }  // namespace tme
@@ -2866,6 +3029,8 @@ struct Server {
     if (p.method != TM_METHOD_LANCZOS && p.method != TM_METHOD_BLOCK) throw ArgError("unknown method");
     Operator* op = static_cast<Operator*>(p.op);
     if (p.harmonic < 0 || p.harmonic > op->mesh->S.N / 2) throw ArgError("harmonic index out of range 0..N/2");
+    if (p.precision != TM_PREC_FP64)
+      throw ArgError("precision: only TM_PREC_FP64 is provided (no complex-FP32 factor path)");
     auto r = std::make_unique<Req>();
     r->hp.in = p;
     r->hp.out = out;
@@ -2876,6 +3041,9 @@ struct Server {
       out->restarts = 0;
       out->status = TM_ERR_RUNTIME;
       out->seconds = 0;
+      out->factor_retries = 0;  // the server drops a failed problem without a retry
+      out->pivot_flags = 0;
+      out->shift_used = is_qep(op->mesh->S, p) ? 0.0 : p.shift;
     }
     std::lock_guard<std::mutex> g(mu);
     if (!fatal.empty()) throw HipError("server stopped: " + fatal);

@@ -85,10 +85,25 @@ struct FactorArgs {
   double2* U;
   long long PE, UE;
   int* status;
+  int* pivinfo;           // per problem: negative pivots, smallest |d_j| / |a_jj| (float bits); may be null
+  const int* diag_slot;   // per free node: slot of its diagonal block
   const ProbParam* params;
   const int* active;  // per problem: factor it (null: all)
   const OpArrays* ops_b;  // per problem operator arrays (null: op for all)
 };
+
+// Diagonal entry (i, i) of A(k) = K + s dK - sigma M (+ the phase parts) at DOF comp of
+// free node I, before elimination (1 at a pinned DOF), and |sigma| times that of M(k)
+// (0 at a pinned DOF).  On the diagonal block the transposed slot is the slot itself,
+// so the phase parts add 2 Re(P) times the real diagonal of the coupling block.
+__device__ __forceinline__ double2 diag_entry(const FactorArgs& fa, const OpArrays& op, const ProbParam& pp,
+                                              int node, int comp) {
+  if (fa.pinned[3 * node + comp]) return make_double2(1.0, 0.0);
+  const long long e = (long long)fa.diag_slot[node] * 9 + 4 * comp;
+  const double m = op.Mc[e] + 2.0 * pp.pre * op.Mp[e];
+  const double c = op.Kc[e] + pp.s * op.dKc[e] + 2.0 * pp.pre * (op.Kp[e] + pp.s * op.dKp[e]);
+  return make_double2(c - pp.sigma * m, fabs(pp.sigma * m));
+}
 
 __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const AsmTask* __restrict__ tasks,
                                                         int t0) {
@@ -166,6 +181,15 @@ __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const Asm
   }
 }
 
+// pivot statistics of the problems about to be factored: no negative pivot yet,
+// smallest relative pivot +inf
+__global__ void k_pivinfo_init(int* pivinfo, const int* active, int B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B || (active && !active[b])) return;
+  pivinfo[2 * b] = 0;
+  pivinfo[2 * b + 1] = (int)__float_as_uint(INFINITY);
+}
+
 __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __restrict__ tasks, int t0,
                                                   int kb) {
   const int id = tasks[t0 + blockIdx.x];
@@ -181,10 +205,28 @@ __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __re
     int i = idx % jb, j = idx / jb;
     if (i >= j) A[i][j] = Pn[(kb + i) + (long long)(kb + j) * m];
   }
+  // the diagonals of A(k) and of sigma M(k) at these columns before elimination, for the
+  // pivot-size test
+  __shared__ double2 a0[NBMAX];
+  if (tid < jb) {
+    const int g = kb + tid;
+    a0[tid] = diag_entry(fa, fa.ops_b ? fa.ops_b[b] : fa.op, fa.params[b], s.col0 + g / 3, g % 3);
+  }
   __syncthreads();
+  int nneg = 0, tiny = 0;
+  float rmin = INFINITY;
   for (int j = 0; j < jb; ++j) {
     const double d = A[j][j].x;
-    if (tid == 0 && (d == 0.0 || !isfinite(d))) atomicOr(&fa.status[b], 1);
+    if (tid == 0) {
+      if (d == 0.0 || !isfinite(d)) atomicOr(&fa.status[b], 1);
+      nneg += d < 0.0 ? 1 : 0;
+      rmin = fminf(rmin, (float)(fabs(d) / fabs(a0[j].x)));
+      // near-singular: the pivot has all but vanished (relative to the entry it came
+      // from) and what cancelled it is the shift (the pivot is small against sigma M_jj
+      // too): the shift sits on an eigenvalue.  A pivot that is small only against
+      // K_jj, as sigma = 1 leaves above the rigid-body modes of a free structure, is not
+      tiny |= fabs(d) <= kTinyPivot * fabs(a0[j].x) && fabs(d) <= kTinyShiftPivot * a0[j].y;
+    }
     const double inv = 1.0 / d;
     for (int i = j + 1 + tid; i < jb; i += 256) A[i][j] = make_double2(A[i][j].x * inv, A[i][j].y * inv);
     __syncthreads();
@@ -228,6 +270,14 @@ __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __re
     }
   }
   __syncthreads();
+  if (tid == 0) {
+    if (tiny) atomicOr(&fa.status[b], 2);
+    if (fa.pivinfo) {
+      if (nneg) atomicAdd(&fa.pivinfo[2 * b], nneg);
+      // non-negative floats order as their bit patterns
+      atomicMin(reinterpret_cast<unsigned int*>(&fa.pivinfo[2 * b + 1]), __float_as_uint(rmin));
+    }
+  }
   for (int idx = tid; idx < jb * jb; idx += 256) {
     int i = idx % jb, j = idx / jb;
     if (i > j) {
@@ -1113,6 +1163,8 @@ static FactorArgs factor_args(const Mesh& M, const Plan& P, const OpArrays& ops,
   fa.PE = P.panel_entries;
   fa.UE = P.upool_entries;
   fa.status = bb.status;
+  fa.pivinfo = bb.pivinfo;
+  fa.diag_slot = M.d_diag_slot.as<int>();
   fa.params = bb.params;
   fa.active = nullptr;
   fa.ops_b = nullptr;
@@ -1126,6 +1178,8 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
   fa.ops_b = ops_b;
   InvArgs ia{P.d_sn.as<DevSn>(), bb.L, bb.S, P.panel_entries, P.inv_scratch_entries, active};
   const int B = bb.B;
+  if (bb.pivinfo)
+    hipLaunchKernelGGL(k_pivinfo_init, dim3((B + 63) / 64), dim3(64), 0, ctx->stream, bb.pivinfo, active, B);
   KTimer t(ctx, "factor", 0, M.S.flops * (nactive >= 0 ? nactive : B));
   for (const LevelPlan& L : P.levels) {
     if (L.nasm)

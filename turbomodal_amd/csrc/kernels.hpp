@@ -114,6 +114,7 @@ struct Mesh {
   DevBuf d_kc_ptr, d_kc_src, d_kp_ptr, d_kp_src, d_f_ptr, d_f_src;
   DevBuf d_pinned, d_free_of_full, d_node_class;
   DevBuf d_struct, d_relmap, d_asm_slot, d_asm_pos;
+  DevBuf d_diag_slot;  // per free node: slot of its diagonal block (I, I)
 };
 
 // Assembled operator (mesh + material): FP64 phase-split node blocks, 9 doubles
@@ -148,10 +149,22 @@ struct BatchBufs {
   double2* U;        // B * upool_entries
   double2* S;        // B * inv_scratch_entries
   double2* T;        // B * p * tvec_entries (p right-hand sides)
-  int* status;       // B
+  int* status;       // B: bit 0 zero / non-finite pivot, bit 1 near-singular pivot
+                     // (|d_j| <= kTinyPivot |a_jj|, a_jj the diagonal of A(k) before elimination)
+  int* pivinfo;      // 2 B (may be null): negative pivots, bits of the smallest |d_j| / |a_jj| (float)
   const ProbParam* params;  // B
   int B;
 };
+
+// A factorization is flagged near-singular (status bit 1) when a pivot d_j has
+// |d_j| <= kTinyPivot |a_jj| (a_jj the diagonal of A(k) before elimination) and
+// |d_j| <= kTinyShiftPivot |sigma M_jj|.  On the reference's fixtures a shift on an
+// eigenvalue leaves |d_j| / |a_jj| at 1e-14 .. 1e-10 (thin plates at the top of that
+// range), a shift 1e-6 away from one at 1e-9 .. 1e-7, and sigma = 1 below the spectrum
+// at 1e-6 .. 1e-4; sigma = 1 above the rigid-body modes of a free structure leaves
+// about 1e-12, but there the pivot is of the size of sigma M_jj, not far below it.
+constexpr double kTinyPivot = 1e-10;
+constexpr double kTinyShiftPivot = 1e-3;
 
 void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& ops,
                   const BatchBufs& bb, const int* active = nullptr, int nactive = -1,
@@ -164,6 +177,9 @@ void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs&
 
 struct HostProblem;
 void create_operator(Context* ctx, Mesh& M, Operator& op);
+void add_springs(Context* ctx, Operator& op, const double* spring_full);
+void operator_inertia(Context* ctx, Operator& op, double rpm, int k, double shift, int64_t* neg, double* ratio,
+                      int* status);
 OpArrays op_arrays(const Operator& op);
 const double2* harmonic_mass(Context* ctx, Operator& op, int k);
 // solves every problem; results written through each problem's tm_result

@@ -133,6 +133,16 @@ def _validate_modal_results(results):
     return results
 
 
+def compute_modal_mass(mesh: Mesh, material: Material, results, *, temperature: float | None = None):
+    """Full-annulus modal mass N * Re(phi^H M phi) per mode, one (n_modes,) array per result
+    (python/turbomodal/solver.py:56-114).  The engine computes it on the device with the
+    modes, from the sector mass matrix of the material the results were solved with, so
+    ``material`` must be that material (the temperature law changes only E,
+    src/material.cpp:33-38, and leaves M as it is); ``mesh`` is not needed."""
+    del mesh, material, temperature
+    return [np.asarray(getattr(r, "_result", r).modal_mass, dtype=np.float64) for r in results]
+
+
 def _enrich(results):
     return [EnrichedModalResult(_result=r, modal_mass_kg=np.asarray(r.modal_mass, dtype=np.float64))
             for r in results]
