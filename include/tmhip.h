@@ -266,6 +266,16 @@ int tm_operator_add_springs(tm_operator* op, const double* spring_full);
 int tm_operator_inertia(tm_operator* op, double rpm, int32_t harmonic, double shift,
                         int64_t* n_negative, double* min_pivot_ratio, int32_t* pivot_status);
 
+/* tm_operator_inertia without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops or harnesses that forbid hipMalloc).  The factor workspace comes from a static
+ * 64 MiB device array of the library (one per device, held until the process ends); a mesh
+ * whose one-problem workspace exceeds it is rejected with TM_ERR_ARG (the 24-sector
+ * fixtures and the 4.9k-DOF flat disk fit, the 45k-DOF bladed sector does not).  Calls that
+ * overlap on one device share that array and race: serialise them. */
+int tm_operator_inertia_noalloc(tm_operator* op, double rpm, int32_t harmonic, double shift,
+                                int64_t* n_negative, double* min_pivot_ratio, int32_t* pivot_status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,7 @@ EXPORTS = (
     "tm_problem_footprint", "tm_device_free_bytes", "tm_operator_export",
     "tm_operator_static_u", "tm_server_create", "tm_server_submit", "tm_server_wait",
     "tm_server_destroy", "tm_set_memory_reserve", "tm_operator_add_springs", "tm_operator_inertia",
+    "tm_operator_inertia_noalloc",
 )
 
 
@@ -112,6 +113,7 @@ def lib():
     L.tm_server_destroy.restype = None
     L.tm_operator_add_springs.argtypes = [vp, P(dbl)]
     L.tm_operator_inertia.argtypes = [vp, dbl, i32, dbl, P(i64), P(dbl), P(i32)]
+    L.tm_operator_inertia_noalloc.argtypes = [vp, dbl, i32, dbl, P(i64), P(dbl), P(i32)]
     _lib = L
     return L
 
@@ -247,11 +249,13 @@ class EngineOperator:
             raise ValueError("spring_full must hold 3 * n_nodes stiffnesses")
         self.mesh.ctx.check(self.mesh.ctx.L.tm_operator_add_springs(self.h, _ptr(k, C.c_double)))
 
-    def inertia(self, rpm: float, harmonic: int, shift: float):
-        """(negative pivots of the LDL^H of K(k) - shift M(k), smallest |d| / |a_jj|, status bits)."""
+    def inertia(self, rpm: float, harmonic: int, shift: float, noalloc: bool = False):
+        """(negative pivots of the LDL^H of K(k) - shift M(k), smallest |d| / |a_jj|, status bits);
+        noalloc: through tm_operator_inertia_noalloc (the library's static device scratch)."""
         neg, ratio, st = C.c_int64(), C.c_double(), C.c_int32()
-        self.mesh.ctx.check(self.mesh.ctx.L.tm_operator_inertia(self.h, float(rpm), int(harmonic), float(shift),
-                                                                C.byref(neg), C.byref(ratio), C.byref(st)))
+        fn = self.mesh.ctx.L.tm_operator_inertia_noalloc if noalloc else self.mesh.ctx.L.tm_operator_inertia
+        self.mesh.ctx.check(fn(self.h, float(rpm), int(harmonic), float(shift), C.byref(neg), C.byref(ratio),
+                               C.byref(st)))
         return int(neg.value), float(ratio.value), int(st.value)
 
     def footprint(self, num_modes, harmonic, precision=TM_PREC_FP64, method=None):

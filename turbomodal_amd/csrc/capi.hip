@@ -14,6 +14,15 @@ namespace {
 
 thread_local std::string g_error;
 
+// Device scratch of tm_operator_inertia_noalloc: one problem's factor workspace for
+// meshes up to this size (the 24-sector fixtures and the 4.9k-DOF flat disk fit; the
+// 45k-DOF bladed sector does not).  Every device the library runs on holds its own copy
+// until the process ends.  Host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (Context::scratch), never through
+// the array's name.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 template <class F>
 int guarded(Context* ctx, F&& f) {
   try {
@@ -92,8 +101,13 @@ int tm_context_create(int device, tm_context** out) {
     if (device < 0) TM_HIP(hipGetDevice(&device));
     if (device >= ndev) throw ArgError("device index out of range");
     TM_HIP(hipSetDevice(device));
+    // this device's copy of the scratch array (with the device current)
+    void* sp = nullptr;
+    TM_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
     auto* ctx = new tm_context();
     ctx->device = device;
+    ctx->scratch = sp;
+    ctx->scratch_bytes = kScratchBytes;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
       delete ctx;
@@ -357,6 +371,23 @@ int tm_operator_inertia(tm_operator* op, double rpm, int32_t harmonic, double sh
     double r = 0;
     int st = 0;
     operator_inertia(ctx, *op, rpm, harmonic, shift, n_negative, &r, &st);
+    if (min_pivot_ratio) *min_pivot_ratio = r;
+    if (pivot_status) *pivot_status = st;
+  });
+}
+
+int tm_operator_inertia_noalloc(tm_operator* op, double rpm, int32_t harmonic, double shift,
+                                int64_t* n_negative, double* min_pivot_ratio, int32_t* pivot_status) {
+  if (!op || !n_negative) return TM_ERR_ARG;
+  Context* ctx = op->mesh->ctx;
+  return guarded(ctx, [&] {
+    if (harmonic < 0 || harmonic > op->mesh->S.N / 2) throw ArgError("harmonic index out of range 0..N/2");
+    if (!std::isfinite(shift) || !std::isfinite(rpm)) throw ArgError("rpm and shift must be finite");
+    if (!ctx->scratch) throw HipError("device scratch not resolved");
+    TM_HIP(hipSetDevice(ctx->device));
+    double r = 0;
+    int st = 0;
+    operator_inertia_in(ctx, *op, rpm, harmonic, shift, ctx->scratch, ctx->scratch_bytes, n_negative, &r, &st);
     if (min_pivot_ratio) *min_pivot_ratio = r;
     if (pivot_status) *pivot_status = st;
   });

@@ -762,6 +762,40 @@ void operator_inertia(Context* ctx, Operator& op, double rpm, int k, double shif
   inertia_into(ctx, op, rpm, k, shift, bm.bufs(op.mesh->plan, 1), neg, ratio, status);
 }
 
+// Bytes of one problem's factor workspace laid out by operator_inertia_in.
+static size_t factor_workspace_bytes(const Plan& P) {
+  auto al = [](size_t n) { return (n + 255) / 256 * 256; };
+  return al(sizeof(double2) * P.panel_entries) + al(sizeof(double2) * P.upool_entries) +
+         al(sizeof(double2) * P.inv_scratch_entries) + al(sizeof(int)) + al(2 * sizeof(int)) + al(sizeof(ProbParam));
+}
+
+// The same with the workspace carved out of caller-given device memory (no allocation).
+void operator_inertia_in(Context* ctx, Operator& op, double rpm, int k, double shift, void* work, size_t work_bytes,
+                         int64_t* neg, double* ratio, int* status) {
+  const Plan& P = op.mesh->plan;
+  const size_t need = factor_workspace_bytes(P);
+  if (need > work_bytes)
+    throw ArgError("the factor workspace of this mesh (" + std::to_string(need) + " B) exceeds the " +
+                   std::to_string(work_bytes) + " B device scratch of the allocation-free entry point");
+  auto al = [](size_t n) { return (n + 255) / 256 * 256; };
+  char* p = static_cast<char*>(work);
+  BatchBufs bb;
+  bb.L = reinterpret_cast<double2*>(p);
+  p += al(sizeof(double2) * P.panel_entries);
+  bb.U = reinterpret_cast<double2*>(p);
+  p += al(sizeof(double2) * P.upool_entries);
+  bb.T = nullptr;  // the factorization uses no solve vectors
+  bb.S = reinterpret_cast<double2*>(p);
+  p += al(sizeof(double2) * P.inv_scratch_entries);
+  bb.status = reinterpret_cast<int*>(p);
+  p += al(sizeof(int));
+  bb.pivinfo = reinterpret_cast<int*>(p);
+  p += al(2 * sizeof(int));
+  bb.params = reinterpret_cast<const ProbParam*>(p);
+  bb.B = 1;
+  inertia_into(ctx, op, rpm, k, shift, bb, neg, ratio, status);
+}
+
 // M(k) blocks of the operator, cached per harmonic.  Operators are shared by
 // the server's slot groups (each on its own thread and stream): the first user
 // builds the blocks on its stream and records an event that every later user's

@@ -211,6 +211,10 @@ struct KernelStat {
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
+  // device address of the library's static scratch array on this context's device
+  // (hipGetSymbolAddress at context creation), for the allocation-free entry point
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
   DevPool pool;  // workspaces of the solve batches
   std::string last_error;
   bool profiling = false;

@@ -180,6 +180,10 @@ void create_operator(Context* ctx, Mesh& M, Operator& op);
 void add_springs(Context* ctx, Operator& op, const double* spring_full);
 void operator_inertia(Context* ctx, Operator& op, double rpm, int k, double shift, int64_t* neg, double* ratio,
                       int* status);
+// the same, its one-problem factor workspace carved out of work (work_bytes of device
+// memory the caller owns); no allocation
+void operator_inertia_in(Context* ctx, Operator& op, double rpm, int k, double shift, void* work, size_t work_bytes,
+                         int64_t* neg, double* ratio, int* status);
 OpArrays op_arrays(const Operator& op);
 const double2* harmonic_mass(Context* ctx, Operator& op, int k);
 // solves every problem; results written through each problem's tm_result
