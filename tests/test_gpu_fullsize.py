@@ -16,12 +16,10 @@ size-independent properties.
 Bars: relative frequency error <= 1e-8, MAC >= 0.999 (on the fixture's fixed DOF subset),
 ||K phi - lambda M phi|| / ||K phi|| <= max(1e-8, 4 x its FP64 floor).
 """
-import os
-
 import numpy as np
 import pytest
 
-from common import (GOLDEN, engine_mesh, exported, free_vector, generated, load_golden, oracle_model,
+from common import (engine_mesh, exported, free_vector, generated, load_golden, oracle_model,
                     residual_floor, subset_mac)
 from oracle import cyclic
 from turbomodal_amd import meshgen
@@ -201,20 +199,8 @@ def test_c3_doubled_sector(ctx):
         assert np.allclose(o.modal_mass, N, rtol=1e-9)
         neg, ratio, st = _complete(op, o, nu)
         assert st == 0 and neg == nu, (o.harmonic, neg, nu, ratio)
-    gold = load_golden("c3_nd01.npz") if os.path.exists(os.path.join(GOLDEN, "c3_nd01.npz")) else None
+    gold = load_golden("c3_nd01.npz")
     for k in (0, 1):
-        o = outs[k]
-        if gold is not None:
-            _check_sample(m, op, o, gold, f"k{k}", n)
-        else:  # residuals and orthonormality only
-            K = exported(op, rpm, k, 0, n)
-            M = exported(op, rpm, k, 1, n)
-            step = 1 if k == 0 else 2
-            Xv = np.stack([free_vector(m, o.mode_shapes[:, q]) for q in range(0, 20, step)], axis=1)
-            for j in range(Xv.shape[1]):
-                res, fl = residual_floor(K, M, Xv[:, j], o.eigenvalues[j])
-                assert res <= max(RES_TOL, 4 * fl), (k, j, res, fl)
-            G = Xv.conj().T @ (M @ Xv)
-            assert np.abs(G - np.eye(Xv.shape[1])).max() <= 1e-9
+        _check_sample(m, op, outs[k], gold, f"k{k}", n)
     op.close()
     em.close()

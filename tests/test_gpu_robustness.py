@@ -161,3 +161,40 @@ def test_mixed_precision_is_refused(ctx):
     assert op.footprint(4, 1) > 0
     op.close()
     em.close()
+
+
+def test_many_modes_block_and_coriolis(ctx):
+    # more wanted pairs than the default bases hold: the block method takes a longer basis
+    # (k = 0: 60 pairs, basis 160; k = N/2: 100 pairs, basis 240), and so does the Coriolis
+    # pencil (31 modes: basis 112), against the oracle (the reference requests
+    # min(6 M, 2 n - 1) QEP eigenvalues, src/modal_solver.cpp:222-229, and keeps M)
+    from turbomodal_amd import capi
+    X, T, S, N = generated("flat_disk_c1")
+    m = oracle_model(X, T, S, N)
+    em = engine_mesh(ctx, m)
+    op = capi.EngineOperator(em, 200e9, 0.3, 7800.0, True)
+    rpm = 6000.0
+    ops = cyclic.sector_operators(m, 200e9, 0.3, 7800.0, rpm)
+    outs = capi.solve_batch(ctx, [dict(op=op, rpm=rpm, harmonic=0, num_modes=60),
+                                  dict(op=op, rpm=rpm, harmonic=N // 2, num_modes=100)])
+    for o, nm in zip(outs, (60, 100)):
+        assert o.status == 0 and o.converged and len(o.frequencies) == nm
+        ref = cyclic.solve_harmonic(m, ops, o.harmonic, nm, rpm, "tight")
+        assert np.max(np.abs(o.frequencies / ref.frequencies - 1)) <= 1e-8, o.harmonic
+        neg, ratio, st = op.inertia(rpm, o.harmonic, float(o.eigenvalues[nm - 1]) * (1 + 1e-7))
+        assert neg == nm
+    op.close()
+    em.close()
+    X, T, S, N = fixture_mesh("blade_sector.msh")
+    m = oracle_model(X, T, S, N)
+    em = engine_mesh(ctx, m)
+    op = capi.EngineOperator(em, 200e9, 0.3, 7800.0, True)
+    rpm = 20000.0
+    ops = cyclic.sector_operators(m, 200e9, 0.3, 7800.0, rpm)
+    q = capi.solve_batch(ctx, [dict(op=op, rpm=rpm, harmonic=2, num_modes=31, coriolis=True)])[0]
+    assert q.status == 0 and len(q.frequencies) == 31
+    ref = cyclic.solve_harmonic_qep(m, ops, 2, 31, rpm)
+    assert np.max(np.abs(q.frequencies / ref.frequencies - 1)) <= 1e-8
+    assert np.array_equal(q.whirl, ref.whirl_direction)
+    op.close()
+    em.close()

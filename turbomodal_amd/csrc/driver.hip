@@ -838,7 +838,16 @@ inline int block_width(int nev, bool qep) { return (nev <= kNarrowNev && !qep) ?
 // Basis length of the block method: 8 blocks of 8; 16-wide blocks 8 of them (6
 // for the Coriolis pencil, whose vectors are twice as long).  With these the C4
 // sweep restarts almost never and the Coriolis problems about twice on average.
-inline int basis_length(int pb, bool qep) { return pb == 8 ? 64 : (qep ? 96 : 128); }
+// Problems wanting more pairs get a longer basis: its kept half holds the nev pairs and
+// one more block.  Up to nev = 24 (48 for the Coriolis pencil) every problem of a block
+// size has the same basis.
+inline int basis_length(int pb, bool qep, int nev) {
+  const int base = pb == 8 ? 64 : (qep ? 96 : 128);
+  const int need = (2 * (nev + pb) + pb - 1) / pb * pb;
+  return std::max(base, need);
+}
+// the most pairs the block method takes on (basis of at most 256 vectors)
+constexpr int kBlockMaxNev = 112;
 
 struct HostProblem {
   tm_problem in;
@@ -1783,8 +1792,8 @@ class BatchLanczos {
 
   void block_setup() {
     pb_ = block_width(nevmax_, qep_);
-    mmax_ = basis_length(pb_, qep_);
-    if (mmax_ > kTsMaxRows || mmax_ % pb_) throw ArgError("basis length");
+    mmax_ = basis_length(pb_, qep_, nevmax_);
+    if (mmax_ > 2 * kTsMaxRows || mmax_ % pb_) throw ArgError("basis length");
     nkeep_ = mmax_ / 2;
     static_assert(kTsMaxRows >= 128, "tsmm coefficient rows");
     ldv_ = (long long)mmax_ * nvec_;
@@ -1883,9 +1892,10 @@ class BatchLanczos {
     if (len < 0) len = nvec_;
     DevBuf dc = upload(pool(), C, stream());
     KTimer t(ctx_, "vec", 0, 0);
-    hipLaunchKernelGGL(k_tsmm, dim3((len + 255) / 256, B_), dim3(256), 0, stream(), A, lda, len, nvec_, dc.as<double2>(),
-                       ka.as<int>(), kc, act.as<int>(), mmax_, alpha, beta, Y, ldy,
-                       yoff ? yoff->as<int>() : (const int*)nullptr);
+    for (int r0 = 0; r0 < mmax_; r0 += kTsMaxRows)
+      hipLaunchKernelGGL(k_tsmm, dim3((len + 255) / 256, B_), dim3(256), 0, stream(), A, lda, len, nvec_,
+                         dc.as<double2>(), ka.as<int>(), kc, act.as<int>(), mmax_, alpha, r0 ? 1.0 : beta, Y, ldy,
+                         yoff ? yoff->as<int>() : (const int*)nullptr, r0);
   }
 
   // Y_b[:, c] (+)= A_b X_b[:, c], c < ncol_b (<= cmax), n rows; problems ld apart,
@@ -2691,12 +2701,16 @@ static bool use_block(const Symbolic& S, const tm_problem& p) {
   fill_policy(S, hp);
   if (hp.qep) {
     if (S.n_free_dofs < kQepMinFree) throw ArgError("the Coriolis QEP needs at least 64 free DOFs");
-    if (hp.nev > 24) throw ArgError("the Coriolis QEP is limited to 22 modes per harmonic");
+    // a basis longer than the default one needs vectors (2 n long) of twice its length
+    if (hp.nev > kBlockMaxNev || (hp.nev > 24 && S.n_free_dofs < basis_length(16, true, hp.nev)))
+      throw ArgError("the Coriolis QEP takes at most 110 modes per harmonic, and 31 or more modes need at least "
+                     "as many free DOFs as basis vectors");
     return true;
   }
   if (p.method != TM_METHOD_BLOCK || p.conv_policy == TM_CONV_REFERENCE || S.n_free_dofs < kBlockMinFree)
     return false;
-  return hp.nev <= 24;  // the kept Ritz block (32 vectors) must cover nev with room to spare
+  // the basis must stay well inside the space; longer requests run the Lanczos method
+  return hp.nev <= kBlockMaxNev && S.n_free_dofs >= 2 * basis_length(block_width(hp.nev, false), false, hp.nev);
 }
 
 static int64_t problem_footprint(const Operator& op, const tm_problem& p) {
@@ -2711,7 +2725,7 @@ static int64_t problem_footprint(const Operator& op, const tm_problem& p) {
   const int pcols = std::min(16, std::max(block ? pb : 1, hp.nev));
   int64_t bytes = 16 * (P.panel_entries + P.upool_entries + pcols * P.tvec_entries + P.inv_scratch_entries);
   if (block) {
-    const int mmax = basis_length(pb, hp.qep);
+    const int mmax = basis_length(pb, hp.qep, hp.nev);
     const int64_t nv = hp.qep ? 2 * n : n;           // pencil vectors are 2n long
     bytes += 3 * 16 * (int64_t)mmax * nv;            // V, MV, KV
     bytes += 3 * 16 * (int64_t)pb * nv;              // W, MW, KW
@@ -2949,8 +2963,13 @@ static std::vector<int> solve_problems_impl(Context* ctx, const tm_problem* prob
       std::vector<std::vector<HostProblem*>> runs;
       for (size_t i = 0; i < list.size();) {
         size_t j = i;
-        const int pb_i = block_width(list[i]->nev, false);
-        while (j < list.size() && (qep || block_width(list[j]->nev, false) == pb_i)) ++j;
+        const int pb_i = block_width(list[i]->nev, qep);
+        const int mm_i = basis_length(pb_i, qep, list[i]->nev);
+        // one block size and one basis length per run: a problem's arithmetic does not
+        // depend on what shares its run
+        while (j < list.size() && block_width(list[j]->nev, qep) == pb_i &&
+               basis_length(pb_i, qep, list[j]->nev) == mm_i)
+          ++j;
         runs.emplace_back(list.begin() + i, list.begin() + j);
         i = j;
       }
@@ -3000,8 +3019,9 @@ static std::vector<int> solve_problems_impl(Context* ctx, const tm_problem* prob
 // admits queued problems as slots free up (factorizations batched, masked to
 // the new slots), steps all its slots together, retires finished problems into
 // the caller's result buffers, and returns its memory when it runs dry.
-// Problems too small for the block method run as Lanczos batches on the
-// dispatcher thread.
+// Problems too small for the block method (and the Lanczos method's requests) run
+// as Lanczos batches on a worker thread of their own, with its own stream and pool,
+// so they never hold up the dispatcher's admissions.
 // ---------------------------------------------------------------------------
 namespace tme {
 
@@ -3015,6 +3035,7 @@ struct Server {
     const Mesh* mesh = nullptr;
     Operator* rep = nullptr;  // any operator of the mesh (plan, pattern)
     int pbclass = 8;
+    int nevclass = 0;  // a wanted-pair count of its basis class (any one: they share the basis length)
     bool qep = false;
     std::unique_ptr<SubContext> sub;
     std::unique_ptr<BatchLanczos> bl;
@@ -3039,7 +3060,11 @@ struct Server {
   std::map<uint64_t, std::unique_ptr<Req>> reqs;
   std::map<HostProblem*, uint64_t> ticket_of;
   uint64_t next_ticket = 1;
-  std::map<std::pair<const Mesh*, int>, std::unique_ptr<Group>> groups;  // (mesh, block size + 1000 qep)
+  std::map<std::pair<const Mesh*, int>, std::unique_ptr<Group>> groups;  // (mesh, block size + 1000 qep + basis)
+  // Lanczos-method requests: batches per operator for the worker thread (guarded by mu)
+  std::deque<std::pair<Operator*, std::vector<HostProblem*>>> lanczos_q;
+  bool lanczos_quit = false;  // guarded by mu
+  std::thread lanczos_th;
 
   Server(Context* c, int slots, double f) : ctx(c), max_slots(std::max(1, std::min(64, slots))), frac(f) {
     if (!(frac > 0.0 && frac <= 0.95)) frac = 0.85;
@@ -3163,7 +3188,7 @@ struct Server {
         }
         if (need_setup) {
           g->bl = std::make_unique<BatchLanczos>(&g->sub->c, *g->rep, std::vector<HostProblem*>{}, true);
-          g->bl->slot_setup((int)per, g->pbclass == 8 ? 6 : 24, g->qep);
+          g->bl->slot_setup((int)per, g->pbclass == 8 ? kNarrowNev : std::max(24, g->nevclass), g->qep);
         }
         {
           std::lock_guard<std::mutex> lk(mu);
@@ -3202,6 +3227,41 @@ struct Server {
     }
   }
 
+  // Lanczos batches, one operator at a time (requests queued meanwhile for the same
+  // operator join the next batch), on a context of the worker's own
+  void run_lanczos() {
+    try {
+      SubContext sc(ctx->device);
+      while (true) {
+        std::map<Operator*, std::vector<HostProblem*>> batch;
+        {
+          std::unique_lock<std::mutex> lk(mu);
+          cv_group.wait(lk, [&] { return lanczos_quit || !lanczos_q.empty() || !fatal.empty(); });
+          if (!fatal.empty()) return;
+          if (lanczos_q.empty()) return;  // quit with nothing left
+          while (!lanczos_q.empty()) {
+            auto& f = lanczos_q.front();
+            auto& v = batch[f.first];
+            v.insert(v.end(), f.second.begin(), f.second.end());
+            lanczos_q.pop_front();
+          }
+        }
+        for (auto& kv : batch) {
+          for (size_t i = 0; i < kv.second.size(); i += 64) {  // at most 64 problems per batch
+            std::vector<HostProblem*> chunk(kv.second.begin() + i,
+                                            kv.second.begin() + std::min(kv.second.size(), i + 64));
+            BatchLanczos bl(&sc.c, *kv.first, chunk, false);
+            bl.run();
+            sc.c.deliverer().drain();
+            finish(chunk);
+          }
+        }
+      }
+    } catch (const std::exception& e) {
+      fail(e.what());
+    }
+  }
+
   void dispatch() {
     try {
       TM_HIP(hipSetDevice(ctx->device));
@@ -3228,14 +3288,16 @@ struct Server {
             continue;
           }
           const int pbc = block_width(hp->nev, hp->qep);
+          const int mmc = basis_length(pbc, hp->qep, hp->nev);
           std::lock_guard<std::mutex> g(mu);
-          auto& gp = groups[{op->mesh, pbc + (hp->qep ? 1000 : 0)}];
+          auto& gp = groups[{op->mesh, pbc + (hp->qep ? 1000 : 0) + 10000 * mmc}];
           if (!gp) {
             gp = std::make_unique<Group>();
             gp->srv = this;
             gp->mesh = op->mesh;
             gp->rep = op;
             gp->pbclass = pbc;
+            gp->nevclass = hp->nev;
             gp->qep = hp->qep;
             gp->sub = std::make_unique<SubContext>(ctx->device);
             Group* raw = gp.get();
@@ -3244,24 +3306,26 @@ struct Server {
           gp->queue.push_back(hp);
           cv_group.notify_all();
         }
-        for (auto& kv : small) {
-          BatchLanczos bl(ctx, *kv.first, kv.second, false);
-          bl.run();
-          ctx->deliverer().drain();
-          finish(kv.second);
+        if (!small.empty()) {
+          std::lock_guard<std::mutex> g(mu);
+          if (!lanczos_th.joinable()) lanczos_th = std::thread([this] { run_lanczos(); });
+          for (auto& kv : small) lanczos_q.emplace_back(kv.first, std::move(kv.second));
+          cv_group.notify_all();
         }
       }
     } catch (const std::exception& e) {
       fail(e.what());
     }
-    // drain: every group finishes its queue, then stops
+    // drain: every group and the Lanczos worker finish their queues, then stop
     {
       std::lock_guard<std::mutex> g(mu);
       for (auto& kv : groups) kv.second->quit = true;
+      lanczos_quit = true;
       cv_group.notify_all();
     }
     for (auto& kv : groups)
       if (kv.second->th.joinable()) kv.second->th.join();
+    if (lanczos_th.joinable()) lanczos_th.join();
     if (std::getenv("TMHIP_SERVER_STATS"))
       for (auto& kv : groups) {
         const Group& g = *kv.second;

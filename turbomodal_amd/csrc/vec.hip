@@ -505,22 +505,24 @@ __global__ void k_tsgemm_reduce(const double2* __restrict__ part, const int* __r
   C[((long long)b * kamax + a) * 16 + c] = make_double2(re, im);
 }
 
-// Y_b[:, c] = beta Y_b[:, c] + sum_{a < ka_b} A_b[:, a] C_b[a][c], c < kc
-// (C_b row-major kamax x 16); one thread per row
+// Y_b[:, c] = beta Y_b[:, c] + sum_{r0 <= a < min(ka_b, r0 + kTsMaxRows)} A_b[:, a] C_b[a][c],
+// c < kc (C_b row-major kamax x 16); one thread per row.  Bases longer than kTsMaxRows
+// take one launch per kTsMaxRows rows of C (r0 > 0 with beta = 1).
 __global__ __launch_bounds__(256) void k_tsmm(const double2* __restrict__ A, long long lda_b, long long n,
                                               long long cs, const double2* __restrict__ C, const int* __restrict__ ka_b,
                                               int kc, const int* __restrict__ act, int kamax, double alpha,
                                               double beta, double2* __restrict__ Y, long long ldy_b,
-                                              const int* __restrict__ yoff) {
+                                              const int* __restrict__ yoff, int r0) {
   const int b = blockIdx.y;
   if (act && !act[b]) return;
-  const int ka = ka_b[b];
+  const int ka = min(ka_b[b] - r0, kTsMaxRows);
+  if (r0 > 0 && ka <= 0) return;  // nothing to add to Y
   __shared__ double2 Cs[kTsMaxRows * 16];
-  for (int idx = threadIdx.x; idx < ka * 16; idx += 256) Cs[idx] = C[(long long)b * kamax * 16 + idx];
+  for (int idx = threadIdx.x; idx < ka * 16; idx += 256) Cs[idx] = C[((long long)b * kamax + r0) * 16 + idx];
   __syncthreads();
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double2* Ab = A + b * lda_b;
+  const double2* Ab = A + b * lda_b + (long long)r0 * cs;
   double2 acc[16];
 #pragma unroll
   for (int c = 0; c < 16; ++c) acc[c] = make_double2(0.0, 0.0);

@@ -76,7 +76,7 @@ __global__ void k_tsgemm_reduce(const double2* part, const int* ka_b, int kb, co
                                 double2* C);
 __global__ void k_tsmm(const double2* A, long long lda_b, long long n, long long cs, const double2* C, const int* ka_b, int kc,
                        const int* act, int kamax, double alpha, double beta, double2* Y, long long ldy_b,
-                       const int* yoff);
+                       const int* yoff, int r0);
 __global__ void k_copy_cols(const double2* src, long long lds, const int* s0v, double2* dst, long long ldd,
                             const int* d0v, int ncol, long long n, const int* act);
 template <int CPL>
