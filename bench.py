@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 METRIC = "ND modal solves/sec + p99 latency, TET10 bladed sector, freq err ≤1e-8, MAC≥.999"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 
 def build_solver(workload, precision):
@@ -261,7 +262,8 @@ def main():
     ap.add_argument("--c5-plain", type=int, default=24)
     ap.add_argument("--c5-c3", action="store_true", help="include the ~330k-DOF C3 generator in the stream")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
+                    help="mixed: complex-FP32 factor with FP64 refinement of every shift-invert solve")
     ap.add_argument("--method", default="block", choices=["block", "lanczos"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 one-at-a-time measurement")
@@ -291,6 +293,7 @@ def main():
     tm, sm, mesh, mat, solver = build_solver(args.workload, args.precision)
     from turbomodal_amd import meshgen
     solver.engine_method = args.method
+    solver.engine_precision = args.precision
     rpms, num_modes = workload_spec(args.workload)
     n_problems = len(rpms) * (sm.num_sectors // 2 + 1)
     core = tm._core
@@ -350,29 +353,32 @@ def main():
     measured_on = ("the timed steps (groups serial)" if args.serial_groups else
                    "one sweep after the timed steps with the block-size groups serial")
     roofline = roofline_tri = None
-    fr = rate(kstats, "factor")
+    mixed = args.precision == "mixed"
+    fr = rate(kstats, "factor_fp32" if mixed else "factor")
     if fr:
         avg_ms, _, flops_per = fr
         ach = flops_per / (avg_ms * 1e-3) / 1e12
+        peak = FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
         roofline = {"bound": "mfma", "kernel": "factorization (batched supernodal LDL^H: front assembly, diagonal "
-                                              "blocks, panel solves, Schur tiles on v_mfma_f64_16x16x4, pivot-block "
-                                              "inversion)",
-                    "achieved": round(ach, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": measured_traffic(args, "factor"),
+                                              "blocks, panel solves, Schur tiles on v_mfma_%s_16x16x4, pivot-block "
+                                              "inversion)" % ("f32" if mixed else "f64"),
+                    "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+                    "frac": round(ach / peak, 4), "traffic": None if mixed else measured_traffic(args, "factor"),
                     "avg_launch_ms": round(avg_ms, 3), "alg_flops_per_launch": flops_per,
                     "measured_on": measured_on}
-    tr = rate(kstats, "trisolve")
+    tr = rate(kstats, "trisolve_fp32" if mixed else "trisolve")
     if tr:
         avg_ms, bytes_per, _ = tr
         ach = bytes_per / (avg_ms * 1e-3) / 1e9
         roofline_tri = {"bound": "hbm", "kernel": "trisolve (L, D, L^H sweep pair)", "achieved": round(ach, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                        "traffic": measured_traffic(args, "trisolve"), "avg_launch_ms": round(avg_ms, 3),
+                        "traffic": None if mixed else measured_traffic(args, "trisolve"),
+                        "avg_launch_ms": round(avg_ms, 3),
                         "alg_bytes_per_launch": bytes_per, "measured_on": measured_on}
-        if not args.serial_groups and rate(stats, "trisolve"):
-            a2, b2, _ = rate(stats, "trisolve")
+        if not args.serial_groups and rate(stats, "trisolve_fp32" if mixed else "trisolve"):
+            a2, b2, _ = rate(stats, "trisolve_fp32" if mixed else "trisolve")
             roofline_tri["frac_timed_region_groups_concurrent"] = round(b2 / (a2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-    tri = stats.get("trisolve", {})
+    tri = stats.get("trisolve_fp32" if mixed else "trisolve", {})
     all_res = np.concatenate([np.asarray(r.residuals) for row in results for r in row])
 
     out = {
@@ -386,7 +392,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128 (complex f64)",
+        "dtype": "c128 (complex f64)" if args.precision == "fp64" else "c64 factor (complex f32) + c128 refinement",
         "data": "synthetic (seeded Kuhn TET10 lattice meshes, %s)" % ("titanium" if args.workload == "c3" else "steel"),
         "config": {"workload": {"c4": "rpm sweep bladed sector 20 rpm x 19 ND", "c2": "bladed sector 19 ND @10000 rpm",
                                 "c1": "flat disk 19 ND @0 rpm",
@@ -439,12 +445,14 @@ def main():
     if not args.no_plain and args.workload in ("c4", "c2"):
         rpm_p = float(rpms[len(rpms) * 2 // 3])
         solver.engine_method = "lanczos"
+        solver.engine_precision = "fp64"
         n_h = sm.num_sectors // 2 + 1
         tp = time.perf_counter()
         for k in range(n_h):
             solver.solve_at_rpm(rpm_p, num_modes, [k], include_coriolis=args.coriolis)
         plain = n_h / (time.perf_counter() - tp)
         solver.engine_method = args.method
+        solver.engine_precision = args.precision
         out["plain_fp64_one_at_a_time"] = {"value": round(plain, 3), "unit": "ND solves/s", "problems": n_h,
                                            "sample": f"ND 0-{n_h - 1} at {rpm_p:.0f} rpm, one solve_at_rpm call each",
                                            "eigensolver": "lanczos (single vector)"}

@@ -54,9 +54,12 @@ extern "C" {
 
 /* precision of the shift-invert factor */
 #define TM_PREC_FP64 0       /* complex FP64 factor, FP64 everywhere */
-#define TM_PREC_MIXED 1      /* complex FP32 factor with FP64 refinement: not provided by this
-                                engine; tm_solve_batch and tm_problem_footprint reject it
-                                (TM_ERR_ARG / -1) */
+#define TM_PREC_MIXED 1      /* complex FP32 factor (FP32 MFMA Schur updates, factor stored in
+                                FP32), every shift-invert solve refined in FP64 against FP64
+                                K(k), M(k) until it stalls; FP64 Rayleigh-Ritz and residuals.
+                                Block method on the (K, M) pencil only (not the Lanczos
+                                method, TM_CONV_REFERENCE, the Coriolis pencil or the server):
+                                those requests return TM_ERR_ARG */
 
 /* eigen-solver convergence policy */
 #define TM_CONV_TIGHT 0      /* relative Ritz residual <= tol_rel (default 1e-12) */

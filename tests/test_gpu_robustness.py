@@ -7,7 +7,6 @@
 * Grounded springs (ELASTIC_SUPPORT, src/cyclic_solver.cpp:193-212, 747-751) through the
   C ABI and through the drop-in ConstraintGroup, against the oracle with the same springs.
 * Sylvester inertia of K(k) - sigma M(k) against the dense spectrum.
-* TM_PREC_MIXED is refused (no complex-FP32 factor path).
 """
 import numpy as np
 import pytest
@@ -147,20 +146,6 @@ def test_springs_through_the_dropin_constraint_groups():
     for r in res:
         ref = cyclic.solve_harmonic(m, ops, r.harmonic_index, 5, 3000.0, "tight")
         assert np.max(np.abs(np.asarray(r.frequencies) / ref.frequencies - 1)) <= 1e-8
-
-
-def test_mixed_precision_is_refused(ctx):
-    from turbomodal_amd import capi
-    X, T, S, N = fixture_mesh("wedge_sector.msh")
-    m = oracle_model(X, T, S, N)
-    em = engine_mesh(ctx, m)
-    op = capi.EngineOperator(em, 200e9, 0.3, 7800.0, True)
-    with pytest.raises(ValueError):
-        capi.solve_batch(ctx, [dict(op=op, rpm=0.0, harmonic=1, num_modes=4, precision=capi.TM_PREC_MIXED)])
-    assert op.footprint(4, 1, precision=capi.TM_PREC_MIXED) == -1
-    assert op.footprint(4, 1) > 0
-    op.close()
-    em.close()
 
 
 def test_many_modes_block_and_coriolis(ctx):

@@ -269,8 +269,8 @@ int tm_solve_batch(tm_context* ctx, const tm_problem* problems, int32_t n, tm_re
   return guarded(ctx, [&] {
     if (ctx->server) throw std::runtime_error("the context is owned by a server; submit to it instead");
     for (int i = 0; i < n; ++i)
-      if (problems[i].precision != TM_PREC_FP64)
-        throw ArgError("precision: only TM_PREC_FP64 is provided (no complex-FP32 factor path)");
+      if (problems[i].precision != TM_PREC_FP64 && problems[i].precision != TM_PREC_MIXED)
+        throw ArgError("precision must be TM_PREC_FP64 or TM_PREC_MIXED");
     TM_HIP(hipSetDevice(ctx->device));
     for (int i = 0; i < n; ++i)
       if (results) {
@@ -321,9 +321,9 @@ void tm_server_destroy(tm_server* s) {
 
 int64_t tm_problem_footprint(const tm_operator* op, int32_t num_modes, int32_t precision,
                              int32_t harmonic, int32_t method) {
-  if (!op || precision != TM_PREC_FP64) return -1;
+  if (!op || (precision != TM_PREC_FP64 && precision != TM_PREC_MIXED)) return -1;
   try {
-    return problem_footprint(*op, num_modes, harmonic, method);
+    return problem_footprint(*op, num_modes, harmonic, method, precision);
   } catch (...) {
     return -1;
   }

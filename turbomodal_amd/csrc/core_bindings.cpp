@@ -595,6 +595,9 @@ class CyclicSymmetrySolver {
  public:
   // eigensolver of the engine: TM_METHOD_BLOCK (default) or TM_METHOD_LANCZOS
   int method_ = TM_METHOD_BLOCK;
+  // factor precision: TM_PREC_FP64 (default) or TM_PREC_MIXED (complex-FP32 factor,
+  // FP64 refinement of every shift-invert solve; block method only)
+  int precision_ = TM_PREC_FP64;
 
  private:
 
@@ -689,7 +692,7 @@ class CyclicSymmetrySolver {
         p.rpm = rpm;
         p.harmonic = k;
         p.num_modes = num_modes;
-        p.precision = TM_PREC_FP64;
+        p.precision = precision_;
         p.conv_policy = TM_CONV_TIGHT;
         p.shift = 1.0;
         p.tol_rel = 1e-12;
@@ -1025,6 +1028,14 @@ PYBIND11_MODULE(_core, m) {
             if (v == "block") s.method_ = TM_METHOD_BLOCK;
             else if (v == "lanczos") s.method_ = TM_METHOD_LANCZOS;
             else throw std::invalid_argument("engine_method: 'block' or 'lanczos'");
+          })
+      .def_property(
+          "engine_precision",
+          [](const CyclicSymmetrySolver& s) { return std::string(s.precision_ == TM_PREC_MIXED ? "mixed" : "fp64"); },
+          [](CyclicSymmetrySolver& s, const std::string& v) {
+            if (v == "fp64") s.precision_ = TM_PREC_FP64;
+            else if (v == "mixed") s.precision_ = TM_PREC_MIXED;
+            else throw std::invalid_argument("engine_precision: 'fp64' or 'mixed'");
           })
       .def("build_constrained_dofs", &CyclicSymmetrySolver::build_constrained_dofs)
       .def_static("compute_stationary_frame", &CyclicSymmetrySolver::compute_stationary_frame,

@@ -571,24 +571,28 @@ void herm_gen_eig(int n, const std::vector<double>& A, const std::vector<double>
 struct BatchMem {
   DevBuf L, U, T, S, status, pivinfo, params;
   int pcols = 1;
+  bool fp32 = false;  // complex-FP32 factor (TM_PREC_MIXED)
   BatchBufs bufs(const Plan&, int B) {
     BatchBufs bb;
-    bb.L = L.as<double2>();
-    bb.U = U.as<double2>();
+    bb.L = L.p;
+    bb.U = U.p;
+    bb.S = S.p;
+    bb.fp32 = fp32;
     bb.T = T.as<double2>();
-    bb.S = S.as<double2>();
     bb.status = status.as<int>();
     bb.pivinfo = pivinfo.as<int>();
     bb.params = params.as<ProbParam>();
     bb.B = B;
     return bb;
   }
-  void alloc(DevPool* pool, const Plan& P, int B, int p = 1) {
+  void alloc(DevPool* pool, const Plan& P, int B, int p = 1, bool single = false) {
     pcols = p;
-    L.alloc(pool, sizeof(double2) * P.panel_entries * B);
-    U.alloc(pool, sizeof(double2) * P.upool_entries * B);
+    fp32 = single;
+    const size_t es = single ? sizeof(float2) : sizeof(double2);
+    L.alloc(pool, es * P.panel_entries * B);
+    U.alloc(pool, es * P.upool_entries * B);
     T.alloc(pool, sizeof(double2) * P.tvec_entries * B * p);
-    S.alloc(pool, sizeof(double2) * P.inv_scratch_entries * B);
+    S.alloc(pool, es * P.inv_scratch_entries * B);
     status.alloc(pool, sizeof(int) * B);
     pivinfo.alloc(pool, sizeof(int) * 2 * B);
     params.alloc(pool, sizeof(ProbParam) * B);
@@ -780,12 +784,12 @@ void operator_inertia_in(Context* ctx, Operator& op, double rpm, int k, double s
   auto al = [](size_t n) { return (n + 255) / 256 * 256; };
   char* p = static_cast<char*>(work);
   BatchBufs bb;
-  bb.L = reinterpret_cast<double2*>(p);
+  bb.L = p;
   p += al(sizeof(double2) * P.panel_entries);
-  bb.U = reinterpret_cast<double2*>(p);
+  bb.U = p;
   p += al(sizeof(double2) * P.upool_entries);
   bb.T = nullptr;  // the factorization uses no solve vectors
-  bb.S = reinterpret_cast<double2*>(p);
+  bb.S = p;
   p += al(sizeof(double2) * P.inv_scratch_entries);
   bb.status = reinterpret_cast<int*>(p);
   p += al(sizeof(int));
@@ -1420,6 +1424,74 @@ class BatchLanczos {
     }
   }
 
+  // FP64 iterative refinement of the shift-invert solves done with the complex-FP32
+  // factor: W_b[:, c] (the W block's layout) approximates A_b^-1 Y_b[:, s0_b + c],
+  // c < nc_b, A = K(k) - sigma M(k).  Each step forms R = Y - (K W - sigma M W) in FP64
+  // (the split K/M product) and, while the largest ||R_c|| / ||Y_c|| of a problem still
+  // falls by 10 % or more per step and is above kMixedTol, adds A~^-1 R (the FP32 factor
+  // A~) to W; at most kMixedMaxRefine steps.  The residual's own FP64 rounding
+  // (eps ||A|| ||W||) is where it stalls, as an FP64 factor's solve would.  The norms are
+  // reduced in a fixed order, so the step count depends on the problem only.  KW and MW
+  // serve as scratch.
+  static constexpr int kMixedMaxRefine = 40;
+  static constexpr double kMixedTol = 1e-14;
+  void mixed_refine(double2* W, const double2* Y, long long ldy, const DevBuf* s0, const std::vector<int>& nc) {
+    int ncol = 0;
+    std::vector<int> act(B_, 0);
+    for (int b = 0; b < B_; ++b)
+      if (nc[b] > 0) {
+        act[b] = 1;
+        ncol = std::max(ncol, nc[b]);
+      }
+    std::vector<double> prev(B_, std::numeric_limits<double>::infinity());
+    BatchBufs bb = bm_.bufs(P_, B_);
+    const dim3 g((unsigned)((n_ + 255) / 256), B_);
+    DevBuf part(pool(), sizeof(double) * 2 * (size_t)B_ * ncol * nchunk_);
+    std::vector<double> hp(2 * (size_t)B_ * ncol * nchunk_);
+    for (int it = 0; it <= kMixedMaxRefine; ++it) {
+      int nact = 0;
+      for (int b = 0; b < B_; ++b) nact += act[b];
+      if (!nact) break;
+      std::vector<int> ncv(B_, 0);
+      for (int b = 0; b < B_; ++b) ncv[b] = act[b] ? nc[b] : 0;
+      DevBuf dnc = ints(ncv), dact = ints(act);
+      spmv_km(W, ldw_, dnc, ncol, KWb_.as<double2>(), MWb_.as<double2>());
+      {
+        KTimer t(ctx_, "vec", 0, 0);
+        hipLaunchKernelGGL(k_mixed_residual, g, dim3(256), 0, stream(), Y, ldy, s0 ? s0->as<int>() : (const int*)nullptr,
+                           KWb_.as<double2>(), MWb_.as<double2>(), ldw_, bm_.params.as<ProbParam>(), ncol, n_,
+                           dact.as<int>());
+        hipLaunchKernelGGL(k_colnorm_partial, dim3(nchunk_, B_), dim3(256), 0, stream(), KWb_.as<double2>(), ldw_, Y,
+                           ldy, s0 ? s0->as<int>() : (const int*)nullptr, ncol, n_, dact.as<int>(),
+                           part.as<double>(), nchunk_);
+      }
+      TM_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, stream()));
+      TM_HIP(hipStreamSynchronize(stream()));
+      for (int b = 0; b < B_; ++b) {
+        if (!act[b]) continue;
+        double worst = 0.0;
+        for (int c = 0; c < nc[b]; ++c) {
+          double rr = 0.0, yy = 0.0;
+          for (int q = 0; q < nchunk_; ++q) {
+            rr += hp[(((size_t)b * ncol + c) * nchunk_ + q) * 2];
+            yy += hp[(((size_t)b * ncol + c) * nchunk_ + q) * 2 + 1];
+          }
+          worst = std::max(worst, yy > 0.0 ? std::sqrt(rr / yy) : 0.0);
+        }
+        if (!(worst > kMixedTol) || !(worst < 0.9 * prev[b]) || it == kMixedMaxRefine) act[b] = 0;
+        prev[b] = worst;
+      }
+      nact = 0;
+      for (int b = 0; b < B_; ++b) nact += act[b];
+      if (!nact) break;
+      DevBuf dact2 = ints(act);
+      trisolve_batch(ctx_, M_, P_, bb, KWb_.as<double2>(), ldw_, dact2.as<int>(), ncol, -1, nact);
+      KTimer t(ctx_, "vec", 0, 0);
+      hipLaunchKernelGGL(k_add_cols, g, dim3(256), 0, stream(), W, KWb_.as<double2>(), ldw_, ncol, n_,
+                         dact2.as<int>());
+    }
+  }
+
   // block inverse iteration + Rayleigh-Ritz for the problems with nr[b] > 0
   void refine(double2* X, double2* Y, double2* KY, double2* MY, long long ldx, int nemax, const std::vector<int>& nr,
               const DevBuf& kp) {
@@ -1433,6 +1505,24 @@ class BatchLanczos {
     for (int q0 = 0; q0 < nemax; q0 += bm_.pcols)
       trisolve_batch(ctx_, M_, P_, bb, Y + (long long)q0 * n_, ldx, act_.as<int>(), std::min(bm_.pcols, nemax - q0),
                      -1, nact);
+    if (mixed_)  // refine the columns in W-block chunks: copy in, refine against M X, copy back
+      for (int q0 = 0; q0 < nemax; q0 += pb_) {
+        const int kc = std::min(pb_, nemax - q0);
+        std::vector<int> nc(B_, 0);
+        for (int b = 0; b < B_; ++b) nc[b] = std::max(0, std::min(kc, nr[b] - q0));
+        DevBuf dnc = ints(nc);
+        {
+          KTimer t(ctx_, "vec", 0, 0);
+          hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), Y + (long long)q0 * n_,
+                             ldx, (const int*)nullptr, Wb_.as<double2>(), ldw_, (const int*)nullptr, kc, n_,
+                             dnc.as<int>());
+        }
+        mixed_refine(Wb_.as<double2>(), MY + (long long)q0 * n_, ldx, nullptr, nc);
+        KTimer t(ctx_, "vec", 0, 0);
+        hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), Wb_.as<double2>(), ldw_,
+                           (const int*)nullptr, Y + (long long)q0 * n_, ldx, (const int*)nullptr, kc, n_,
+                           dnc.as<int>());
+      }
     if (!qep_ && &kp == &kptr_) spmv_km(Y, ldx, dnr, nemax, KY, MY);  // K Y, M Y in one pass
     else spmv_pair(kp, &mk_ptrs_, Y, ldx, dnr, nemax, KY, MY, false, false);
     const size_t gsz = (size_t)B_ * nemax * nemax * nchunk_;
@@ -1772,6 +1862,8 @@ class BatchLanczos {
   bool block_ = false;
   // Coriolis pencil (L1, L2) on 2n-long vectors z = [a; b] instead of (K, M) on n
   bool qep_ = false;
+  // TM_PREC_MIXED: complex-FP32 factor; every shift-invert solve refined in FP64
+  bool mixed_ = false;
   long long nvec_ = 0;  // basis vector length: n, or 2n for the pencil
   long long cs_ = 0;    // column stride of the vectors handed to the SpMVs
   std::vector<DevBuf> dblk_;
@@ -2160,19 +2252,20 @@ class BatchLanczos {
  public:
   // Allocates the workspaces of `capacity` slots for problems of at most nevmax
   // wanted pairs (one block size).
-  void slot_setup(int capacity, int nevmax, bool qep = false) {
+  void slot_setup(int capacity, int nevmax, bool qep = false, bool mixed = false) {
     HostTimer h(ctx_, "setup");
     B_ = capacity;
     probs_.assign(B_, nullptr);
     n_ = 3LL * S_.nf;
     qep_ = qep;
+    mixed_ = mixed;
     nvec_ = qep ? 2 * n_ : n_;
     cs_ = nvec_;
     nevmax_ = nevmax;
     ncvmax_ = 1;
     nchunk_ = static_cast<int>((nvec_ + kDotChunk - 1) / kDotChunk);
     const int pcols = std::min(16, std::max(block_width(nevmax_, qep_), nevmax_));
-    bm_.alloc(pool(), P_, B_, pcols);
+    bm_.alloc(pool(), P_, B_, pcols, mixed_);
     act_.alloc(pool(), sizeof(int) * B_);
     h_params_.assign(B_, ProbParam{1.0, 0.0, 0.0, 1.0});
     h_ops_.assign(B_, op_arrays(op_));
@@ -2270,6 +2363,11 @@ class BatchLanczos {
                            dy0.as<int>(), Wb_.as<double2>(), ldw_, (const int*)nullptr, pb_, n_, dact.as<int>());
       }
       trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>(), ldw_, dact.as<int>(), pb_, -1, nit);
+      if (mixed_) {
+        std::vector<int> nc(B_, 0);
+        for (int b = 0; b < B_; ++b) nc[b] = iter_[b] ? pb_ : 0;
+        mixed_refine(Wb_.as<double2>(), MVb_.as<double2>(), ldv_, &dy0, nc);
+      }
     } else {
       // W = L1^-1 L2 Y: for Y = [a; b], W = [b; K^-1 (M a - D b)] (one solve with K(k))
       std::vector<int> pbv(B_, 0);
@@ -2328,10 +2426,10 @@ class BatchLanczos {
   // the slots between admissions (best when the whole queue is known: the
   // factorizations stay fully batched); smaller values let problems that
   // converge early hand their slots on.
-  void run_slots(std::vector<HostProblem*> queue, int capacity, int refill, bool qep = false) {
+  void run_slots(std::vector<HostProblem*> queue, int capacity, int refill, bool qep = false, bool mixed = false) {
     int nevmax = 1;
     for (auto* hp : queue) nevmax = std::max(nevmax, hp->nev);
-    slot_setup(capacity, nevmax, qep);
+    slot_setup(capacity, nevmax, qep, mixed);
     refill = std::max(1, std::min(refill, capacity));
     size_t next = 0;
     while (true) {
@@ -2723,7 +2821,8 @@ static int64_t problem_footprint(const Operator& op, const tm_problem& p) {
   const bool block = use_block(S, hp.in);
   const int pb = block_width(hp.nev, hp.qep);
   const int pcols = std::min(16, std::max(block ? pb : 1, hp.nev));
-  int64_t bytes = 16 * (P.panel_entries + P.upool_entries + pcols * P.tvec_entries + P.inv_scratch_entries);
+  const int64_t es = p.precision == TM_PREC_MIXED ? 8 : 16;  // factor entry: complex FP32 or FP64
+  int64_t bytes = es * (P.panel_entries + P.upool_entries + P.inv_scratch_entries) + 16 * pcols * P.tvec_entries;
   if (block) {
     const int mmax = basis_length(pb, hp.qep, hp.nev);
     const int64_t nv = hp.qep ? 2 * n : n;           // pencil vectors are 2n long
@@ -2740,11 +2839,12 @@ static int64_t problem_footprint(const Operator& op, const tm_problem& p) {
   return bytes;
 }
 
-int64_t problem_footprint(const Operator& op, int num_modes, int harmonic, int method) {
+int64_t problem_footprint(const Operator& op, int num_modes, int harmonic, int method, int precision) {
   tm_problem p{};
   p.harmonic = harmonic;
   p.num_modes = num_modes;
   p.method = method;
+  p.precision = precision;
   return problem_footprint(op, p);
 }
 
@@ -2815,7 +2915,7 @@ static void merge_stats(Context* ctx, Context* g) {
 // proportion to the runs' sizes.  One run (or TMHIP_SERIAL_GROUPS=1): one after
 // another on the caller's context.
 static void run_block_groups(Context* ctx, Operator& op, std::vector<std::vector<HostProblem*>>& runs, int64_t per,
-                             bool qep) {
+                             bool qep, bool mixed) {
   per = std::max<int64_t>(per, 1);
   const char* ser = std::getenv("TMHIP_SERIAL_GROUPS");
   const bool serial = runs.size() == 1 || (ser && std::atoi(ser) != 0);
@@ -2828,7 +2928,7 @@ static void run_block_groups(Context* ctx, Operator& op, std::vector<std::vector
       const int64_t budget = (int64_t)(ctx->batch_fraction * (double)(free_b + ctx->pool.cached()));
       const int cap = (int)std::max<int64_t>(1, std::min<int64_t>({64, (int64_t)q.size(), budget / per}));
       BatchLanczos bl(ctx, op, {}, true);
-      bl.run_slots(q, cap, cap, qep);
+      bl.run_slots(q, cap, cap, qep, mixed);
     }
     return;
   }
@@ -2854,7 +2954,7 @@ static void run_block_groups(Context* ctx, Operator& op, std::vector<std::vector
         const int cap = (int)std::max<int64_t>(1, std::min<int64_t>({64, (int64_t)runs[g].size(), share / per}));
         {
           BatchLanczos bl(c, op, {}, true);
-          bl.run_slots(runs[g], cap, cap, qep);
+          bl.run_slots(runs[g], cap, cap, qep, mixed);
         }
         if (c->deliver) c->deliver->drain();
       } catch (...) {
@@ -2941,7 +3041,13 @@ static std::vector<int> solve_problems_impl(Context* ctx, const tm_problem* prob
       throw ArgError("harmonic index out of range 0..N/2");
     const bool blk = use_block(op->mesh->S, problems[i]);
     const bool qep = blk && is_qep(op->mesh->S, problems[i]);
-    const auto key = std::make_pair(blk ? (const void*)op->mesh : (const void*)op, blk ? (qep ? 2 : 1) : 0);
+    const bool mixed = problems[i].precision == TM_PREC_MIXED;
+    if (mixed && (!blk || qep))
+      throw ArgError("TM_PREC_MIXED runs the block method on the (K, M) pencil only: not the Lanczos method, the "
+                     "reference convergence policy, the Coriolis pencil or systems below 256 free DOFs");
+    // key: 0 Lanczos (per operator); per mesh: 1 block method, 2 Coriolis pencil, 3 block method with
+    // the complex-FP32 factor
+    const auto key = std::make_pair(blk ? (const void*)op->mesh : (const void*)op, blk ? (qep ? 2 : (mixed ? 3 : 1)) : 0);
     if (!groups.count(key)) {
       order.push_back(key);
       rep[key] = op;
@@ -2985,7 +3091,7 @@ static std::vector<int> solve_problems_impl(Context* ctx, const tm_problem* prob
         runs[big].swap(a);
         runs.insert(runs.begin() + big + 1, std::move(b));
       }
-      run_block_groups(ctx, *op, runs, per, qep);
+      run_block_groups(ctx, *op, runs, per, qep, key.second == 3);
       continue;
     }
     size_t free_b = 0, total_b = 0;
@@ -3088,8 +3194,7 @@ struct Server {
     if (p.method != TM_METHOD_LANCZOS && p.method != TM_METHOD_BLOCK) throw ArgError("unknown method");
     Operator* op = static_cast<Operator*>(p.op);
     if (p.harmonic < 0 || p.harmonic > op->mesh->S.N / 2) throw ArgError("harmonic index out of range 0..N/2");
-    if (p.precision != TM_PREC_FP64)
-      throw ArgError("precision: only TM_PREC_FP64 is provided (no complex-FP32 factor path)");
+    if (p.precision != TM_PREC_FP64) throw ArgError("precision: the server runs TM_PREC_FP64 problems");
     auto r = std::make_unique<Req>();
     r->hp.in = p;
     r->hp.out = out;

@@ -44,6 +44,29 @@ __device__ __forceinline__ double2 cconjmul(double2 a, double2 b) {
 }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// the same for the complex-FP32 factor
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// element type of a factor: complex FP64 (double2) or complex FP32 (float2)
+template <class CT> struct real_of;
+template <> struct real_of<double2> { using type = double; };
+template <> struct real_of<float2> { using type = float; };
+template <class CT> using real_t = typename real_of<CT>::type;
+template <class CT> __device__ __forceinline__ CT mk(double re, double im);
+template <> __device__ __forceinline__ double2 mk<double2>(double re, double im) { return make_double2(re, im); }
+template <> __device__ __forceinline__ float2 mk<float2>(double re, double im) {
+  return make_float2((float)re, (float)im);
+}
+// a factor entry as complex FP64 (the triangular sweeps accumulate in FP64)
+__device__ __forceinline__ double2 ldc(double2 v) { return v; }
+__device__ __forceinline__ double2 ldc(float2 v) { return make_double2(v.x, v.y); }
 // acc += a b and acc += conj(a) b as four FMAs each
 __device__ __forceinline__ void cmac(double2& acc, double2 a, double2 b) {
   acc.x = fma(a.x, b.x, acc.x);
@@ -57,10 +80,16 @@ __device__ __forceinline__ void cmacc(double2& acc, double2 a, double2 b) {
   acc.y = fma(a.x, b.y, acc.y);
   acc.y = fma(-a.y, b.x, acc.y);
 }
+__device__ __forceinline__ void cmac(float2& acc, float2 a, float2 b) {
+  acc.x = fmaf(a.x, b.x, acc.x);
+  acc.x = fmaf(-a.y, b.y, acc.x);
+  acc.y = fmaf(a.x, b.y, acc.y);
+  acc.y = fmaf(a.y, b.x, acc.y);
+}
 
 // front element (i, j), i >= j: panel for pivot columns, update matrix otherwise
-__device__ __forceinline__ double2* front_at(double2* Pn, double2* Un, int m, int w, int r, int i,
-                                             int j) {
+template <class CT>
+__device__ __forceinline__ CT* front_at(CT* Pn, CT* Un, int m, int w, int r, int i, int j) {
   return j < w ? Pn + i + (long long)j * m : Un + (i - w) + (long long)(j - w) * r;
 }
 
@@ -81,8 +110,8 @@ struct FactorArgs {
   const int* tslot;
   const unsigned char* pinned;
   OpArrays op;
-  double2* L;
-  double2* U;
+  void* L;  // factor panels and update matrices: double2 or float2 (the kernels' CT)
+  void* U;
   long long PE, UE;
   int* status;
   int* pivinfo;           // per problem: negative pivots, smallest |d_j| / |a_jj| (float bits); may be null
@@ -105,6 +134,7 @@ __device__ __forceinline__ double2 diag_entry(const FactorArgs& fa, const OpArra
   return make_double2(c - pp.sigma * m, fabs(pp.sigma * m));
 }
 
+template <class CT>
 __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const AsmTask* __restrict__ tasks,
                                                         int t0) {
   const AsmTask t = tasks[t0 + blockIdx.x];
@@ -112,14 +142,14 @@ __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const Asm
   if (fa.active && !fa.active[b]) return;
   const DevSn s = fa.sn[t.sn];
   const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r;
-  double2* Pn = fa.L + b * fa.PE + s.poff;
-  double2* Un = fa.U + b * fa.UE + s.uoff;
+  CT* Pn = static_cast<CT*>(fa.L) + b * fa.PE + s.poff;
+  CT* Un = static_cast<CT*>(fa.U) + b * fa.UE + s.uoff;
   const int tid = threadIdx.x;
   const int j0 = 3 * t.n0, j1 = 3 * t.n1;
   // children's update entries pulled through the inverse relmaps: every lower
   // entry of columns [j0, j1) is written once, child0's term first
   const int* inv[2] = {nullptr, nullptr};
-  const double2* Uc[2] = {nullptr, nullptr};
+  const CT* Uc[2] = {nullptr, nullptr};
   int rc[2] = {0, 0};
   {
     int o = s.inv0;
@@ -129,7 +159,7 @@ __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const Asm
       const DevSn cs = fa.sn[c];
       inv[k] = fa.invmap + o;
       o += s.ncol + s.nstr;
-      Uc[k] = fa.U + b * fa.UE + cs.uoff;
+      Uc[k] = static_cast<const CT*>(fa.U) + b * fa.UE + cs.uoff;
       rc[k] = 3 * cs.nstr;
     }
   }
@@ -140,9 +170,9 @@ __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const Asm
 #pragma unroll
     for (int k = 0; k < 2; ++k) qj[k] = inv[k] ? inv[k][j / 3] : -1;
     const int ioff = j < w ? 0 : w;  // row i of column j at col[i - ioff]
-    double2* col = j < w ? Pn + (long long)j * m : Un + (long long)(j - w) * r;
+    CT* col = j < w ? Pn + (long long)j * m : Un + (long long)(j - w) * r;
     for (int i = j + lane; i < m; i += 64) {
-      double2 v = make_double2(0.0, 0.0);
+      CT v = mk<CT>(0.0, 0.0);
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         if (qj[k] < 0) continue;
@@ -164,9 +194,9 @@ __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const Asm
     if (gi < gj) continue;
     const int rnode = rowpos < s.ncol ? s.col0 + rowpos : fa.struct_nodes[s.str0 + rowpos - s.ncol];
     const int cnode = s.col0 + colpos;
-    double2* dst = front_at(Pn, Un, m, w, r, gi, gj);
+    CT* dst = front_at(Pn, Un, m, w, r, gi, gj);
     if (fa.pinned[3 * rnode + ci] || fa.pinned[3 * cnode + cj]) {
-      *dst = make_double2(gi == gj ? 1.0 : 0.0, 0.0);
+      *dst = mk<CT>(gi == gj ? 1.0 : 0.0, 0.0);
     } else {
       const long long sl = fa.asm_slot[q];
       const long long tt = fa.tslot[sl];
@@ -176,7 +206,7 @@ __global__ __launch_bounds__(256) void k_front_assemble(FactorArgs fa, const Asm
       double c = op.Kc[e1] + pp.s * op.dKc[e1] - pp.sigma * op.Mc[e1];
       double p = op.Kp[e1] + pp.s * op.dKp[e1] - pp.sigma * op.Mp[e1];
       double qv = op.Kp[e2] + pp.s * op.dKp[e2] - pp.sigma * op.Mp[e2];
-      *dst = cadd(*dst, make_double2(c + pp.pre * (p + qv), pp.pim * (p - qv)));
+      *dst = cadd(*dst, mk<CT>(c + pp.pre * (p + qv), pp.pim * (p - qv)));
     }
   }
 }
@@ -190,6 +220,7 @@ __global__ void k_pivinfo_init(int* pivinfo, const int* active, int B) {
   pivinfo[2 * b + 1] = (int)__float_as_uint(INFINITY);
 }
 
+template <class CT>
 __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __restrict__ tasks, int t0,
                                                   int kb) {
   const int id = tasks[t0 + blockIdx.x];
@@ -198,8 +229,9 @@ __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __re
   const DevSn s = fa.sn[id];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int jb = min(NBMAX, w - kb);
-  double2* Pn = fa.L + b * fa.PE + s.poff;
-  __shared__ double2 A[NBMAX][NBMAX + 1];
+  CT* Pn = static_cast<CT*>(fa.L) + b * fa.PE + s.poff;
+  using RT = real_t<CT>;
+  __shared__ CT A[NBMAX][NBMAX + 1];
   const int tid = threadIdx.x;
   for (int idx = tid; idx < jb * jb; idx += 256) {
     int i = idx % jb, j = idx / jb;
@@ -216,27 +248,27 @@ __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __re
   int nneg = 0, tiny = 0;
   float rmin = INFINITY;
   for (int j = 0; j < jb; ++j) {
-    const double d = A[j][j].x;
+    const RT d = A[j][j].x;
     if (tid == 0) {
       if (d == 0.0 || !isfinite(d)) atomicOr(&fa.status[b], 1);
       nneg += d < 0.0 ? 1 : 0;
-      rmin = fminf(rmin, (float)(fabs(d) / fabs(a0[j].x)));
+      rmin = fminf(rmin, (float)(fabs((double)d) / fabs(a0[j].x)));
       // near-singular: the pivot has all but vanished (relative to the entry it came
       // from) and what cancelled it is the shift (the pivot is small against sigma M_jj
       // too): the shift sits on an eigenvalue.  A pivot that is small only against
       // K_jj, as sigma = 1 leaves above the rigid-body modes of a free structure, is not
-      tiny |= fabs(d) <= kTinyPivot * fabs(a0[j].x) && fabs(d) <= kTinyShiftPivot * a0[j].y;
+      tiny |= fabs((double)d) <= kTinyPivot * fabs(a0[j].x) && fabs((double)d) <= kTinyShiftPivot * a0[j].y;
     }
-    const double inv = 1.0 / d;
-    for (int i = j + 1 + tid; i < jb; i += 256) A[i][j] = make_double2(A[i][j].x * inv, A[i][j].y * inv);
+    const RT inv = RT(1) / d;
+    for (int i = j + 1 + tid; i < jb; i += 256) A[i][j] = mk<CT>(A[i][j].x * inv, A[i][j].y * inv);
     __syncthreads();
     const int nt = jb - j - 1;
     for (int idx = tid; idx < nt * nt; idx += 256) {
       int ii = j + 1 + idx % nt, kk = j + 1 + idx / nt;
       if (ii >= kk) {
-        double2 l = A[ii][j], lk = A[kk][j];
-        double2 u = cmulc(l, lk);
-        A[ii][kk] = make_double2(A[ii][kk].x - u.x * d, A[ii][kk].y - u.y * d);
+        CT l = A[ii][j], lk = A[kk][j];
+        CT u = cmulc(l, lk);
+        A[ii][kk] = mk<CT>(A[ii][kk].x - u.x * d, A[ii][kk].y - u.y * d);
       }
     }
     __syncthreads();
@@ -248,15 +280,15 @@ __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __re
   // row's dot product split over the group and summed by shuffles within it.  A
   // group's LDS writes precede its next reads (one wave, in order; the waits keep
   // the compiler from moving them).
-  __shared__ double2 Li[NBMAX][NBMAX + 1];
+  __shared__ CT Li[NBMAX][NBMAX + 1];
   {
     const int c = tid >> 3, l8 = tid & 7;
     if (c < jb) {
-      if (l8 == 0) Li[c][c] = make_double2(1.0, 0.0);
+      if (l8 == 0) Li[c][c] = mk<CT>(1.0, 0.0);
       __builtin_amdgcn_s_waitcnt(0xc07f);
       __builtin_amdgcn_wave_barrier();
       for (int i = c + 1; i < jb; ++i) {
-        double2 v = make_double2(0.0, 0.0);
+        CT v = mk<CT>(0.0, 0.0);
         for (int k = c + l8; k < i; k += 8) v = csub(v, cmul(A[i][k], Li[k][c]));
 #pragma unroll
         for (int o = 1; o < 8; o <<= 1) {
@@ -284,11 +316,12 @@ __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __re
       Pn[(kb + i) + (long long)(kb + j) * m] = A[i][j];
       Pn[(kb + j) + (long long)(kb + i) * m] = Li[i][j];
     } else if (i == j) {
-      Pn[(kb + i) + (long long)(kb + j) * m] = make_double2(A[i][i].x, 0.0);
+      Pn[(kb + i) + (long long)(kb + j) * m] = mk<CT>(A[i][i].x, 0.0);
     }
   }
 }
 
+template <class CT>
 __global__ __launch_bounds__(256) void k_panel_trsm(FactorArgs fa, const RowTask* __restrict__ tasks,
                                                     int t0, int kb) {
   const RowTask t = tasks[t0 + blockIdx.x];
@@ -297,27 +330,28 @@ __global__ __launch_bounds__(256) void k_panel_trsm(FactorArgs fa, const RowTask
   const DevSn s = fa.sn[t.sn];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int jb = min(NBMAX, w - kb);
-  double2* Pn = fa.L + b * fa.PE + s.poff;
+  CT* Pn = static_cast<CT*>(fa.L) + b * fa.PE + s.poff;
+  using RT = real_t<CT>;
   // Li = inverse of the unit lower diagonal block (zero above the diagonal and
   // outside jb x jb), so that x = b L11^-H is a product with static indices;
   // k_diag_ldl left it transposed in the block's strictly upper triangle
-  __shared__ double2 Li[NBMAX][NBMAX + 1];
-  __shared__ double dk[NBMAX];
+  __shared__ CT Li[NBMAX][NBMAX + 1];
+  __shared__ RT dk[NBMAX];
   const int tid = threadIdx.x;
   for (int idx = tid; idx < NBMAX * NBMAX; idx += 256) {
     const int i = idx / NBMAX, j = idx % NBMAX;  // row i of Li is a contiguous piece of column kb + i
     Li[i][j] = (i > j && i < jb) ? Pn[(kb + j) + (long long)(kb + i) * m]
-                                 : make_double2(i == j && i < jb ? 1.0 : 0.0, 0.0);
+                                 : mk<CT>(i == j && i < jb ? 1.0 : 0.0, 0.0);
     if (i == j && i < jb) dk[i] = Pn[(kb + i) + (long long)(kb + i) * m].x;
   }
   __syncthreads();
   const int i = t.r0 + tid;
   if (i >= t.r1) return;
-  double2 x[NBMAX];
+  CT x[NBMAX];
 #pragma unroll
-  for (int k = 0; k < NBMAX; ++k) x[k] = k < jb ? Pn[i + (long long)(kb + k) * m] : make_double2(0.0, 0.0);
+  for (int k = 0; k < NBMAX; ++k) x[k] = k < jb ? Pn[i + (long long)(kb + k) * m] : mk<CT>(0.0, 0.0);
   for (int j = 0; j < jb; ++j) {
-    double2 v = make_double2(0.0, 0.0);
+    CT v = mk<CT>(0.0, 0.0);
 #pragma unroll
     for (int k = 0; k < NBMAX; ++k) {  // v += x_k conj(L_jk)
       v.x = fma(x[k].x, Li[j][k].x, v.x);
@@ -325,8 +359,8 @@ __global__ __launch_bounds__(256) void k_panel_trsm(FactorArgs fa, const RowTask
       v.y = fma(x[k].y, Li[j][k].x, v.y);
       v.y = fma(-x[k].x, Li[j][k].y, v.y);
     }
-    const double inv = 1.0 / dk[j];
-    Pn[i + (long long)(kb + j) * m] = make_double2(v.x * inv, v.y * inv);
+    const RT inv = RT(1) / dk[j];
+    Pn[i + (long long)(kb + j) * m] = mk<CT>(v.x * inv, v.y * inv);
   }
 }
 
@@ -341,7 +375,23 @@ __global__ __launch_bounds__(256) void k_panel_trsm(FactorArgs fa, const RowTask
 // Columns are staged SCHUR_KC at a time (34 KB of LDS per block, so three to
 // four blocks share a CU and one's staging overlaps another's products).
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 constexpr int SCHUR_KC = 16;
+// one 16x16x4 real product on the matrix cores, FP64 or FP32 (FP32 accumulation)
+template <class RT> struct v4of;
+template <> struct v4of<double> { using type = v4d; };
+template <> struct v4of<float> { using type = v4f; };
+__device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mfma16(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+// row of result register j of lane l in a 16x16 block (column l % 16): measured on
+// gfx950 with an in-bounds probe, v_mfma_f64_16x16x4 puts C[4 j + l / 16] there and
+// v_mfma_f32_16x16x4 C[4 (l / 16) + j]
+__device__ __forceinline__ int mfma_row(double, int j, int l4) { return 4 * j + l4; }
+__device__ __forceinline__ int mfma_row(float, int j, int l4) { return 4 * l4 + j; }
 
 // Work-groups are dealt round-robin over the eight XCDs (each with its own L2):
 // in a row of n blocks, the blocks with the same x mod 8 share an XCD.  Block x
@@ -352,6 +402,7 @@ __device__ __forceinline__ int xcd_task(int x, int n) {
   return c * q + min(c, rem) + (x >> 3);
 }
 
+template <class CT>
 __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTask* __restrict__ tasks,
                                                     int t0, int kb) {
   (void)kb;
@@ -361,21 +412,23 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
   const DevSn s = fa.sn[t.sn];
   const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r;
   const int jlim = t.jend < 0 ? m : min(t.jend, w);
-  double2* Pn = fa.L + b * fa.PE + s.poff;
-  double2* Un = fa.U + b * fa.UE + s.uoff;
-  __shared__ double2 Li[TILE][SCHUR_KC + 1];
-  __shared__ double2 Lj[TILE][SCHUR_KC + 1];
+  using RT = real_t<CT>;
+  using V4 = typename v4of<RT>::type;
+  CT* Pn = static_cast<CT*>(fa.L) + b * fa.PE + s.poff;
+  CT* Un = static_cast<CT*>(fa.U) + b * fa.UE + s.uoff;
+  __shared__ CT Li[TILE][SCHUR_KC + 1];
+  __shared__ CT Lj[TILE][SCHUR_KC + 1];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int r0 = 32 * (wv & 1), q0 = 32 * (wv >> 1);  // this wave's rows / columns in the tile
   const int l16 = lane & 15, l4 = lane >> 4;
-  v4d cr[2][2], ci[2][2];
+  V4 cr[2][2], ci[2][2];
 #pragma unroll
   for (int p = 0; p < 2; ++p)
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      cr[p][q] = v4d{0.0, 0.0, 0.0, 0.0};
-      ci[p][q] = v4d{0.0, 0.0, 0.0, 0.0};
+      cr[p][q] = V4{0, 0, 0, 0};
+      ci[p][q] = V4{0, 0, 0, 0};
     }
   // 16 x 16 blocks of this wave whose result is written: not wholly above the
   // diagonal, below row m, or right of column jlim (uniform over the wave)
@@ -390,7 +443,7 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
   // staging of one chunk: this thread's SCHUR_KC * TILE / 256 elements of each
   // panel, loaded one chunk ahead so the loads overlap the previous products
   constexpr int PER = TILE * SCHUR_KC / 256;
-  double2 ri[PER], rj[PER];
+  CT ri[PER], rj[PER];
   auto fetch = [&](int c0) {
     const int jb = min(SCHUR_KC, t.tk + t.kd - c0);
 #pragma unroll
@@ -399,16 +452,16 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
       const int gi = t.i0 + ii, gj = t.j0 + ii;
       const bool in = k < jb;
       const long long col = (long long)(c0 + k) * m;
-      ri[e] = in && gi < m ? Pn[gi + col] : make_double2(0.0, 0.0);
-      rj[e] = in && gj < m ? Pn[gj + col] : make_double2(0.0, 0.0);
+      ri[e] = in && gi < m ? Pn[gi + col] : mk<CT>(0.0, 0.0);
+      rj[e] = in && gj < m ? Pn[gj + col] : mk<CT>(0.0, 0.0);
     }
   };
-  __shared__ double dk[SCHUR_KC];
+  __shared__ RT dk[SCHUR_KC];
   fetch(t.tk);
   for (int c0 = t.tk; c0 < t.tk + t.kd; c0 += SCHUR_KC) {
     const int jb = min(SCHUR_KC, t.tk + t.kd - c0);
     __syncthreads();
-    if (tid < SCHUR_KC) dk[tid] = tid < jb ? Pn[(c0 + tid) + (long long)(c0 + tid) * m].x : 0.0;
+    if (tid < SCHUR_KC) dk[tid] = tid < jb ? Pn[(c0 + tid) + (long long)(c0 + tid) * m].x : RT(0);
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
       const int idx = tid + 256 * e, ii = idx % TILE, k = idx / TILE;
@@ -420,12 +473,12 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
     const int kend = (jb + 3) & ~3;  // entries past jb are zero
     for (int kk = 0; kk < kend; kk += 4) {
       const int kx = kk + l4;
-      const double d = dk[kx];
-      double2 a[2], bq[2];
+      const RT d = dk[kx];
+      CT a[2], bq[2];
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
-        const double2 v = Li[r0 + 16 * p + l16][kx];
-        a[p] = make_double2(v.x * d, v.y * d);
+        const CT v = Li[r0 + 16 * p + l16][kx];
+        a[p] = mk<CT>(v.x * d, v.y * d);
       }
 #pragma unroll
       for (int q = 0; q < 2; ++q) bq[q] = Lj[q0 + 16 * q + l16][kx];
@@ -434,10 +487,10 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
           if (!need[p][q]) continue;
-          cr[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p].x, bq[q].x, cr[p][q], 0, 0, 0);
-          cr[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p].y, bq[q].y, cr[p][q], 0, 0, 0);
-          ci[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p].y, bq[q].x, ci[p][q], 0, 0, 0);
-          ci[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[p].x, bq[q].y, ci[p][q], 0, 0, 0);
+          cr[p][q] = mfma16(a[p].x, bq[q].x, cr[p][q]);
+          cr[p][q] = mfma16(a[p].y, bq[q].y, cr[p][q]);
+          ci[p][q] = mfma16(a[p].y, bq[q].x, ci[p][q]);
+          ci[p][q] = mfma16(-a[p].x, bq[q].y, ci[p][q]);
         }
     }
   }
@@ -449,10 +502,10 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
     for (int p = 0; p < 2; ++p)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int gi = t.i0 + r0 + 16 * p + 4 * j + l4;
+        const int gi = t.i0 + r0 + 16 * p + mfma_row(RT(0), j, l4);
         if (gi >= m || gi < gj) continue;
-        double2* dst = front_at(Pn, Un, m, w, r, gi, gj);
-        *dst = make_double2(dst->x - cr[p][q][j], dst->y - ci[p][q][j]);
+        CT* dst = front_at(Pn, Un, m, w, r, gi, gj);
+        *dst = mk<CT>(dst->x - cr[p][q][j], dst->y - ci[p][q][j]);
       }
   }
 }
@@ -464,8 +517,8 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
 // ---------------------------------------------------------------------------
 struct InvArgs {
   const DevSn* sn;
-  double2* L;
-  double2* S;
+  void* L;  // double2 or float2 (the kernels' CT)
+  void* S;
   long long PE, SE;
   const int* active;  // per problem (null: all)
 };
@@ -477,18 +530,18 @@ struct InvArgs {
 //   X21 = -X22 (L21 X11)
 // in two passes (T = L21 X11 into L21's place, then X21), 256 threads, outputs
 // with the column fastest (row reads broadcast, column reads contiguous).
-template <int h, int nblk>
-__device__ __forceinline__ void inv_combine(double2 (*A)[64]) {
+template <int h, int nblk, class CT>
+__device__ __forceinline__ void inv_combine(CT (*A)[64]) {
   constexpr int nout = nblk * h * h, per = (nout + 255) / 256;
   static_assert(nout % 256 == 0, "whole passes of 256 outputs");
   const int tid = threadIdx.x;
-  double2 v[per];
+  CT v[per];
   // T[i][j] = L21[i][j] + sum_{k>j} L21[i][k] X11[k][j]
 #pragma unroll
   for (int u = 0; u < per; ++u) {
     const int e = tid + 256 * u;
     const int j = e % h, i = (e / h) % h, o = 2 * h * (e / (h * h));
-    double2 t = A[o + h + i][o + j];
+    CT t = A[o + h + i][o + j];
     for (int k = j + 1; k < h; ++k) cmac(t, A[o + h + i][o + k], A[o + k][o + j]);
     v[u] = t;
   }
@@ -505,9 +558,9 @@ __device__ __forceinline__ void inv_combine(double2 (*A)[64]) {
   for (int u = 0; u < per; ++u) {
     const int e = tid + 256 * u;
     const int j = e % h, i = (e / h) % h, o = 2 * h * (e / (h * h));
-    double2 t = A[o + h + i][o + j];
+    CT t = A[o + h + i][o + j];
     for (int k = 0; k < i; ++k) cmac(t, A[o + h + i][o + h + k], A[o + h + k][o + j]);
-    v[u] = make_double2(-t.x, -t.y);
+    v[u] = mk<CT>(-t.x, -t.y);
   }
   __syncthreads();
 #pragma unroll
@@ -523,6 +576,7 @@ __device__ __forceinline__ void inv_combine(double2 (*A)[64]) {
 // rows; the padding is an identity block), in place in LDS: the four 16 x 16
 // diagonal blocks by the row recurrence, one wave each (lanes 0..15 own the
 // columns), then two levels of inv_combine; written to the scratch X columns.
+template <class CT>
 __global__ __launch_bounds__(256) void k_inv_diag(InvArgs ia, const int* __restrict__ tasks, int t0, int J) {
   const int id = tasks[t0 + blockIdx.x];
   const int b = blockIdx.y;
@@ -530,13 +584,13 @@ __global__ __launch_bounds__(256) void k_inv_diag(InvArgs ia, const int* __restr
   const DevSn s = ia.sn[id];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int j0 = 64 * J, nb = min(64, w - j0);
-  const double2* Pn = ia.L + b * ia.PE + s.poff;
-  double2* Sc = ia.S + b * ia.SE + s.soff;  // w x 128, column-major (ld = w): X | Z
-  __shared__ double2 A[64][64];
+  const CT* Pn = static_cast<const CT*>(ia.L) + b * ia.PE + s.poff;
+  CT* Sc = static_cast<CT*>(ia.S) + b * ia.SE + s.soff;  // w x 128, column-major (ld = w): X | Z
+  __shared__ CT A[64][64];
   const int tid = threadIdx.x;
   for (int idx = tid; idx < 64 * 64; idx += 256) {
     const int i = idx / 64, t = idx % 64;
-    A[i][t] = (t < i && i < nb) ? Pn[(j0 + i) + (long long)(j0 + t) * m] : make_double2(0.0, 0.0);
+    A[i][t] = (t < i && i < nb) ? Pn[(j0 + i) + (long long)(j0 + t) * m] : mk<CT>(0.0, 0.0);
   }
   __syncthreads();
   {
@@ -544,9 +598,9 @@ __global__ __launch_bounds__(256) void k_inv_diag(InvArgs ia, const int* __restr
     // k < i already hold X
     const int o = 16 * (tid >> 6), t = tid & 63;
     for (int i = 1; i < 16; ++i) {
-      double2 v = make_double2(0.0, 0.0);
+      CT v = mk<CT>(0.0, 0.0);
       if (t < i) {
-        v = make_double2(-A[o + i][o + t].x, -A[o + i][o + t].y);
+        v = mk<CT>(-A[o + i][o + t].x, -A[o + i][o + t].y);
         for (int k = t + 1; k < i; ++k) v = csub(v, cmul(A[o + i][o + k], A[o + k][o + t]));
       }
       __syncthreads();
@@ -555,33 +609,33 @@ __global__ __launch_bounds__(256) void k_inv_diag(InvArgs ia, const int* __restr
     }
   }
   __syncthreads();
-  inv_combine<16, 2>(A);
-  inv_combine<32, 1>(A);
+  inv_combine<16, 2, CT>(A);
+  inv_combine<32, 1, CT>(A);
   for (int idx = tid; idx < nb * nb; idx += 256) {
     const int i = idx % nb, t = idx / nb;
-    Sc[(j0 + i) + (long long)t * w] = (t < i) ? A[i][t] : make_double2(t == i ? 1.0 : 0.0, 0.0);
+    Sc[(j0 + i) + (long long)t * w] = (t < i) ? A[i][t] : mk<CT>(t == i ? 1.0 : 0.0, 0.0);
   }
 }
 
 // 64x64 tile product acc += A[rows, k] B[k, cols] over k in [k0, k1); A and B
 // are read through callbacks so the same loop serves both inversion products.
-template <class FA, class FB>
-__device__ __forceinline__ void tile_gemm(double2 (&acc)[4][4], int k0, int k1, FA fa, FB fb) {
-  __shared__ double2 As[64][17];
-  __shared__ double2 Bs[16][65];
+template <class CT, class FA, class FB>
+__device__ __forceinline__ void tile_gemm(CT (&acc)[4][4], int k0, int k1, FA fa, FB fb) {
+  __shared__ CT As[64][17];
+  __shared__ CT Bs[16][65];
   const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
   for (int kc = k0; kc < k1; kc += 16) {
     const int kn = min(16, k1 - kc);
     __syncthreads();
     for (int idx = tid; idx < 64 * 16; idx += 256) {
       const int r = idx % 64, kk = idx / 64;
-      As[r][kk] = kk < kn ? fa(r, kc + kk) : make_double2(0.0, 0.0);
+      As[r][kk] = kk < kn ? fa(r, kc + kk) : mk<CT>(0.0, 0.0);
       const int c = idx % 64, kr = idx / 64;
-      Bs[kr][c] = kr < kn ? fb(kc + kr, c) : make_double2(0.0, 0.0);
+      Bs[kr][c] = kr < kn ? fb(kc + kr, c) : mk<CT>(0.0, 0.0);
     }
     __syncthreads();
     for (int kk = 0; kk < kn; ++kk) {
-      double2 a[4], bb[4];
+      CT a[4], bb[4];
       for (int p = 0; p < 4; ++p) a[p] = As[tx + 16 * p][kk];
       for (int q = 0; q < 4; ++q) bb[q] = Bs[kk][ty + 16 * q];
       for (int p = 0; p < 4; ++p)
@@ -591,6 +645,7 @@ __device__ __forceinline__ void tile_gemm(double2 (&acc)[4][4], int k0, int k1, 
 }
 
 // Z_IJ = L_IJ X_JJ (scratch columns 64..127), I > J
+template <class CT>
 __global__ __launch_bounds__(256) void k_inv_z(InvArgs ia, const TileTask* __restrict__ tasks, int t0) {
   const TileTask tk = tasks[t0 + blockIdx.x];
   const int b = blockIdx.y;
@@ -598,14 +653,14 @@ __global__ __launch_bounds__(256) void k_inv_z(InvArgs ia, const TileTask* __res
   const DevSn s = ia.sn[tk.sn];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int i0 = 64 * tk.i0, ni = min(64, w - i0), jj0 = 64 * tk.j0, nj = min(64, w - jj0);
-  const double2* Pn = ia.L + b * ia.PE + s.poff;
-  double2* Sc = ia.S + b * ia.SE + s.soff;
-  double2 acc[4][4];
+  const CT* Pn = static_cast<const CT*>(ia.L) + b * ia.PE + s.poff;
+  CT* Sc = static_cast<CT*>(ia.S) + b * ia.SE + s.soff;
+  CT acc[4][4];
   for (int p = 0; p < 4; ++p)
-    for (int q = 0; q < 4; ++q) acc[p][q] = make_double2(0.0, 0.0);
+    for (int q = 0; q < 4; ++q) acc[p][q] = mk<CT>(0.0, 0.0);
   tile_gemm(acc, 0, nj,
-            [&](int r, int k) { return r < ni ? Pn[(i0 + r) + (long long)(jj0 + k) * m] : make_double2(0.0, 0.0); },
-            [&](int k, int c) { return c < nj ? Sc[(jj0 + k) + (long long)c * w] : make_double2(0.0, 0.0); });
+            [&](int r, int k) { return r < ni ? Pn[(i0 + r) + (long long)(jj0 + k) * m] : mk<CT>(0.0, 0.0); },
+            [&](int k, int c) { return c < nj ? Sc[(jj0 + k) + (long long)c * w] : mk<CT>(0.0, 0.0); });
   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
   for (int p = 0; p < 4; ++p)
     for (int q = 0; q < 4; ++q) {
@@ -615,6 +670,7 @@ __global__ __launch_bounds__(256) void k_inv_z(InvArgs ia, const TileTask* __res
 }
 
 // X_IJ = -sum_{J<K<=I} X_IK Z_KJ (scratch columns 0..63), I > J
+template <class CT>
 __global__ __launch_bounds__(256) void k_inv_tile(InvArgs ia, const TileTask* __restrict__ tasks, int t0) {
   const TileTask tk = tasks[t0 + blockIdx.x];
   const int b = blockIdx.y;
@@ -622,28 +678,29 @@ __global__ __launch_bounds__(256) void k_inv_tile(InvArgs ia, const TileTask* __
   const DevSn s = ia.sn[tk.sn];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int i0 = 64 * tk.i0, ni = min(64, w - i0), jj0 = 64 * tk.j0, nj = min(64, w - jj0);
-  const double2* Pn = ia.L + b * ia.PE + s.poff;
-  double2* Sc = ia.S + b * ia.SE + s.soff;
-  double2 acc[4][4];
+  const CT* Pn = static_cast<const CT*>(ia.L) + b * ia.PE + s.poff;
+  CT* Sc = static_cast<CT*>(ia.S) + b * ia.SE + s.soff;
+  CT acc[4][4];
   for (int p = 0; p < 4; ++p)
-    for (int q = 0; q < 4; ++q) acc[p][q] = make_double2(0.0, 0.0);
+    for (int q = 0; q < 4; ++q) acc[p][q] = mk<CT>(0.0, 0.0);
   tile_gemm(acc, jj0 + 64, i0 + ni,
             [&](int r, int k) {
-              if (r >= ni) return make_double2(0.0, 0.0);
+              if (r >= ni) return mk<CT>(0.0, 0.0);
               const int gi = i0 + r;
               if (k < gi) return Pn[gi + (long long)k * m];  // X_IK, already inverted
-              return make_double2(k == gi ? 1.0 : 0.0, 0.0);
+              return mk<CT>(k == gi ? 1.0 : 0.0, 0.0);
             },
-            [&](int k, int c) { return c < nj ? Sc[k + (long long)(64 + c) * w] : make_double2(0.0, 0.0); });
+            [&](int k, int c) { return c < nj ? Sc[k + (long long)(64 + c) * w] : mk<CT>(0.0, 0.0); });
   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
   for (int p = 0; p < 4; ++p)
     for (int q = 0; q < 4; ++q) {
       const int r = tx + 16 * p, c = ty + 16 * q;
-      if (r < ni && c < nj) Sc[(i0 + r) + (long long)c * w] = make_double2(-acc[p][q].x, -acc[p][q].y);
+      if (r < ni && c < nj) Sc[(i0 + r) + (long long)c * w] = mk<CT>(-acc[p][q].x, -acc[p][q].y);
     }
 }
 
 // write the inverted block column J back into the panel (diagonal keeps D)
+template <class CT>
 __global__ __launch_bounds__(256) void k_inv_store(InvArgs ia, const int* __restrict__ tasks, int t0, int J) {
   const int id = tasks[t0 + blockIdx.x];
   const int b = blockIdx.y;
@@ -651,8 +708,8 @@ __global__ __launch_bounds__(256) void k_inv_store(InvArgs ia, const int* __rest
   const DevSn s = ia.sn[id];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int j0 = 64 * J, nj = min(64, w - j0);
-  double2* Pn = ia.L + b * ia.PE + s.poff;
-  const double2* Sc = ia.S + b * ia.SE + s.soff;
+  CT* Pn = static_cast<CT*>(ia.L) + b * ia.PE + s.poff;
+  const CT* Sc = static_cast<CT*>(ia.S) + b * ia.SE + s.soff;
   // wave v copies columns v, v + 4, ...: four columns in flight per block
   for (int c = threadIdx.x >> 6; c < nj; c += 4)
     for (int i = j0 + c + 1 + (threadIdx.x & 63); i < w; i += 64)
@@ -671,7 +728,7 @@ struct SolveArgs {
   const int* struct_nodes;
   const int* relmap;
   const int* invmap;  // per front and child: front node position -> child update node, or -1
-  const double2* L;
+  const void* L;      // the factor: double2 or float2 (the kernels' LT); vectors are double2
   double2* T;
   long long PE, TE;
   double2* X;
@@ -727,14 +784,14 @@ __global__ __launch_bounds__(256) void k_fwd_asm(SolveArgs sa, const int* __rest
 }
 
 // y[i] = x[i] + sum_{j<i} Linv[i, j] x[j]  -> pivot temp (rows [r0, r1) of the pivot block)
-template <int PM>
+template <int PM, class LT>
 __global__ __launch_bounds__(64) void k_fwd_inv(SolveArgs sa, const RowTask* __restrict__ tasks, int t0) {
   const int b = blockIdx.y;
   if (sa.active && !sa.active[b]) return;
   const RowTask t = tasks[t0 + blockIdx.x];
   const DevSn s = sa.sn[t.sn];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr, p = sa.p;
-  const double2* Pn = sa.L + b * sa.PE + s.poff;
+  const LT* Pn = static_cast<const LT*>(sa.L) + b * sa.PE + s.poff;
   const double2* x = sa.X + b * sa.ldx + 3LL * s.col0;
   __shared__ double2 xs[64][PM];
   const int i = t.r0 + threadIdx.x;
@@ -750,31 +807,31 @@ __global__ __launch_bounds__(64) void k_fwd_inv(SolveArgs sa, const RowTask* __r
     __syncthreads();
     if (i < t.r1) {
       if (j0 + nj <= t.r0) {  // strictly below this task's diagonal: every lane takes all nj
-        const double2* lp = Pn + i + (long long)j0 * m;
+        const LT* lp = Pn + i + (long long)j0 * m;
         int jj = 0;
         for (; jj + 4 <= nj; jj += 4) {  // four loads in flight; entries past p are zero
           double2 l[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) l[u] = lp[(long long)(jj + u) * m];
+          for (int u = 0; u < 4; ++u) l[u] = ldc(lp[(long long)(jj + u) * m]);
 #pragma unroll
           for (int u = 0; u < 4; ++u)
 #pragma unroll
             for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], xs[jj + u][c]);
         }
         for (; jj < nj; ++jj) {
-          const double2 l = lp[(long long)jj * m];
+          const double2 l = ldc(lp[(long long)jj * m]);
 #pragma unroll
           for (int c = 0; c < PM; ++c) cmac(acc[c], l, xs[jj][c]);
         }
       } else {  // the diagonal chunk: the same pass, entries at or above the diagonal not loaded
-        const double2* lp = Pn + i + (long long)j0 * m;
+        const LT* lp = Pn + i + (long long)j0 * m;
         int jj = 0;
         for (; jj + 4 <= nj; jj += 4) {
           double2 l[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const int j = j0 + jj + u;
-            l[u] = j < i ? lp[(long long)(jj + u) * m] : make_double2(j == i ? 1.0 : 0.0, 0.0);
+            l[u] = j < i ? ldc(lp[(long long)(jj + u) * m]) : make_double2(j == i ? 1.0 : 0.0, 0.0);
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u)
@@ -783,7 +840,7 @@ __global__ __launch_bounds__(64) void k_fwd_inv(SolveArgs sa, const RowTask* __r
         }
         for (; jj < nj; ++jj) {
           const int j = j0 + jj;
-          const double2 l = j < i ? lp[(long long)jj * m] : make_double2(j == i ? 1.0 : 0.0, 0.0);
+          const double2 l = j < i ? ldc(lp[(long long)jj * m]) : make_double2(j == i ? 1.0 : 0.0, 0.0);
 #pragma unroll
           for (int c = 0; c < PM; ++c) cmac(acc[c], l, xs[jj][c]);
         }
@@ -798,14 +855,14 @@ __global__ __launch_bounds__(64) void k_fwd_inv(SolveArgs sa, const RowTask* __r
 // k_fwd_inv with the column chunks of a task dealt round-robin to four waves
 // (their partial sums added in wave order): for the upper tree levels, whose
 // few tasks with long rows leave most SIMDs idle under one wave per task
-template <int PM>
+template <int PM, class LT>
 __global__ __launch_bounds__(256) void k_fwd_inv4(SolveArgs sa, const RowTask* __restrict__ tasks, int t0) {
   const int b = blockIdx.y;
   if (sa.active && !sa.active[b]) return;
   const RowTask t = tasks[t0 + blockIdx.x];
   const DevSn s = sa.sn[t.sn];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr, p = sa.p;
-  const double2* Pn = sa.L + b * sa.PE + s.poff;
+  const LT* Pn = static_cast<const LT*>(sa.L) + b * sa.PE + s.poff;
   const double2* x = sa.X + b * sa.ldx + 3LL * s.col0;
   __shared__ double2 xs[4][64][PM];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -823,20 +880,20 @@ __global__ __launch_bounds__(256) void k_fwd_inv4(SolveArgs sa, const RowTask* _
     }
     __syncthreads();
     if (i < t.r1 && nj > 0) {
-      const double2* lp = Pn + i + (long long)j0 * m;
+      const LT* lp = Pn + i + (long long)j0 * m;
       if (j0 + nj <= t.r0) {  // strictly below this task's diagonal
         int jj = 0;
         for (; jj + 4 <= nj; jj += 4) {
           double2 l[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) l[u] = lp[(long long)(jj + u) * m];
+          for (int u = 0; u < 4; ++u) l[u] = ldc(lp[(long long)(jj + u) * m]);
 #pragma unroll
           for (int u = 0; u < 4; ++u)
 #pragma unroll
             for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], xs[wv][jj + u][c]);
         }
         for (; jj < nj; ++jj) {
-          const double2 l = lp[(long long)jj * m];
+          const double2 l = ldc(lp[(long long)jj * m]);
 #pragma unroll
           for (int c = 0; c < PM; ++c) cmac(acc[c], l, xs[wv][jj][c]);
         }
@@ -847,7 +904,7 @@ __global__ __launch_bounds__(256) void k_fwd_inv4(SolveArgs sa, const RowTask* _
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const int j = j0 + jj + u;
-            l[u] = j < i ? lp[(long long)(jj + u) * m] : make_double2(j == i ? 1.0 : 0.0, 0.0);
+            l[u] = j < i ? ldc(lp[(long long)(jj + u) * m]) : make_double2(j == i ? 1.0 : 0.0, 0.0);
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u)
@@ -856,7 +913,7 @@ __global__ __launch_bounds__(256) void k_fwd_inv4(SolveArgs sa, const RowTask* _
         }
         for (; jj < nj; ++jj) {
           const int j = j0 + jj;
-          const double2 l = j < i ? lp[(long long)jj * m] : make_double2(j == i ? 1.0 : 0.0, 0.0);
+          const double2 l = j < i ? ldc(lp[(long long)jj * m]) : make_double2(j == i ? 1.0 : 0.0, 0.0);
 #pragma unroll
           for (int c = 0; c < PM; ++c) cmac(acc[c], l, xs[wv][jj][c]);
         }
@@ -876,14 +933,14 @@ __global__ __launch_bounds__(256) void k_fwd_inv4(SolveArgs sa, const RowTask* _
 }
 
 // rows < w: write the pivot solution back into x; rows >= w: t -= L21 y
-template <int PM>
+template <int PM, class LT>
 __global__ __launch_bounds__(256) void k_fwd_upd(SolveArgs sa, const RowTask* __restrict__ tasks, int t0) {
   const int b = blockIdx.y;
   if (sa.active && !sa.active[b]) return;
   const RowTask t = tasks[t0 + blockIdx.x];
   const DevSn s = sa.sn[t.sn];
   const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r, p = sa.p;
-  const double2* Pn = sa.L + b * sa.PE + s.poff;
+  const LT* Pn = static_cast<const LT*>(sa.L) + b * sa.PE + s.poff;
   constexpr int CH = 2048 / PM;
   __shared__ double2 ys[CH][PM];
   const int i = t.r0 + threadIdx.x;
@@ -900,19 +957,19 @@ __global__ __launch_bounds__(256) void k_fwd_upd(SolveArgs sa, const RowTask* __
       }
       __syncthreads();
       if (upd) {
-        const double2* lp = Pn + i + (long long)j0 * m;
+        const LT* lp = Pn + i + (long long)j0 * m;
         int jj = 0;
         for (; jj + 8 <= nj; jj += 8) {  // eight loads in flight; entries past p are zero
           double2 l[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) l[u] = lp[(long long)(jj + u) * m];
+          for (int u = 0; u < 8; ++u) l[u] = ldc(lp[(long long)(jj + u) * m]);
 #pragma unroll
           for (int u = 0; u < 8; ++u)
 #pragma unroll
             for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], ys[jj + u][c]);
         }
         for (; jj < nj; ++jj) {
-          const double2 l = lp[(long long)jj * m];
+          const double2 l = ldc(lp[(long long)jj * m]);
 #pragma unroll
           for (int c = 0; c < PM; ++c) cmac(acc[c], l, ys[jj][c]);
         }
@@ -942,14 +999,14 @@ __global__ __launch_bounds__(256) void k_fwd_upd(SolveArgs sa, const RowTask* __
 // k_fwd_upd with the pivot columns dealt in 64-column chunks round-robin to NG
 // groups of 256 threads (their partial sums added in group order), for the upper
 // tree levels, whose few long-row tasks leave most SIMDs idle
-template <int PM, int NG>
+template <int PM, int NG, class LT>
 __global__ __launch_bounds__(256 * NG) void k_fwd_upd_split(SolveArgs sa, const RowTask* __restrict__ tasks, int t0) {
   const int b = blockIdx.y;
   if (sa.active && !sa.active[b]) return;
   const RowTask t = tasks[t0 + blockIdx.x];
   const DevSn s = sa.sn[t.sn];
   const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r, p = sa.p;
-  const double2* Pn = sa.L + b * sa.PE + s.poff;
+  const LT* Pn = static_cast<const LT*>(sa.L) + b * sa.PE + s.poff;
   constexpr int CH = 64;
   constexpr int SH = NG * CH * PM > 256 * PM ? NG * CH * PM : 256 * PM;
   __shared__ double2 sh[SH];  // NG tiles of CH x PM solution entries, then the partial sums
@@ -971,19 +1028,19 @@ __global__ __launch_bounds__(256 * NG) void k_fwd_upd_split(SolveArgs sa, const 
       }
       __syncthreads();
       if (upd && nj > 0) {
-        const double2* lp = Pn + i + (long long)j0 * m;
+        const LT* lp = Pn + i + (long long)j0 * m;
         int jj = 0;
         for (; jj + 8 <= nj; jj += 8) {
           double2 l[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) l[u] = lp[(long long)(jj + u) * m];
+          for (int u = 0; u < 8; ++u) l[u] = ldc(lp[(long long)(jj + u) * m]);
 #pragma unroll
           for (int u = 0; u < 8; ++u)
 #pragma unroll
             for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], ys[(jj + u) * PM + c]);
         }
         for (; jj < nj; ++jj) {
-          const double2 l = lp[(long long)jj * m];
+          const double2 l = ldc(lp[(long long)jj * m]);
 #pragma unroll
           for (int c = 0; c < PM; ++c) cmac(acc[c], l, ys[jj * PM + c]);
         }
@@ -1041,7 +1098,7 @@ __global__ __launch_bounds__(256) void k_bwd_gather(SolveArgs sa, const int* __r
 // loads; thread (j, part) owns column j and right-hand sides part, part + NP,
 // ... (PM >= 4), or rows part, part + 4, ... of every tile with a reduction of
 // the four partial sums at the end (PM = 1).  Four waves per block.
-template <int PM>
+template <int PM, class LT>
 __global__ __launch_bounds__(256) void k_bwd_tile(SolveArgs sa, const RowTask* __restrict__ tasks, int t0,
                                                   int stage) {
   constexpr int NP = 4;
@@ -1054,7 +1111,7 @@ __global__ __launch_bounds__(256) void k_bwd_tile(SolveArgs sa, const RowTask* _
   const DevSn s = sa.sn[t.sn];
   const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r, p = sa.p;
   const int j0 = t.r0, nj = t.r1 - t.r0;
-  const double2* Pn = sa.L + b * sa.PE + s.poff;
+  const LT* Pn = static_cast<const LT*>(sa.L) + b * sa.PE + s.poff;
   __shared__ double2 At[TR][64 + 1];
   __shared__ double2 vs[TR][PM];
   const int tid = threadIdx.x, jj = tid % 64, part = tid / 64;
@@ -1072,7 +1129,7 @@ __global__ __launch_bounds__(256) void k_bwd_tile(SolveArgs sa, const RowTask* _
 #pragma unroll
     for (int e = 0; e < APER; ++e) {
       const int idx = tid + 256 * e, ii = idx % TR, jc = idx / TR;
-      ra[e] = (ii < ni && jc < nj) ? Pn[(i0 + ii) + (long long)(j0 + jc) * m] : make_double2(0.0, 0.0);
+      ra[e] = (ii < ni && jc < nj) ? ldc(Pn[(i0 + ii) + (long long)(j0 + jc) * m]) : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int e = 0; e < VPER; ++e) {
@@ -1124,7 +1181,7 @@ __global__ __launch_bounds__(256) void k_bwd_tile(SolveArgs sa, const RowTask* _
   if (jj >= nj) return;
   const double2* x = sa.X + b * sa.ldx + 3LL * s.col0;
   if (stage == 0) {
-    const double d = Pn[j + (long long)j * m].x;
+    const double d = ldc(Pn[j + (long long)j * m]).x;
 #pragma unroll
     for (int u = 0; u < CPT; ++u) {
       const int c = part + NP * u;
@@ -1171,6 +1228,38 @@ static FactorArgs factor_args(const Mesh& M, const Plan& P, const OpArrays& ops,
   return fa;
 }
 
+template <class CT>
+static void factor_levels(Context* ctx, const Plan& P, const FactorArgs& fa, const InvArgs& ia, int B) {
+  for (const LevelPlan& L : P.levels) {
+    if (L.nasm)
+      hipLaunchKernelGGL(k_front_assemble<CT>, dim3(L.nasm, B), dim3(256), 0, ctx->stream, fa,
+                         P.d_asm.as<AsmTask>(), L.asm0);
+    for (const FactorStep& st : L.steps) {
+      if (st.ndiag)
+        hipLaunchKernelGGL(k_diag_ldl<CT>, dim3(st.ndiag, B), dim3(256), 0, ctx->stream, fa,
+                           P.d_diag.as<int>(), st.diag0, st.kb);
+      if (st.ntrsm)
+        hipLaunchKernelGGL(k_panel_trsm<CT>, dim3(st.ntrsm, B), dim3(256), 0, ctx->stream, fa,
+                           P.d_row.as<RowTask>(), st.trsm0, st.kb);
+      if (st.ntile)
+        hipLaunchKernelGGL(k_schur_tile<CT>, dim3(st.ntile, B), dim3(256), 0, ctx->stream, fa,
+                           P.d_tile.as<TileTask>(), st.tile0, st.kb);
+    }
+    for (const InvStep& iv : L.inv) {
+      hipLaunchKernelGGL(k_inv_diag<CT>, dim3(iv.ndiag, B), dim3(256), 0, ctx->stream, ia, P.d_inv_diag.as<int>(),
+                         iv.diag0, iv.J);
+      if (iv.ntile) {
+        hipLaunchKernelGGL(k_inv_z<CT>, dim3(iv.ntile, B), dim3(256), 0, ctx->stream, ia,
+                           P.d_inv_tiles.as<TileTask>(), iv.tile0);
+        hipLaunchKernelGGL(k_inv_tile<CT>, dim3(iv.ntile, B), dim3(256), 0, ctx->stream, ia,
+                           P.d_inv_tiles.as<TileTask>(), iv.tile0);
+      }
+      hipLaunchKernelGGL(k_inv_store<CT>, dim3(iv.ndiag, B), dim3(256), 0, ctx->stream, ia, P.d_inv_diag.as<int>(),
+                         iv.diag0, iv.J);
+    }
+  }
+}
+
 void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& ops,
                   const BatchBufs& bb, const int* active, int nactive, const OpArrays* ops_b) {
   FactorArgs fa = factor_args(M, P, ops, bb);
@@ -1180,35 +1269,9 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
   const int B = bb.B;
   if (bb.pivinfo)
     hipLaunchKernelGGL(k_pivinfo_init, dim3((B + 63) / 64), dim3(64), 0, ctx->stream, bb.pivinfo, active, B);
-  KTimer t(ctx, "factor", 0, M.S.flops * (nactive >= 0 ? nactive : B));
-  for (const LevelPlan& L : P.levels) {
-    if (L.nasm)
-      hipLaunchKernelGGL(k_front_assemble, dim3(L.nasm, B), dim3(256), 0, ctx->stream, fa,
-                         P.d_asm.as<AsmTask>(), L.asm0);
-    for (const FactorStep& st : L.steps) {
-      if (st.ndiag)
-        hipLaunchKernelGGL(k_diag_ldl, dim3(st.ndiag, B), dim3(256), 0, ctx->stream, fa,
-                           P.d_diag.as<int>(), st.diag0, st.kb);
-      if (st.ntrsm)
-        hipLaunchKernelGGL(k_panel_trsm, dim3(st.ntrsm, B), dim3(256), 0, ctx->stream, fa,
-                           P.d_row.as<RowTask>(), st.trsm0, st.kb);
-      if (st.ntile)
-        hipLaunchKernelGGL(k_schur_tile, dim3(st.ntile, B), dim3(256), 0, ctx->stream, fa,
-                           P.d_tile.as<TileTask>(), st.tile0, st.kb);
-    }
-    for (const InvStep& iv : L.inv) {
-      hipLaunchKernelGGL(k_inv_diag, dim3(iv.ndiag, B), dim3(256), 0, ctx->stream, ia, P.d_inv_diag.as<int>(),
-                         iv.diag0, iv.J);
-      if (iv.ntile) {
-        hipLaunchKernelGGL(k_inv_z, dim3(iv.ntile, B), dim3(256), 0, ctx->stream, ia,
-                           P.d_inv_tiles.as<TileTask>(), iv.tile0);
-        hipLaunchKernelGGL(k_inv_tile, dim3(iv.ntile, B), dim3(256), 0, ctx->stream, ia,
-                           P.d_inv_tiles.as<TileTask>(), iv.tile0);
-      }
-      hipLaunchKernelGGL(k_inv_store, dim3(iv.ndiag, B), dim3(256), 0, ctx->stream, ia, P.d_inv_diag.as<int>(),
-                         iv.diag0, iv.J);
-    }
-  }
+  KTimer t(ctx, bb.fp32 ? "factor_fp32" : "factor", 0, M.S.flops * (nactive >= 0 ? nactive : B));
+  if (bb.fp32) factor_levels<float2>(ctx, P, fa, ia, B);
+  else factor_levels<double2>(ctx, P, fa, ia, B);
   TM_HIP(hipGetLastError());
 }
 
@@ -1220,7 +1283,7 @@ constexpr int kFwdInvSplit = 56;
 // tasks slower, up to 383 -> 617 us, and 3 tasks 6 -> 10 us)
 constexpr int kFwdUpdSplitLo = 8, kFwdUpdSplitHi = 24;
 
-template <int PM>
+template <int PM, class LT>
 static void trisolve_levels(Context* ctx, const Plan& P, const SolveArgs& sa, int B) {
   const int* lsn = P.d_level_sn.as<int>();
   for (const LevelPlan& L : P.levels) {
@@ -1228,26 +1291,26 @@ static void trisolve_levels(Context* ctx, const Plan& P, const SolveArgs& sa, in
     // levels with few pivot-row tasks (the upper tree) split each task's columns
     // over four waves; the choice depends on the level only, never on the batch
     if (L.nfinv > kFwdInvSplit)
-      hipLaunchKernelGGL(k_fwd_inv<PM>, dim3(L.nfinv, B), dim3(64), 0, ctx->stream, sa, P.d_finv.as<RowTask>(),
+      hipLaunchKernelGGL((k_fwd_inv<PM, LT>), dim3(L.nfinv, B), dim3(64), 0, ctx->stream, sa, P.d_finv.as<RowTask>(),
                          L.finv0);
     else if (L.nfinv)
-      hipLaunchKernelGGL(k_fwd_inv4<PM>, dim3(L.nfinv, B), dim3(256), 0, ctx->stream, sa, P.d_finv.as<RowTask>(),
+      hipLaunchKernelGGL((k_fwd_inv4<PM, LT>), dim3(L.nfinv, B), dim3(256), 0, ctx->stream, sa, P.d_finv.as<RowTask>(),
                          L.finv0);
     constexpr int NG = PM <= 8 ? 4 : 2;
     if (L.nfup > kFwdUpdSplitHi || (L.nfup && L.nfup < kFwdUpdSplitLo))
-      hipLaunchKernelGGL(k_fwd_upd<PM>, dim3(L.nfup, B), dim3(256), 0, ctx->stream, sa, P.d_fup.as<RowTask>(),
+      hipLaunchKernelGGL((k_fwd_upd<PM, LT>), dim3(L.nfup, B), dim3(256), 0, ctx->stream, sa, P.d_fup.as<RowTask>(),
                          L.fup0);
     else if (L.nfup)
-      hipLaunchKernelGGL((k_fwd_upd_split<PM, NG>), dim3(L.nfup, B), dim3(256 * NG), 0, ctx->stream, sa,
+      hipLaunchKernelGGL((k_fwd_upd_split<PM, NG, LT>), dim3(L.nfup, B), dim3(256 * NG), 0, ctx->stream, sa,
                          P.d_fup.as<RowTask>(), L.fup0);
   }
   for (int h = static_cast<int>(P.levels.size()) - 1; h >= 0; --h) {
     const LevelPlan& L = P.levels[h];
     if (!L.nbcol) continue;
     hipLaunchKernelGGL(k_bwd_gather, dim3(L.nsn, B), dim3(256), 0, ctx->stream, sa, lsn, L.sn0);
-    hipLaunchKernelGGL(k_bwd_tile<PM>, dim3(L.nbcol, B), dim3(256), 0, ctx->stream, sa, P.d_bcol.as<RowTask>(),
+    hipLaunchKernelGGL((k_bwd_tile<PM, LT>), dim3(L.nbcol, B), dim3(256), 0, ctx->stream, sa, P.d_bcol.as<RowTask>(),
                        L.bcol0, 0);
-    hipLaunchKernelGGL(k_bwd_tile<PM>, dim3(L.nbcol, B), dim3(256), 0, ctx->stream, sa, P.d_bcol.as<RowTask>(),
+    hipLaunchKernelGGL((k_bwd_tile<PM, LT>), dim3(L.nbcol, B), dim3(256), 0, ctx->stream, sa, P.d_bcol.as<RowTask>(),
                        L.bcol0, 1);
   }
 }
@@ -1274,11 +1337,20 @@ void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs&
   const double vec_bytes = 5.0 * 3 * M.S.nf * 16.0;
   const double l_bytes = P.trisolve_bytes_c16 - vec_bytes;
   const int nb = nactive >= 0 ? std::min(nactive, B) : B;  // problems that do the work
-  KTimer t(ctx, "trisolve", (l_bytes + p * vec_bytes) * nb, 8.0 * p * nb * l_bytes / 20.0);
-  if (p == 1) trisolve_levels<1>(ctx, P, sa, B);
-  else if (p <= 4) trisolve_levels<4>(ctx, P, sa, B);
-  else if (p <= 8) trisolve_levels<8>(ctx, P, sa, B);
-  else trisolve_levels<16>(ctx, P, sa, B);
+  // a complex-FP32 factor entry is 8 + 4 B instead of 16 + 4
+  const double lb = bb.fp32 ? l_bytes * 12.0 / 20.0 : l_bytes;
+  KTimer t(ctx, bb.fp32 ? "trisolve_fp32" : "trisolve", (lb + p * vec_bytes) * nb, 8.0 * p * nb * l_bytes / 20.0);
+  if (bb.fp32) {
+    if (p == 1) trisolve_levels<1, float2>(ctx, P, sa, B);
+    else if (p <= 4) trisolve_levels<4, float2>(ctx, P, sa, B);
+    else if (p <= 8) trisolve_levels<8, float2>(ctx, P, sa, B);
+    else trisolve_levels<16, float2>(ctx, P, sa, B);
+  } else {
+    if (p == 1) trisolve_levels<1, double2>(ctx, P, sa, B);
+    else if (p <= 4) trisolve_levels<4, double2>(ctx, P, sa, B);
+    else if (p <= 8) trisolve_levels<8, double2>(ctx, P, sa, B);
+    else trisolve_levels<16, double2>(ctx, P, sa, B);
+  }
   TM_HIP(hipGetLastError());
 }
 

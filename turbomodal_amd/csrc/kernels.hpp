@@ -145,10 +145,13 @@ struct Operator {
 
 // ---- factorization (factor.hip) ------------------------------------------------
 struct BatchBufs {
-  double2* L;        // B * panel_entries
-  double2* U;        // B * upool_entries
-  double2* S;        // B * inv_scratch_entries
-  double2* T;        // B * p * tvec_entries (p right-hand sides)
+  // the factor's panels, update matrices and inversion scratch: complex FP64 (double2),
+  // or complex FP32 (float2) when fp32
+  void* L;           // B * panel_entries
+  void* U;           // B * upool_entries
+  void* S;           // B * inv_scratch_entries
+  bool fp32 = false;
+  double2* T;        // B * p * tvec_entries (p right-hand sides; FP64 in either case)
   int* status;       // B: bit 0 zero / non-finite pivot, bit 1 near-singular pivot
                      // (|d_j| <= kTinyPivot |a_jj|, a_jj the diagonal of A(k) before elimination)
   int* pivinfo;      // 2 B (may be null): negative pivots, bits of the smallest |d_j| / |a_jj| (float)
@@ -195,7 +198,7 @@ Server* server_create(Context* ctx, int max_slots, double hbm_fraction);
 void server_destroy(Server* s);
 uint64_t server_submit(Server* s, const tm_problem& p, tm_result* out);
 bool server_wait(Server* s, uint64_t ticket, double timeout_s);
-int64_t problem_footprint(const Operator& op, int num_modes, int harmonic, int method);
+int64_t problem_footprint(const Operator& op, int num_modes, int harmonic, int method, int precision);
 
 }  // namespace tme
 

@@ -866,6 +866,78 @@ __global__ void k_copy_cols(const double2* __restrict__ src, long long lds, cons
   for (int c = 0; c < ncol; ++c) dst[b * ldd + (long long)(d0 + c) * n + i] = src[b * lds + (long long)(s0 + c) * n + i];
 }
 
+// Refinement of a shift-invert solve with the complex-FP32 factor: for c < ncol,
+//   R_b[:, c] = Y_b[:, s0_b + c] - (K W_b[:, c] - sigma_b M W_b[:, c])
+// with Y = M V (the right-hand sides), KW and MW the FP64 products of the current
+// solution W; R overwrites KW.
+__global__ void k_mixed_residual(const double2* __restrict__ Y, long long ldy, const int* __restrict__ s0v,
+                                 double2* __restrict__ KW, const double2* __restrict__ MW, long long ldw,
+                                 const ProbParam* __restrict__ params, int ncol, long long n,
+                                 const int* __restrict__ act) {
+  const int b = blockIdx.y;
+  if (act && !act[b]) return;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double sg = params[b].sigma;
+  const int s0 = s0v ? s0v[b] : 0;
+  for (int c = 0; c < ncol; ++c) {
+    const double2 y = Y[b * ldy + (long long)(s0 + c) * n + i];
+    const long long e = b * ldw + (long long)c * n + i;
+    const double2 k = KW[e], m = MW[e];
+    KW[e] = make_double2(y.x - k.x + sg * m.x, y.y - k.y + sg * m.y);
+  }
+}
+
+// Partial sums of |R_b[:, c]|^2 and |Y_b[:, s0_b + c]|^2 over kDotChunk rows per block
+// (grid x), reduced on the host in chunk order: part[((b ncol + c) nchunk + x) 2 + {0, 1}]
+__global__ __launch_bounds__(256) void k_colnorm_partial(const double2* __restrict__ R, long long ldr,
+                                                         const double2* __restrict__ Y, long long ldy,
+                                                         const int* __restrict__ s0v, int ncol, long long n,
+                                                         const int* __restrict__ act, double* __restrict__ part,
+                                                         int nchunk) {
+  const int b = blockIdx.y;
+  if (act && !act[b]) return;
+  const int s0 = s0v ? s0v[b] : 0;
+  const long long r0 = (long long)blockIdx.x * kDotChunk;
+  __shared__ double red[2][256];
+  for (int c = 0; c < ncol; ++c) {
+    double sr = 0.0, sy = 0.0;
+    for (long long i = r0 + threadIdx.x; i < r0 + kDotChunk && i < n; i += 256) {
+      const double2 r = R[b * ldr + (long long)c * n + i], y = Y[b * ldy + (long long)(s0 + c) * n + i];
+      sr = fma(r.x, r.x, fma(r.y, r.y, sr));
+      sy = fma(y.x, y.x, fma(y.y, y.y, sy));
+    }
+    red[0][threadIdx.x] = sr;
+    red[1][threadIdx.x] = sy;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) {
+        red[0][threadIdx.x] += red[0][threadIdx.x + o];
+        red[1][threadIdx.x] += red[1][threadIdx.x + o];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      part[(((long long)b * ncol + c) * nchunk + blockIdx.x) * 2] = red[0][0];
+      part[(((long long)b * ncol + c) * nchunk + blockIdx.x) * 2 + 1] = red[1][0];
+    }
+    __syncthreads();
+  }
+}
+
+// W_b[:, c] += D_b[:, c], c < ncol
+__global__ void k_add_cols(double2* __restrict__ W, const double2* __restrict__ D, long long ld, int ncol,
+                           long long n, const int* __restrict__ act) {
+  const int b = blockIdx.y;
+  if (act && !act[b]) return;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < ncol; ++c) {
+    const long long e = b * ld + (long long)c * n + i;
+    W[e] = make_double2(W[e].x + D[e].x, W[e].y + D[e].y);
+  }
+}
+
 // Y_b[:, c] = |A_b| |X_b[:, c]| (entrywise moduli; real result in .x) for c < ncol_b
 // |A_q| |x| for up to three block arrays over the same columns (blocks3 optional):
 // Y_q[:, c] = |A_q| |X[:, c]| (real, in the .x part), c < ncol_b.  One wave per node

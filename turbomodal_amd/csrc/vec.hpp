@@ -77,6 +77,11 @@ __global__ void k_tsgemm_reduce(const double2* part, const int* ka_b, int kb, co
 __global__ void k_tsmm(const double2* A, long long lda_b, long long n, long long cs, const double2* C, const int* ka_b, int kc,
                        const int* act, int kamax, double alpha, double beta, double2* Y, long long ldy_b,
                        const int* yoff, int r0);
+__global__ void k_mixed_residual(const double2* Y, long long ldy, const int* s0v, double2* KW, const double2* MW,
+                                 long long ldw, const ProbParam* params, int ncol, long long n, const int* act);
+__global__ void k_colnorm_partial(const double2* R, long long ldr, const double2* Y, long long ldy, const int* s0v,
+                                  int ncol, long long n, const int* act, double* part, int nchunk);
+__global__ void k_add_cols(double2* W, const double2* D, long long ld, int ncol, long long n, const int* act);
 __global__ void k_copy_cols(const double2* src, long long lds, const int* s0v, double2* dst, long long ldd,
                             const int* d0v, int ncol, long long n, const int* act);
 template <int CPL>
