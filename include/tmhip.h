@@ -159,7 +159,14 @@ typedef struct tm_problem {
                                 Lancaster pencil (src/cyclic_solver.cpp:986-1005,
                                 src/modal_solver.cpp:149-312); forward and backward modes
                                 of smallest |w|, whirl = sign(w); block method, >= 64 free DOFs */
-  int32_t reserved;
+  int32_t warm_start;        /* != 0: rpm continuation in a sweep.  A block-method problem
+                                (not the Coriolis pencil) may start its Krylov basis from the
+                                converged Ritz vectors of the problem of the same operator,
+                                harmonic and mode count at the adjacent rpm of the same call,
+                                solved in an earlier wave of its slot group, and then checks
+                                convergence from its fifth block on.  Results match the
+                                oracle to the same bars; they differ from a cold start at
+                                rounding level.  tm_solve_batch only (the server ignores it) */
 } tm_problem;
 
 /* Result of one problem, in the reference's ModalResult layout

@@ -97,7 +97,9 @@ def test_c4_full_sweep(ctx, c2):
     gold = load_golden("c4_sample.npz")
     rpms = np.linspace(0.0, 15000.0, 20)
     N = m.num_sectors
-    probs = [dict(op=op, rpm=float(r), harmonic=k, num_modes=10) for r in rpms for k in range(N // 2 + 1)]
+    # as rpm_sweep submits it: rpm continuation between adjacent points (tm_problem.warm_start)
+    probs = [dict(op=op, rpm=float(r), harmonic=k, num_modes=10, warm_start=True)
+             for r in rpms for k in range(N // 2 + 1)]
     outs = capi.solve_batch(ctx, probs)
     n = em.info()["n_free"]
     above, worst_err, worst_floor = 0, 0.0, 0.0

@@ -57,7 +57,7 @@ class Problem(C.Structure):
                 ("num_modes", C.c_int32), ("precision", C.c_int32), ("conv_policy", C.c_int32),
                 ("shift", C.c_double), ("tol_rel", C.c_double), ("max_restarts", C.c_int32),
                 ("method", C.c_int32), ("min_frequency", C.c_double), ("coriolis", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("warm_start", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -297,6 +297,7 @@ def _prepare(p, want_shapes, prob, res):
     prob.method = int(p.get("method", TM_METHOD_BLOCK))
     prob.min_frequency = float(p.get("min_frequency", 0.0))
     prob.coriolis = 1 if p.get("coriolis", False) else 0
+    prob.warm_start = 1 if p.get("warm_start", False) else 0
     nm = prob.num_modes
     nn = op.mesh.n_nodes
     f = np.zeros(nm)

@@ -699,7 +699,8 @@ class CyclicSymmetrySolver {
         p.max_restarts = 1000;
         p.method = method_;
         p.coriolis = coriolis ? 1 : 0;
-        p.reserved = 0;
+        // rpm continuation between the sweep's adjacent rpm points
+        p.warm_start = rpms.size() > 1 ? 1 : 0;
         p.min_frequency = min_freq;
         freq[q].assign(num_modes, 0.0);
         ev[q].assign(num_modes, 0.0);

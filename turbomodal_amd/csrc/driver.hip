@@ -18,6 +18,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <set>
 #include <numeric>
 #include <condition_variable>
 #include <deque>
@@ -863,6 +865,7 @@ struct HostProblem {
   int nev_target = 0, nev = 0, ncv = 0;
   int restarts = 0, max_restarts = 1000;
   bool done = false, converged = false, failed = false;
+  bool warm_started = false;  // its start block holds an adjacent rpm's Ritz vectors
   int n_extract = 0;
   std::vector<double> theta_sorted_lambda;  // extracted eigenvalues
   std::vector<double> S_extract;            // ncv x n_extract (row-major k, c)
@@ -2336,6 +2339,14 @@ class BatchLanczos {
     }
     HostTimer h(ctx_, "iterate");
     random_block(mask, std::vector<int>(B_, 0));
+    for (int b : slots) {  // rpm continuation: the adjacent point's Ritz vectors lead the start block
+      const std::pair<DevBuf, int>* src = mask[b] ? warm_source(probs_[b]) : nullptr;
+      if (!src) continue;
+      TM_HIP(hipMemcpyAsync(Wb_.as<double2>() + b * ldw_, src->first.p, sizeof(double2) * (size_t)src->second * n_,
+                            hipMemcpyDeviceToDevice, stream()));
+      probs_[b]->warm_started = true;
+    }
+    warm_prune(slots);
     if (qep_) smooth_block(mask);
     append_or_retry(mask);
     for (int b : slots)
@@ -2410,6 +2421,7 @@ class BatchLanczos {
     {
       HostTimer h(ctx_, "extract");
       extract(sel);
+      warm_keep(sel);
     }
     const auto now = std::chrono::steady_clock::now();
     for (int b = 0; b < B_; ++b)
@@ -2430,6 +2442,16 @@ class BatchLanczos {
     int nevmax = 1;
     for (auto* hp : queue) nevmax = std::max(nevmax, hp->nev);
     slot_setup(capacity, nevmax, qep, mixed);
+    warm_rpms_.clear();
+    warm_store_.clear();
+    warm_admitted_.clear();
+    if (!qep)
+      for (auto* hp : queue)
+        if (hp->in.warm_start) warm_rpms_[warm_series(hp)].push_back(hp->in.rpm);
+    for (auto& kv : warm_rpms_) {
+      std::sort(kv.second.begin(), kv.second.end());
+      kv.second.erase(std::unique(kv.second.begin(), kv.second.end()), kv.second.end());
+    }
     refill = std::max(1, std::min(refill, capacity));
     size_t next = 0;
     while (true) {
@@ -2449,6 +2471,61 @@ class BatchLanczos {
  private:
   std::vector<ProbParam> h_params_;
   std::vector<OpArrays> h_ops_;  // per slot: the problem's operator (material) on the shared mesh
+
+  // rpm continuation (tm_problem.warm_start): the converged Ritz vectors of every
+  // finished problem of the run, by (operator, harmonic, wanted pairs) and rpm, so that a
+  // problem admitted later at an adjacent rpm point starts from them.  Only problems of
+  // this slot group and of earlier waves are used: the choice depends on the run alone.
+  using WarmSeries = std::tuple<const void*, int, int>;
+  std::map<WarmSeries, std::vector<double>> warm_rpms_;                 // sorted rpm points
+  std::map<std::pair<WarmSeries, double>, std::pair<DevBuf, int>> warm_store_;  // vectors, columns
+  std::set<std::pair<WarmSeries, double>> warm_admitted_;
+  static WarmSeries warm_series(const HostProblem* hp) {
+    return WarmSeries{hp->in.op, hp->k, hp->nev};
+  }
+  // the stored Ritz vectors of the adjacent rpm point (lower one first), or null
+  const std::pair<DevBuf, int>* warm_source(const HostProblem* hp) const {
+    if (!hp->in.warm_start || qep_) return nullptr;
+    const WarmSeries key = warm_series(hp);
+    auto it = warm_rpms_.find(key);
+    if (it == warm_rpms_.end()) return nullptr;
+    const auto& r = it->second;
+    const long idx = std::lower_bound(r.begin(), r.end(), hp->in.rpm) - r.begin();
+    for (long j : {idx - 1, idx + 1}) {
+      if (j < 0 || j >= (long)r.size()) continue;
+      auto f = warm_store_.find({key, r[j]});
+      if (f != warm_store_.end()) return &f->second;
+    }
+    return nullptr;
+  }
+  // records the admission of the problems in slots and frees the stored vectors that no
+  // problem still to come can use (both adjacent rpm points admitted)
+  void warm_prune(const std::vector<int>& slots) {
+    for (int b : slots)
+      if (probs_[b]->in.warm_start) warm_admitted_.insert({warm_series(probs_[b]), probs_[b]->in.rpm});
+    for (auto it = warm_store_.begin(); it != warm_store_.end();)
+      it = warm_needed(it->first.first, it->first.second) ? std::next(it) : warm_store_.erase(it);
+  }
+  bool warm_needed(const WarmSeries& key, double rpm) const {
+    const auto& r = warm_rpms_.at(key);
+    const long idx = std::lower_bound(r.begin(), r.end(), rpm) - r.begin();
+    for (long j : {idx - 1, idx + 1})
+      if (j >= 0 && j < (long)r.size() && !warm_admitted_.count({key, r[j]})) return true;
+    return false;
+  }
+  // keeps the Ritz vectors of the finished, converged problems of sel (columns 0.. of X)
+  void warm_keep(const std::vector<int>& sel) {
+    for (int b = 0; b < B_; ++b) {
+      HostProblem* hp = sel[b] ? probs_[b] : nullptr;
+      if (!hp || !hp->in.warm_start || qep_ || hp->failed || !hp->converged || !hp->done) continue;
+      const int nc = std::min(hp->n_extract, pb_);
+      if (nc <= 0 || !warm_needed(warm_series(hp), hp->in.rpm)) continue;
+      DevBuf v(pool(), sizeof(double2) * (size_t)nc * n_);
+      TM_HIP(hipMemcpyAsync(v.p, Xb_.as<double2>() + b * ldk_, sizeof(double2) * (size_t)nc * n_,
+                            hipMemcpyDeviceToDevice, stream()));
+      warm_store_[{warm_series(hp), hp->in.rpm}] = std::make_pair(std::move(v), nc);
+    }
+  }
 
   // Appends W; a problem whose block lost rank (an invariant subspace was
   // reached) continues from fresh random directions, up to three times.
@@ -2552,7 +2629,9 @@ class BatchLanczos {
     for (int b = 0; b < B_; ++b) {
       if (!act[b]) continue;
       HostProblem* hp = probs_[b];
-      const bool due = nv_[b] >= mmax_ - pb_ || hp->restarts > 0 || nv_[b] + pb_ > mmax_;
+      // a warm-started problem also checks after its fifth and later blocks
+      const bool early = hp->warm_started && nv_[b] >= 5 * pb_;
+      const bool due = nv_[b] >= mmax_ - pb_ || hp->restarts > 0 || nv_[b] + pb_ > mmax_ || early;
       if (due && nv_[b] >= std::min(hp->nev + pb_, mmax_)) {
         rr[b] = 1;
         anyrr = true;
@@ -2705,8 +2784,8 @@ class BatchLanczos {
           good = true;
         if (!good && first_bad < 0) first_bad = c;
         if (trace_)
-          std::fprintf(stderr, "[tmhip] k=%d rpm=%.0f nv=%d rst=%d c=%d lam=%.10e res=%.2e floor=%.2e\n", hp->k,
-                       hp->in.rpm, nv_[b], hp->restarts, c, theta[b][c], r, fl);
+          std::fprintf(stderr, "[tmhip] k=%d rpm=%.0f warm=%d nv=%d rst=%d c=%d lam=%.10e res=%.2e floor=%.2e\n",
+                       hp->k, hp->in.rpm, hp->warm_started ? 1 : 0, nv_[b], hp->restarts, c, theta[b][c], r, fl);
       }
       prev_res_[b] = res[b];
       const bool converged = first_bad < 0 && ncheck[b] >= hp->nev;
@@ -2917,6 +2996,13 @@ static void merge_stats(Context* ctx, Context* g) {
 static void run_block_groups(Context* ctx, Operator& op, std::vector<std::vector<HostProblem*>>& runs, int64_t per,
                              bool qep, bool mixed) {
   per = std::max<int64_t>(per, 1);
+  // a warm-started run takes at most a third of its problems per wave, so that its later
+  // waves find adjacent rpm points solved
+  auto wave_cap = [&](const std::vector<HostProblem*>& q, int64_t cap) {
+    if (!qep && std::any_of(q.begin(), q.end(), [](const HostProblem* h) { return h->in.warm_start != 0; }))
+      cap = std::min<int64_t>(cap, std::max<int64_t>(1, ((int64_t)q.size() + 2) / 3));
+    return (int)cap;
+  };
   const char* ser = std::getenv("TMHIP_SERIAL_GROUPS");
   const bool serial = runs.size() == 1 || (ser && std::atoi(ser) != 0);
   size_t free_b = 0, total_b = 0;
@@ -2926,7 +3012,7 @@ static void run_block_groups(Context* ctx, Operator& op, std::vector<std::vector
       TM_HIP(hipMemGetInfo(&free_b, &total_b));
       // blocks cached by the context's pool are reusable by this call
       const int64_t budget = (int64_t)(ctx->batch_fraction * (double)(free_b + ctx->pool.cached()));
-      const int cap = (int)std::max<int64_t>(1, std::min<int64_t>({64, (int64_t)q.size(), budget / per}));
+      const int cap = wave_cap(q, std::max<int64_t>(1, std::min<int64_t>({64, (int64_t)q.size(), budget / per})));
       BatchLanczos bl(ctx, op, {}, true);
       bl.run_slots(q, cap, cap, qep, mixed);
     }
@@ -2951,7 +3037,8 @@ static void run_block_groups(Context* ctx, Operator& op, std::vector<std::vector
       try {
         TM_HIP(hipSetDevice(c->device));
         const int64_t share = (int64_t)(budget * (double)runs[g].size() / (double)total);
-        const int cap = (int)std::max<int64_t>(1, std::min<int64_t>({64, (int64_t)runs[g].size(), share / per}));
+        const int cap =
+            wave_cap(runs[g], std::max<int64_t>(1, std::min<int64_t>({64, (int64_t)runs[g].size(), share / per})));
         {
           BatchLanczos bl(c, op, {}, true);
           bl.run_slots(runs[g], cap, cap, qep, mixed);
@@ -3079,15 +3166,47 @@ static std::vector<int> solve_problems_impl(Context* ctx, const tm_problem* prob
         runs.emplace_back(list.begin() + i, list.begin() + j);
         i = j;
       }
+      // rpm continuation (tm_problem.warm_start): a run's problems in the order of a third of
+      // its rpm points (every third one), then the next third, then the rest, harmonic by
+      // harmonic within each point, so that most problems find an adjacent rpm point solved in
+      // an earlier wave of their slot group
+      const auto warm_run = [&](const std::vector<HostProblem*>& r) {
+        return !qep && std::any_of(r.begin(), r.end(), [](const HostProblem* h) { return h->in.warm_start != 0; });
+      };
+      for (auto& r : runs) {
+        if (!warm_run(r)) continue;
+        std::vector<double> pts;
+        for (auto* h : r) pts.push_back(h->in.rpm);
+        std::sort(pts.begin(), pts.end());
+        pts.erase(std::unique(pts.begin(), pts.end()), pts.end());
+        auto idx = [&](const HostProblem* h) { return std::lower_bound(pts.begin(), pts.end(), h->in.rpm) - pts.begin(); };
+        std::stable_sort(r.begin(), r.end(), [&](const HostProblem* x, const HostProblem* y) {
+          const long ix = idx(x), iy = idx(y);
+          if (ix % 3 != iy % 3) return ix % 3 < iy % 3;
+          if (ix != iy) return ix < iy;
+          return x->k < y->k;
+        });
+      }
       // while fewer than three groups: the largest run (16 problems or more) in two
-      // interleaved halves, so that groups overlap one another's host work
+      // interleaved halves, so that groups overlap one another's host work (a warm run by
+      // harmonics, alternately, so that each harmonic's rpm points stay in one group)
       while (runs.size() < 3) {
         size_t big = 0;
         for (size_t g = 1; g < runs.size(); ++g)
           if (runs[g].size() > runs[big].size()) big = g;
         if (runs[big].size() < 16) break;
         std::vector<HostProblem*> a, b;
-        for (size_t i = 0; i < runs[big].size(); ++i) (i % 2 ? b : a).push_back(runs[big][i]);
+        if (warm_run(runs[big])) {
+          std::vector<int> ks;
+          for (auto* h : runs[big]) ks.push_back(h->k);
+          std::sort(ks.begin(), ks.end());
+          ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
+          if (ks.size() < 2) break;
+          for (auto* h : runs[big])
+            ((std::lower_bound(ks.begin(), ks.end(), h->k) - ks.begin()) % 2 ? b : a).push_back(h);
+        } else {
+          for (size_t i = 0; i < runs[big].size(); ++i) (i % 2 ? b : a).push_back(runs[big][i]);
+        }
         runs[big].swap(a);
         runs.insert(runs.begin() + big + 1, std::move(b));
       }
