@@ -123,6 +123,7 @@ void tm_context_destroy(tm_context* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   ctx->groups.clear();  // the slot groups' contexts
   ctx->deliver.reset();  // drains its copies
+  ctx->clear_graphs();
   for (auto& p : ctx->pending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);

@@ -38,6 +38,7 @@ namespace tme {
 Deliverer::Deliverer(int device) : device_(device) {
   TM_HIP(hipSetDevice(device));
   TM_HIP(hipStreamCreateWithFlags(&cs_, hipStreamNonBlocking));
+  TM_HIP(hipStreamCreateWithFlags(&ws_, hipStreamNonBlocking));
   th_ = std::thread([this] { run(); });
 }
 
@@ -50,7 +51,9 @@ Deliverer::~Deliverer() {
   if (th_.joinable()) th_.join();
   for (auto& f : free_) (void)hipFree(f.second);
   for (auto e : events_) (void)hipEventDestroy(e);
+  if (mark_) (void)hipEventDestroy(mark_);
   (void)hipStreamDestroy(cs_);
+  (void)hipStreamDestroy(ws_);
 }
 
 void* Deliverer::stage(size_t n, size_t& cap) {
@@ -95,7 +98,13 @@ void Deliverer::enqueue(hipStream_t s, Job j) {
     }
   }
   if (!j.ev) TM_HIP(hipEventCreateWithFlags(&j.ev, hipEventDisableTiming));
-  TM_HIP(hipEventRecord(j.ev, s));
+  if (!mark_) TM_HIP(hipEventCreateWithFlags(&mark_, hipEventDisableTiming));
+  // the compute stream's point, passed on to the wait stream while the compute
+  // stream is not capturing (this thread is its only user); the worker then waits
+  // for j.ev on the wait stream only
+  TM_HIP(hipEventRecord(mark_, s));
+  TM_HIP(hipStreamWaitEvent(ws_, mark_, 0));
+  TM_HIP(hipEventRecord(j.ev, ws_));
   std::lock_guard<std::mutex> g(mu_);
   q_.push_back(std::move(j));
   cv_.notify_all();
@@ -130,12 +139,13 @@ void Deliverer::run() {
     }
     std::string e;
     hipError_t r = hipSuccess;
+    // j.ev is on the wait stream (never capturing): the compute stream may be
+    // capturing a graph by now, and an event last recorded on a capturing stream can
+    // be neither waited for nor queried
+    r = hipEventSynchronize(j.ev);
     if (j.dst) {
-      r = hipStreamWaitEvent(cs_, j.ev, 0);
       if (r == hipSuccess) r = hipMemcpyAsync(j.dst, j.src, j.n, hipMemcpyDeviceToHost, cs_);
       if (r == hipSuccess) r = hipStreamSynchronize(cs_);
-    } else {
-      r = hipEventSynchronize(j.ev);
     }
     if (r != hipSuccess) e = hipGetErrorString(r);
     {
@@ -803,22 +813,19 @@ void operator_inertia_in(Context* ctx, Operator& op, double rpm, int k, double s
 }
 
 // M(k) blocks of the operator, cached per harmonic.  Operators are shared by
-// the server's slot groups (each on its own thread and stream): the first user
-// builds the blocks on its stream and records an event that every later user's
-// stream waits for, so no host synchronisation is needed.
+// the slot groups (each on its own thread and stream): the first user builds the
+// blocks on its stream and waits for them on the host before it enters them, so
+// later users (on any stream) find them complete.  No event is shared between the
+// streams: a stream cannot wait on, nor the host query, an event last recorded on
+// a stream that is capturing a graph, which another group's may be doing.
 const double2* harmonic_mass(Context* ctx, Operator& op, int k) {
   static std::mutex mu;
   std::lock_guard<std::mutex> g(mu);
   auto it = op.Mk_cache.find(k);
-  if (it != op.Mk_cache.end()) {
-    TM_HIP(hipStreamWaitEvent(ctx->stream, it->second.ready, 0));
-    return it->second.b.as<double2>();
-  }
+  if (it != op.Mk_cache.end()) return it->second.as<double2>();
   Mesh& M = *op.mesh;
   const Symbolic& S = M.S;
-  Operator::MkEntry e;
-  e.b.alloc(sizeof(double2) * 9 * op.nblk);
-  DevBuf& b = e.b;
+  DevBuf b(sizeof(double2) * 9 * op.nblk);
   double alpha = 2.0 * M_PI / S.N;
   hipLaunchKernelGGL(k_build_phase_blocks, dim3((9 * (long long)op.nblk + 255) / 256), dim3(256), 0, ctx->stream,
                      (long long)op.nblk, M.d_slot_row.as<int>(), M.d_colidx.as<int>(), M.d_tslot.as<int>(),
@@ -826,10 +833,9 @@ const double2* harmonic_mass(Context* ctx, Operator& op, int k) {
                      (const double*)nullptr, (const double*)nullptr, 0.0, std::cos(k * alpha),
                      std::sin(k * alpha), b.as<double2>(), 0, make_double2(1.0, 0.0));
   TM_HIP(hipGetLastError());
-  TM_HIP(hipEventCreateWithFlags(&e.ready, hipEventDisableTiming));
-  TM_HIP(hipEventRecord(e.ready, ctx->stream));
+  TM_HIP(hipStreamSynchronize(ctx->stream));
   const double2* p = b.as<double2>();
-  op.Mk_cache.emplace(k, std::move(e));
+  op.Mk_cache.emplace(k, std::move(b));
   return p;
 }
 
@@ -1941,6 +1947,14 @@ class BatchLanczos {
   }
 
   DevBuf ints(const std::vector<int>& v) { return upload(pool(), v, stream()); }
+  // v into the device buffer d, which stays in place from call to call (the
+  // factorization's and the block sweeps' captured launches read it)
+  template <class T>
+  void put(DevBuf& d, const std::vector<T>& v) {
+    if (d.bytes < sizeof(T) * v.size()) throw std::logic_error("put: buffer too small");
+    TM_HIP(hipMemcpyAsync(d.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, stream()));
+    TM_HIP(hipStreamSynchronize(stream()));
+  }
 
   // C_b = A_b^H B_b (ka_b x kb, kb <= 16) over the first len rows (default: all)
   // of columns nvec_ apart, returned on the host as [b][a][c] (rows of 16)
@@ -2270,6 +2284,9 @@ class BatchLanczos {
     const int pcols = std::min(16, std::max(block_width(nevmax_, qep_), nevmax_));
     bm_.alloc(pool(), P_, B_, pcols, mixed_);
     act_.alloc(pool(), sizeof(int) * B_);
+    gmask_.alloc(pool(), sizeof(int) * B_);
+    gact_.alloc(pool(), sizeof(int) * B_);
+    gops_.alloc(pool(), sizeof(OpArrays) * B_);
     h_params_.assign(B_, ProbParam{1.0, 0.0, 0.0, 1.0});
     h_ops_.assign(B_, op_arrays(op_));
     mk_.assign(B_, nullptr);
@@ -2318,13 +2335,13 @@ class BatchLanczos {
     TM_HIP(hipMemcpyAsync(bm_.status.p, zero.data(), sizeof(int) * B_, hipMemcpyHostToDevice, stream()));
     mk_ptrs_ = upload(pool(), mk_, stream());
     build_stiffness_blocks(slots);
-    DevBuf dmask = ints(mask);
+    put(gmask_, mask);
     {
       HostTimer h(ctx_, "factor");
       BatchBufs bb = bm_.bufs(P_, B_);
-      DevBuf dops = upload(pool(), h_ops_, stream());
-      factor_batch(ctx_, M_, P_, op_arrays(op_), bb, dmask.as<int>(), static_cast<int>(slots.size()),
-                   dops.as<OpArrays>());
+      put(gops_, h_ops_);
+      factor_batch(ctx_, M_, P_, op_arrays(op_), bb, gmask_.as<int>(), static_cast<int>(slots.size()),
+                   gops_.as<OpArrays>(), true);
       std::vector<int> status(B_);
       TM_HIP(hipMemcpyAsync(status.data(), bm_.status.p, sizeof(int) * B_, hipMemcpyDeviceToHost, stream()));
       TM_HIP(hipStreamSynchronize(stream()));
@@ -2364,7 +2381,9 @@ class BatchLanczos {
     for (int b = 0; b < B_; ++b) any = any || iter_[b];
     if (!any) return;
     BatchBufs bb = bm_.bufs(P_, B_);
-    DevBuf dact = ints(iter_), dy0 = ints(y0_);
+    put(gact_, iter_);
+    const DevBuf& dact = gact_;
+    DevBuf dy0 = ints(y0_);
     const int nit = (int)std::count_if(iter_.begin(), iter_.end(), [](int v) { return v != 0; });
     if (!qep_) {
       // W = (K - sigma M)^-1 M Y, Y = V[:, y0 : y0 + pb]
@@ -2373,7 +2392,7 @@ class BatchLanczos {
         hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), MVb_.as<double2>(), ldv_,
                            dy0.as<int>(), Wb_.as<double2>(), ldw_, (const int*)nullptr, pb_, n_, dact.as<int>());
       }
-      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>(), ldw_, dact.as<int>(), pb_, -1, nit);
+      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>(), ldw_, dact.as<int>(), pb_, -1, nit, true);
       if (mixed_) {
         std::vector<int> nc(B_, 0);
         for (int b = 0; b < B_; ++b) nc[b] = iter_[b] ? pb_ : 0;
@@ -2396,7 +2415,7 @@ class BatchLanczos {
       }
       // W_b += (-D) Y_b
       spmv_cols(dptr_, Wb_.as<double2>(), ldw_, dpb, pb_, Wb_.as<double2>() + n_, true);
-      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>() + n_, ldw_, dact.as<int>(), pb_, nvec_, nit);
+      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>() + n_, ldw_, dact.as<int>(), pb_, nvec_, nit, true);
     }
     for (int b = 0; b < B_; ++b)
       if (iter_[b]) probs_[b]->steps += 1;
@@ -2471,6 +2490,9 @@ class BatchLanczos {
  private:
   std::vector<ProbParam> h_params_;
   std::vector<OpArrays> h_ops_;  // per slot: the problem's operator (material) on the shared mesh
+  // per slot, in place for the life of the slots: the factorization's mask and
+  // operators and the block sweeps' active mask (read by captured launches)
+  DevBuf gmask_, gact_, gops_;
 
   // rpm continuation (tm_problem.warm_start): the converged Ritz vectors of every
   // finished problem of the run, by (operator, harmonic, wanted pairs) and rpm, so that a
@@ -2934,6 +2956,7 @@ static void release_context(Context& c) {
   (void)hipStreamSynchronize(c.stream);
   c.groups.clear();
   c.deliver.reset();  // drains its copies
+  c.clear_graphs();
   c.pool.trim();
   for (auto& p : c.pending) {
     (void)hipEventDestroy(p.a);

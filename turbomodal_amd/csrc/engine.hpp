@@ -189,7 +189,9 @@ class Deliverer {
   void enqueue(hipStream_t s, Job j);
   void run();
   int device_;
-  hipStream_t cs_ = nullptr;
+  hipStream_t cs_ = nullptr;  // the worker's copies
+  hipStream_t ws_ = nullptr;  // waits for the compute streams' points (queued by their threads)
+  hipEvent_t mark_ = nullptr;
   std::thread th_;
   std::mutex mu_;
   std::condition_variable cv_, cv_done_;
@@ -231,6 +233,14 @@ struct Context {
   // stream, pool and deliverer), kept so their pools' cached workspaces serve
   // the next call; their deleters synchronise and release them
   std::vector<std::shared_ptr<Context>> groups;
+  // captured launch sequences (hipGraph executables) by the bytes of every argument
+  // of their launches (run_captured); a replay issues exactly what a fresh recording
+  // would.  Released with the context.
+  std::map<std::string, hipGraphExec_t> graphs;
+  void clear_graphs() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+  }
   hipEvent_t get_event();
   void flush_timing();  // synchronises and folds pending events into stats
   Deliverer& deliverer() {
@@ -238,6 +248,19 @@ struct Context {
     return *deliver;
   }
 };
+
+// Runs record() (kernel launches on ctx->stream only: no synchronisation, copies,
+// memsets or allocations) through ctx's graph cache: the first call with a key
+// captures the launches into a hipGraph and instantiates it, later calls with the
+// same key replay it.  `key` must hold every value the launches depend on.
+// TMHIP_GRAPHS=0 in the environment launches directly instead.
+void run_captured(Context* ctx, const std::string& key, const std::function<void()>& record);
+
+// Appends the bytes of v to a graph key.
+template <class T>
+inline void key_add(std::string& k, const T& v) {
+  k.append(reinterpret_cast<const char*>(&v), sizeof(T));
+}
 
 // RAII scope that adds its host wall time to stats["host:" + name] (profiling only).
 struct HostTimer {

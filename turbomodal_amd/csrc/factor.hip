@@ -22,6 +22,9 @@
 // Triangular solves reuse the same tree: forward (leaves -> root) with update
 // vectors passed to the parent, diagonal scaling fused into the backward
 // sweep, backward (root -> leaves) gathering already-solved ancestor values.
+#include <cstdlib>
+#include <cstring>
+
 #include "kernels.hpp"
 
 namespace tme {
@@ -1260,18 +1263,68 @@ static void factor_levels(Context* ctx, const Plan& P, const FactorArgs& fa, con
   }
 }
 
+void run_captured(Context* ctx, const std::string& key, const std::function<void()>& record) {
+  const char* env = std::getenv("TMHIP_GRAPHS");
+  if (env && std::strcmp(env, "0") == 0) {
+    record();
+    return;
+  }
+  auto it = ctx->graphs.find(key);
+  if (it == ctx->graphs.end()) {
+    if (ctx->graphs.size() >= 32) ctx->clear_graphs();  // bounded: a new batch shape starts afresh
+    hipGraph_t g = nullptr;
+    TM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      record();
+    } catch (...) {
+      (void)hipStreamEndCapture(ctx->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    const hipError_t le = hipGetLastError();
+    TM_HIP(hipStreamEndCapture(ctx->stream, &g));
+    if (le != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      throw HipError(std::string("captured launch: ") + hipGetErrorString(le));
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    TM_HIP(e);
+    it = ctx->graphs.emplace(key, ex).first;
+  }
+  TM_HIP(hipGraphLaunch(it->second, ctx->stream));
+}
+
 void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& ops,
-                  const BatchBufs& bb, const int* active, int nactive, const OpArrays* ops_b) {
+                  const BatchBufs& bb, const int* active, int nactive, const OpArrays* ops_b, bool graph) {
   FactorArgs fa = factor_args(M, P, ops, bb);
   fa.active = active;
   fa.ops_b = ops_b;
   InvArgs ia{P.d_sn.as<DevSn>(), bb.L, bb.S, P.panel_entries, P.inv_scratch_entries, active};
   const int B = bb.B;
-  if (bb.pivinfo)
-    hipLaunchKernelGGL(k_pivinfo_init, dim3((B + 63) / 64), dim3(64), 0, ctx->stream, bb.pivinfo, active, B);
   KTimer t(ctx, bb.fp32 ? "factor_fp32" : "factor", 0, M.S.flops * (nactive >= 0 ? nactive : B));
-  if (bb.fp32) factor_levels<float2>(ctx, P, fa, ia, B);
-  else factor_levels<double2>(ctx, P, fa, ia, B);
+  auto record = [&] {
+    if (bb.pivinfo)
+      hipLaunchKernelGGL(k_pivinfo_init, dim3((B + 63) / 64), dim3(64), 0, ctx->stream, bb.pivinfo, active, B);
+    if (bb.fp32) factor_levels<float2>(ctx, P, fa, ia, B);
+    else factor_levels<double2>(ctx, P, fa, ia, B);
+  };
+  if (graph) {
+    // every launch argument: the plan (its task tables and level structure), the
+    // argument blocks (all 8-byte fields, no padding), the batch and the element type
+    static_assert(sizeof(FactorArgs) % 8 == 0 && sizeof(InvArgs) % 8 == 0, "unpadded argument blocks");
+    std::string key("factor");
+    key_add(key, P.serial);
+    key_add(key, fa);
+    key_add(key, ia);
+    key_add(key, B);
+    key_add(key, bb.fp32);
+    key_add(key, bb.pivinfo);
+    run_captured(ctx, key, record);
+  } else {
+    record();
+  }
   TM_HIP(hipGetLastError());
 }
 
@@ -1316,7 +1369,7 @@ static void trisolve_levels(Context* ctx, const Plan& P, const SolveArgs& sa, in
 }
 
 void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs& bb, double2* X,
-                    long long ldx, const int* active, int p, long long colstride, int nactive) {
+                    long long ldx, const int* active, int p, long long colstride, int nactive, bool graph) {
   if (p < 1 || p > PMAX) throw ArgError("trisolve: 1 <= p <= 16 right-hand sides");
   SolveArgs sa;
   sa.sn = P.d_sn.as<DevSn>();
@@ -1340,16 +1393,40 @@ void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs&
   // a complex-FP32 factor entry is 8 + 4 B instead of 16 + 4
   const double lb = bb.fp32 ? l_bytes * 12.0 / 20.0 : l_bytes;
   KTimer t(ctx, bb.fp32 ? "trisolve_fp32" : "trisolve", (lb + p * vec_bytes) * nb, 8.0 * p * nb * l_bytes / 20.0);
-  if (bb.fp32) {
-    if (p == 1) trisolve_levels<1, float2>(ctx, P, sa, B);
-    else if (p <= 4) trisolve_levels<4, float2>(ctx, P, sa, B);
-    else if (p <= 8) trisolve_levels<8, float2>(ctx, P, sa, B);
-    else trisolve_levels<16, float2>(ctx, P, sa, B);
+  auto record = [&] {
+    if (bb.fp32) {
+      if (p == 1) trisolve_levels<1, float2>(ctx, P, sa, B);
+      else if (p <= 4) trisolve_levels<4, float2>(ctx, P, sa, B);
+      else if (p <= 8) trisolve_levels<8, float2>(ctx, P, sa, B);
+      else trisolve_levels<16, float2>(ctx, P, sa, B);
+    } else {
+      if (p == 1) trisolve_levels<1, double2>(ctx, P, sa, B);
+      else if (p <= 4) trisolve_levels<4, double2>(ctx, P, sa, B);
+      else if (p <= 8) trisolve_levels<8, double2>(ctx, P, sa, B);
+      else trisolve_levels<16, double2>(ctx, P, sa, B);
+    }
+  };
+  if (graph) {
+    std::string key("trisolve");
+    key_add(key, P.serial);
+    key_add(key, sa.sn);  // the argument block field by field (it ends in padding)
+    key_add(key, sa.struct_nodes);
+    key_add(key, sa.relmap);
+    key_add(key, sa.invmap);
+    key_add(key, sa.L);
+    key_add(key, sa.T);
+    key_add(key, sa.PE);
+    key_add(key, sa.TE);
+    key_add(key, sa.X);
+    key_add(key, sa.ldx);
+    key_add(key, sa.n);
+    key_add(key, sa.active);
+    key_add(key, sa.p);
+    key_add(key, B);
+    key_add(key, bb.fp32);
+    run_captured(ctx, key, record);
   } else {
-    if (p == 1) trisolve_levels<1, double2>(ctx, P, sa, B);
-    else if (p <= 4) trisolve_levels<4, double2>(ctx, P, sa, B);
-    else if (p <= 8) trisolve_levels<8, double2>(ctx, P, sa, B);
-    else trisolve_levels<16, double2>(ctx, P, sa, B);
+    record();
   }
   TM_HIP(hipGetLastError());
 }

@@ -73,6 +73,7 @@ struct LevelPlan {
 };
 
 struct Plan {
+  uint64_t serial = 0;     // distinct for every plan built in the process (graph keys)
   int nb = 32;             // diagonal block width
   int tile = 64;           // Schur tile
   int kw = 128;            // super-panel width: depth of the trailing Schur update
@@ -128,19 +129,9 @@ struct Operator {
   DevBuf real_slot;  // per slot: K(k), M(k) blocks real and unpinned for every harmonic (k_real_slots)
   DevBuf real_pack;  // per slot: Kc, dKc, Mc packed for the split K/M product (k_pack_real_slots)
   std::vector<double> u_static;       // 3*nn at unit omega
-  // harmonic -> complex M(k) blocks, built on the stream of their first user;
-  // `ready` marks the end of the build for the other streams
-  struct MkEntry {
-    DevBuf b;
-    hipEvent_t ready = nullptr;
-    MkEntry() = default;
-    MkEntry(MkEntry&& o) noexcept : b(std::move(o.b)), ready(o.ready) { o.ready = nullptr; }
-    MkEntry(const MkEntry&) = delete;
-    ~MkEntry() {
-      if (ready) (void)hipEventDestroy(ready);
-    }
-  };
-  std::map<int, MkEntry> Mk_cache;
+  // harmonic -> complex M(k) blocks, built on the stream of their first user, which
+  // waits for the build before it enters them
+  std::map<int, DevBuf> Mk_cache;
 };
 
 // ---- factorization (factor.hip) ------------------------------------------------
@@ -171,12 +162,15 @@ constexpr double kTinyShiftPivot = 1e-3;
 
 void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& ops,
                   const BatchBufs& bb, const int* active = nullptr, int nactive = -1,
-                  const OpArrays* ops_b = nullptr);
+                  const OpArrays* ops_b = nullptr, bool graph = false);
 // in-place solve A X = B for each problem's block X_b = X + b*ldx: p columns of
 // length n = 3 nf, column c at X_b + c*n.  bb.T must hold p * tvec_entries per problem.
 // nactive: how many problems `active` selects (-1: all B), for the byte count only.
+// graph: replay the level launches from ctx's graph cache (run_captured); for callers
+// whose argument buffers stay in place from call to call.
 void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs& bb, double2* X,
-                    long long ldx, const int* active, int p = 1, long long colstride = -1, int nactive = -1);
+                    long long ldx, const int* active, int p = 1, long long colstride = -1, int nactive = -1,
+                    bool graph = false);
 
 struct HostProblem;
 void create_operator(Context* ctx, Mesh& M, Operator& op);

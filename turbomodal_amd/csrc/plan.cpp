@@ -6,6 +6,7 @@
 // mesh; every numeric factorization and solve of every problem that shares
 // the mesh replays it.
 #include <algorithm>
+#include <atomic>
 #include <limits>
 #include <stdexcept>
 
@@ -61,6 +62,8 @@ class Pool {
 }  // namespace
 
 void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
+  static std::atomic<uint64_t> plans{0};
+  P.serial = ++plans;
   const int nsn = static_cast<int>(S.sn.size());
   std::vector<DevSn> dsn(nsn);
   int64_t poff = 0, toff = 0;
