@@ -417,8 +417,8 @@ int tm_operator_export(tm_operator* op, double rpm, int32_t harmonic, int32_t wh
       throw ArgError("which must be 0 (K), 1 (M) or 2 (G)");
     }
     const double w = rpm * 2.0 * M_PI / 60.0, s = w * w;
-    const double alpha = 2.0 * M_PI / S.N;
-    const double pre = std::cos(harmonic * alpha), pim = std::sin(harmonic * alpha);
+    double pre, pim;
+    harmonic_phase(S.N, harmonic, pre, pim);
     std::vector<int> sys_to_ref(3 * S.nf, -1);
     for (size_t q = 0; q < S.ref_free_to_sys.size(); ++q) sys_to_ref[S.ref_free_to_sys[q]] = (int)q;
     int64_t cnt = 0;

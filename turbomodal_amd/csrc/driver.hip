@@ -583,13 +583,13 @@ void herm_gen_eig(int n, const std::vector<double>& A, const std::vector<double>
 struct BatchMem {
   DevBuf L, U, T, S, status, pivinfo, params;
   int pcols = 1;
-  bool fp32 = false;  // complex-FP32 factor (TM_PREC_MIXED)
+  FactorElem elem = FactorElem::C64;
   BatchBufs bufs(const Plan&, int B) {
     BatchBufs bb;
     bb.L = L.p;
     bb.U = U.p;
     bb.S = S.p;
-    bb.fp32 = fp32;
+    bb.elem = elem;
     bb.T = T.as<double2>();
     bb.status = status.as<int>();
     bb.pivinfo = pivinfo.as<int>();
@@ -597,10 +597,10 @@ struct BatchMem {
     bb.B = B;
     return bb;
   }
-  void alloc(DevPool* pool, const Plan& P, int B, int p = 1, bool single = false) {
+  void alloc(DevPool* pool, const Plan& P, int B, int p = 1, FactorElem e = FactorElem::C64) {
     pcols = p;
-    fp32 = single;
-    const size_t es = single ? sizeof(float2) : sizeof(double2);
+    elem = e;
+    const size_t es = factor_elem_bytes(e);
     L.alloc(pool, es * P.panel_entries * B);
     U.alloc(pool, es * P.upool_entries * B);
     T.alloc(pool, sizeof(double2) * P.tvec_entries * B * p);
@@ -754,8 +754,9 @@ static void inertia_into(Context* ctx, Operator& op, double rpm, int k, double s
   Mesh& M = *op.mesh;
   const Symbolic& S = M.S;
   hipStream_t st = ctx->stream;
-  const double w = rpm * 2.0 * M_PI / 60.0, alpha = 2.0 * M_PI / S.N;
-  const ProbParam pp{std::cos(k * alpha), std::sin(k * alpha), w * w, shift};
+  const double w = rpm * 2.0 * M_PI / 60.0;
+  ProbParam pp{1.0, 0.0, w * w, shift};
+  harmonic_phase(S.N, k, pp.pre, pp.pim);
   hipLaunchKernelGGL(k_inertia_start, dim3(1), dim3(64), 0, st, const_cast<ProbParam*>(bb.params), pp, bb.status);
   factor_batch(ctx, M, M.plan, op_arrays(op), bb);
   int info[3] = {0, 0, 0};
@@ -826,12 +827,13 @@ const double2* harmonic_mass(Context* ctx, Operator& op, int k) {
   Mesh& M = *op.mesh;
   const Symbolic& S = M.S;
   DevBuf b(sizeof(double2) * 9 * op.nblk);
-  double alpha = 2.0 * M_PI / S.N;
+  double pre, pim;
+  harmonic_phase(S.N, k, pre, pim);
   hipLaunchKernelGGL(k_build_phase_blocks, dim3((9 * (long long)op.nblk + 255) / 256), dim3(256), 0, ctx->stream,
                      (long long)op.nblk, M.d_slot_row.as<int>(), M.d_colidx.as<int>(), M.d_tslot.as<int>(),
                      M.d_pinned.as<unsigned char>(), op.Mc.as<double>(), op.Mp.as<double>(),
-                     (const double*)nullptr, (const double*)nullptr, 0.0, std::cos(k * alpha),
-                     std::sin(k * alpha), b.as<double2>(), 0, make_double2(1.0, 0.0));
+                     (const double*)nullptr, (const double*)nullptr, 0.0, pre, pim, b.as<double2>(), 0,
+                     make_double2(1.0, 0.0));
   TM_HIP(hipGetLastError());
   TM_HIP(hipStreamSynchronize(ctx->stream));
   const double2* p = b.as<double2>();
@@ -887,6 +889,12 @@ struct HostProblem {
   int factor_flags = 0;
 };
 
+// the real factor for real harmonics (TMHIP_REAL_FACTOR=0 in the environment: the complex one)
+static bool real_factor_enabled() {
+  const char* e = std::getenv("TMHIP_REAL_FACTOR");
+  return !(e && std::strcmp(e, "0") == 0);
+}
+
 static const char* factor_message(int flags) {
   return (flags & 1) ? "zero or non-finite pivot in the shift-invert factorization"
                      : "near-singular pivot in the shift-invert factorization (the shift is on an eigenvalue)";
@@ -896,8 +904,7 @@ static void fill_policy(const Symbolic& S, HostProblem& hp) {
   const int N = S.N, max_k = N / 2;
   hp.k = hp.in.harmonic;
   const double alpha = 2.0 * M_PI / N;
-  hp.pre = std::cos(hp.k * alpha);
-  hp.pim = std::sin(hp.k * alpha);
+  harmonic_phase(N, hp.k, hp.pre, hp.pim);
   hp.omega = hp.in.rpm * 2.0 * M_PI / 60.0;
   hp.s = hp.omega * hp.omega;
   hp.degenerate = hp.k > 0 && !(N % 2 == 0 && hp.k == max_k);
@@ -2269,20 +2276,22 @@ class BatchLanczos {
  public:
   // Allocates the workspaces of `capacity` slots for problems of at most nevmax
   // wanted pairs (one block size).
-  void slot_setup(int capacity, int nevmax, bool qep = false, bool mixed = false) {
+  // elem: the factor's element type (FactorElem::C32 is TM_PREC_MIXED; R64 takes real
+  // harmonics only)
+  void slot_setup(int capacity, int nevmax, bool qep = false, FactorElem elem = FactorElem::C64) {
     HostTimer h(ctx_, "setup");
     B_ = capacity;
     probs_.assign(B_, nullptr);
     n_ = 3LL * S_.nf;
     qep_ = qep;
-    mixed_ = mixed;
+    mixed_ = elem == FactorElem::C32;
     nvec_ = qep ? 2 * n_ : n_;
     cs_ = nvec_;
     nevmax_ = nevmax;
     ncvmax_ = 1;
     nchunk_ = static_cast<int>((nvec_ + kDotChunk - 1) / kDotChunk);
     const int pcols = std::min(16, std::max(block_width(nevmax_, qep_), nevmax_));
-    bm_.alloc(pool(), P_, B_, pcols, mixed_);
+    bm_.alloc(pool(), P_, B_, pcols, elem);
     act_.alloc(pool(), sizeof(int) * B_);
     gmask_.alloc(pool(), sizeof(int) * B_);
     gact_.alloc(pool(), sizeof(int) * B_);
@@ -2320,6 +2329,8 @@ class BatchLanczos {
         h_params_[b] = {hp->pre, hp->pim, hp->s, qep_ ? 0.0 : hp->in.shift};
         Operator& sop = *static_cast<Operator*>(hp->in.op);
         if (sop.mesh != &M_) throw ArgError("admit: problem of another mesh");
+        if (bm_.elem == FactorElem::R64 && !real_harmonic(S_.N, hp->k))
+          throw std::logic_error("admit: a complex harmonic in a real-factor slot group");
         mk_[b] = harmonic_mass(ctx_, sop, hp->k);
         h_ops_[b] = op_arrays(sop);
         nv_[b] = 0;
@@ -2457,10 +2468,11 @@ class BatchLanczos {
   // the slots between admissions (best when the whole queue is known: the
   // factorizations stay fully batched); smaller values let problems that
   // converge early hand their slots on.
-  void run_slots(std::vector<HostProblem*> queue, int capacity, int refill, bool qep = false, bool mixed = false) {
+  void run_slots(std::vector<HostProblem*> queue, int capacity, int refill, bool qep = false,
+                 FactorElem elem = FactorElem::C64) {
     int nevmax = 1;
     for (auto* hp : queue) nevmax = std::max(nevmax, hp->nev);
-    slot_setup(capacity, nevmax, qep, mixed);
+    slot_setup(capacity, nevmax, qep, elem);
     warm_rpms_.clear();
     warm_store_.clear();
     warm_admitted_.clear();
@@ -2922,7 +2934,10 @@ static int64_t problem_footprint(const Operator& op, const tm_problem& p) {
   const bool block = use_block(S, hp.in);
   const int pb = block_width(hp.nev, hp.qep);
   const int pcols = std::min(16, std::max(block ? pb : 1, hp.nev));
-  const int64_t es = p.precision == TM_PREC_MIXED ? 8 : 16;  // factor entry: complex FP32 or FP64
+  // factor entry: complex FP32 or real FP64 8 bytes, complex FP64 16
+  const bool real = block && !hp.qep && p.precision != TM_PREC_MIXED && real_factor_enabled() &&
+                    real_harmonic(S.N, hp.k);
+  const int64_t es = p.precision == TM_PREC_MIXED || real ? 8 : 16;
   int64_t bytes = es * (P.panel_entries + P.upool_entries + P.inv_scratch_entries) + 16 * pcols * P.tvec_entries;
   if (block) {
     const int mmax = basis_length(pb, hp.qep, hp.nev);
@@ -3016,13 +3031,21 @@ static void merge_stats(Context* ctx, Context* g) {
 // overlap the other's kernels; the slot capacities share the memory budget in
 // proportion to the runs' sizes.  One run (or TMHIP_SERIAL_GROUPS=1): one after
 // another on the caller's context.
-static void run_block_groups(Context* ctx, Operator& op, std::vector<std::vector<HostProblem*>>& runs, int64_t per,
-                             bool qep, bool mixed) {
-  per = std::max<int64_t>(per, 1);
-  // a warm-started run takes at most a third of its problems per wave, so that its later
-  // waves find adjacent rpm points solved
+static void run_block_groups(Context* ctx, Operator& op, std::vector<std::vector<HostProblem*>>& runs,
+                             bool qep, const std::vector<FactorElem>& elems) {
+  // device bytes per problem of each run: the largest footprint of its problems
+  std::vector<int64_t> per(runs.size(), 1);
+  for (size_t g = 0; g < runs.size(); ++g)
+    for (const HostProblem* hp : runs[g]) per[g] = std::max<int64_t>(per[g], problem_footprint(op, hp->in));
+  // a warm-started run of at least 3 x 32 problems takes at most a third of them per wave,
+  // so that its later waves find adjacent rpm points solved.  Smaller runs stay in one
+  // wave: a sweep launch over fewer than about 32 problems takes nearly as long as over
+  // more (the upper tree levels are latency-bound), so three waves would cost more sweeps
+  // than continuation saves (measured on the 40 real-harmonic problems of the bladed
+  // sector's 20-point sweep)
   auto wave_cap = [&](const std::vector<HostProblem*>& q, int64_t cap) {
-    if (!qep && std::any_of(q.begin(), q.end(), [](const HostProblem* h) { return h->in.warm_start != 0; }))
+    if (!qep && q.size() >= 96 &&
+        std::any_of(q.begin(), q.end(), [](const HostProblem* h) { return h->in.warm_start != 0; }))
       cap = std::min<int64_t>(cap, std::max<int64_t>(1, ((int64_t)q.size() + 2) / 3));
     return (int)cap;
   };
@@ -3031,20 +3054,22 @@ static void run_block_groups(Context* ctx, Operator& op, std::vector<std::vector
   size_t free_b = 0, total_b = 0;
   if (serial) {
     for (auto& g : ctx->groups) g->pool.trim();  // cached there, of no use here
-    for (auto& q : runs) {
+    for (size_t g = 0; g < runs.size(); ++g) {
+      auto& q = runs[g];
       TM_HIP(hipMemGetInfo(&free_b, &total_b));
       // blocks cached by the context's pool are reusable by this call
       const int64_t budget = (int64_t)(ctx->batch_fraction * (double)(free_b + ctx->pool.cached()));
-      const int cap = wave_cap(q, std::max<int64_t>(1, std::min<int64_t>({64, (int64_t)q.size(), budget / per})));
+      const int cap = wave_cap(q, std::max<int64_t>(1, std::min<int64_t>({64, (int64_t)q.size(), budget / per[g]})));
       BatchLanczos bl(ctx, op, {}, true);
-      bl.run_slots(q, cap, cap, qep, mixed);
+      bl.run_slots(q, cap, cap, qep, elems[g]);
     }
     return;
   }
   ctx->pool.trim();
-  size_t total = 0;
+  // the memory budget is shared in proportion to the runs' total footprints
+  double total = 0;
   int64_t cached = 0;
-  for (auto& q : runs) total += q.size();
+  for (size_t g = 0; g < runs.size(); ++g) total += (double)runs[g].size() * (double)per[g];
   std::vector<Context*> gc(runs.size());
   for (size_t g = 0; g < runs.size(); ++g) {
     gc[g] = group_context(ctx, g);
@@ -3059,12 +3084,12 @@ static void run_block_groups(Context* ctx, Operator& op, std::vector<std::vector
       Context* c = gc[g];
       try {
         TM_HIP(hipSetDevice(c->device));
-        const int64_t share = (int64_t)(budget * (double)runs[g].size() / (double)total);
+        const int64_t share = (int64_t)(budget * (double)runs[g].size() * (double)per[g] / total);
         const int cap =
-            wave_cap(runs[g], std::max<int64_t>(1, std::min<int64_t>({64, (int64_t)runs[g].size(), share / per})));
+            wave_cap(runs[g], std::max<int64_t>(1, std::min<int64_t>({64, (int64_t)runs[g].size(), share / per[g]})));
         {
           BatchLanczos bl(c, op, {}, true);
-          bl.run_slots(runs[g], cap, cap, qep, mixed);
+          bl.run_slots(runs[g], cap, cap, qep, elems[g]);
         }
         if (c->deliver) c->deliver->drain();
       } catch (...) {
@@ -3176,15 +3201,21 @@ static std::vector<int> solve_problems_impl(Context* ctx, const tm_problem* prob
     if (key.second >= 1) {
       // block method: one run per block size (the Coriolis group is one run)
       const bool qep = key.second == 2;
+      // a real harmonic on the (K, M) pencil in FP64 takes the real factor
+      // (TMHIP_REAL_FACTOR=0: the complex one)
+      const auto real = [&](const HostProblem* h) {
+        return key.second == 1 && real_factor_enabled() && real_harmonic(op->mesh->S.N, h->k);
+      };
+      std::stable_partition(list.begin(), list.end(), [&](const HostProblem* h) { return !real(h); });
       std::vector<std::vector<HostProblem*>> runs;
       for (size_t i = 0; i < list.size();) {
         size_t j = i;
         const int pb_i = block_width(list[i]->nev, qep);
         const int mm_i = basis_length(pb_i, qep, list[i]->nev);
-        // one block size and one basis length per run: a problem's arithmetic does not
-        // depend on what shares its run
+        // one block size, basis length and factor type per run: a problem's arithmetic
+        // does not depend on what shares its run
         while (j < list.size() && block_width(list[j]->nev, qep) == pb_i &&
-               basis_length(pb_i, qep, list[j]->nev) == mm_i)
+               basis_length(pb_i, qep, list[j]->nev) == mm_i && real(list[j]) == real(list[i]))
           ++j;
         runs.emplace_back(list.begin() + i, list.begin() + j);
         i = j;
@@ -3233,7 +3264,10 @@ static std::vector<int> solve_problems_impl(Context* ctx, const tm_problem* prob
         runs[big].swap(a);
         runs.insert(runs.begin() + big + 1, std::move(b));
       }
-      run_block_groups(ctx, *op, runs, per, qep, key.second == 3);
+      std::vector<FactorElem> elems;
+      for (const auto& r : runs)
+        elems.push_back(key.second == 3 ? FactorElem::C32 : (real(r[0]) ? FactorElem::R64 : FactorElem::C64));
+      run_block_groups(ctx, *op, runs, qep, elems);
       continue;
     }
     size_t free_b = 0, total_b = 0;

@@ -57,19 +57,44 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
-// element type of a factor: complex FP64 (double2) or complex FP32 (float2)
+// Entry of a real factor (a real harmonic, k = 0 or N/2 with N even, whose A(k) is
+// real symmetric, as the reference's real path: src/cyclic_solver.cpp:673-723): 8 bytes,
+// read by the same kernels as a complex entry.  Its imaginary part `y` takes no
+// storage and reads as zero; what is written to it is dropped, which is exact here:
+// every value the kernels write there is the zero imaginary part of a product or sum
+// of real entries (the phase of a real harmonic is exactly +-1).
+struct ZeroIm {
+  __host__ __device__ operator double() const { return 0.0; }
+  __host__ __device__ ZeroIm& operator=(double) { return *this; }
+};
+struct rdouble {
+  double x;
+  [[no_unique_address]] ZeroIm y;
+};
+static_assert(sizeof(rdouble) == sizeof(double), "a real factor entry is one double");
+__device__ __forceinline__ rdouble cmul(rdouble a, rdouble b) { return {a.x * b.x, {}}; }
+__device__ __forceinline__ rdouble cmulc(rdouble a, rdouble b) { return {a.x * b.x, {}}; }
+__device__ __forceinline__ rdouble cadd(rdouble a, rdouble b) { return {a.x + b.x, {}}; }
+__device__ __forceinline__ rdouble csub(rdouble a, rdouble b) { return {a.x - b.x, {}}; }
+
+// element type of a factor: complex FP64 (double2), complex FP32 (float2) or real FP64
 template <class CT> struct real_of;
 template <> struct real_of<double2> { using type = double; };
 template <> struct real_of<float2> { using type = float; };
+template <> struct real_of<rdouble> { using type = double; };
 template <class CT> using real_t = typename real_of<CT>::type;
+template <class CT> constexpr bool is_real_elem = false;
+template <> constexpr bool is_real_elem<rdouble> = true;
 template <class CT> __device__ __forceinline__ CT mk(double re, double im);
 template <> __device__ __forceinline__ double2 mk<double2>(double re, double im) { return make_double2(re, im); }
 template <> __device__ __forceinline__ float2 mk<float2>(double re, double im) {
   return make_float2((float)re, (float)im);
 }
+template <> __device__ __forceinline__ rdouble mk<rdouble>(double re, double) { return {re, {}}; }
 // a factor entry as complex FP64 (the triangular sweeps accumulate in FP64)
 __device__ __forceinline__ double2 ldc(double2 v) { return v; }
 __device__ __forceinline__ double2 ldc(float2 v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ double2 ldc(rdouble v) { return make_double2(v.x, 0.0); }
 // acc += a b and acc += conj(a) b as four FMAs each
 __device__ __forceinline__ void cmac(double2& acc, double2 a, double2 b) {
   acc.x = fma(a.x, b.x, acc.x);
@@ -89,6 +114,7 @@ __device__ __forceinline__ void cmac(float2& acc, float2 a, float2 b) {
   acc.y = fmaf(a.x, b.y, acc.y);
   acc.y = fmaf(a.y, b.x, acc.y);
 }
+__device__ __forceinline__ void cmac(rdouble& acc, rdouble a, rdouble b) { acc.x = fma(a.x, b.x, acc.x); }
 
 // front element (i, j), i >= j: panel for pivot columns, update matrix otherwise
 template <class CT>
@@ -296,7 +322,7 @@ __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __re
 #pragma unroll
         for (int o = 1; o < 8; o <<= 1) {
           v.x += __shfl_xor(v.x, o, 64);
-          v.y += __shfl_xor(v.y, o, 64);
+          if constexpr (!is_real_elem<CT>) v.y += __shfl_xor(v.y, o, 64);
         }
         if (l8 == 0) Li[i][c] = v;
         __builtin_amdgcn_s_waitcnt(0xc07f);
@@ -358,9 +384,11 @@ __global__ __launch_bounds__(256) void k_panel_trsm(FactorArgs fa, const RowTask
 #pragma unroll
     for (int k = 0; k < NBMAX; ++k) {  // v += x_k conj(L_jk)
       v.x = fma(x[k].x, Li[j][k].x, v.x);
-      v.x = fma(x[k].y, Li[j][k].y, v.x);
-      v.y = fma(x[k].y, Li[j][k].x, v.y);
-      v.y = fma(-x[k].x, Li[j][k].y, v.y);
+      if constexpr (!is_real_elem<CT>) {
+        v.x = fma(x[k].y, Li[j][k].y, v.x);
+        v.y = fma(x[k].y, Li[j][k].x, v.y);
+        v.y = fma(-x[k].x, Li[j][k].y, v.y);
+      }
     }
     const RT inv = RT(1) / dk[j];
     Pn[i + (long long)(kb + j) * m] = mk<CT>(v.x * inv, v.y * inv);
@@ -491,9 +519,11 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
         for (int q = 0; q < 2; ++q) {
           if (!need[p][q]) continue;
           cr[p][q] = mfma16(a[p].x, bq[q].x, cr[p][q]);
-          cr[p][q] = mfma16(a[p].y, bq[q].y, cr[p][q]);
-          ci[p][q] = mfma16(a[p].y, bq[q].x, ci[p][q]);
-          ci[p][q] = mfma16(-a[p].x, bq[q].y, ci[p][q]);
+          if constexpr (!is_real_elem<CT>) {
+            cr[p][q] = mfma16(a[p].y, bq[q].y, cr[p][q]);
+            ci[p][q] = mfma16(a[p].y, bq[q].x, ci[p][q]);
+            ci[p][q] = mfma16(-a[p].x, bq[q].y, ci[p][q]);
+          }
         }
     }
   }
@@ -1303,12 +1333,19 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
   fa.ops_b = ops_b;
   InvArgs ia{P.d_sn.as<DevSn>(), bb.L, bb.S, P.panel_entries, P.inv_scratch_entries, active};
   const int B = bb.B;
-  KTimer t(ctx, bb.fp32 ? "factor_fp32" : "factor", 0, M.S.flops * (nactive >= 0 ? nactive : B));
+  // algorithmic flops: 4 sum c_j^2 real flops for a complex factor, a quarter of that
+  // for a real one
+  const bool real = bb.elem == FactorElem::R64;
+  KTimer t(ctx, bb.elem == FactorElem::C32 ? "factor_fp32" : (real ? "factor_real" : "factor"), 0,
+           M.S.flops * (real ? 0.25 : 1.0) * (nactive >= 0 ? nactive : B));
   auto record = [&] {
     if (bb.pivinfo)
       hipLaunchKernelGGL(k_pivinfo_init, dim3((B + 63) / 64), dim3(64), 0, ctx->stream, bb.pivinfo, active, B);
-    if (bb.fp32) factor_levels<float2>(ctx, P, fa, ia, B);
-    else factor_levels<double2>(ctx, P, fa, ia, B);
+    switch (bb.elem) {
+      case FactorElem::C64: factor_levels<double2>(ctx, P, fa, ia, B); break;
+      case FactorElem::C32: factor_levels<float2>(ctx, P, fa, ia, B); break;
+      case FactorElem::R64: factor_levels<rdouble>(ctx, P, fa, ia, B); break;
+    }
   };
   if (graph) {
     // every launch argument: the plan (its task tables and level structure), the
@@ -1319,7 +1356,7 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
     key_add(key, fa);
     key_add(key, ia);
     key_add(key, B);
-    key_add(key, bb.fp32);
+    key_add(key, bb.elem);
     key_add(key, bb.pivinfo);
     run_captured(ctx, key, record);
   } else {
@@ -1390,20 +1427,23 @@ void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs&
   const double vec_bytes = 5.0 * 3 * M.S.nf * 16.0;
   const double l_bytes = P.trisolve_bytes_c16 - vec_bytes;
   const int nb = nactive >= 0 ? std::min(nactive, B) : B;  // problems that do the work
-  // a complex-FP32 factor entry is 8 + 4 B instead of 16 + 4
-  const double lb = bb.fp32 ? l_bytes * 12.0 / 20.0 : l_bytes;
-  KTimer t(ctx, bb.fp32 ? "trisolve_fp32" : "trisolve", (lb + p * vec_bytes) * nb, 8.0 * p * nb * l_bytes / 20.0);
+  // a complex-FP32 or real-FP64 factor entry is 8 + 4 B instead of 16 + 4
+  const bool half = bb.elem != FactorElem::C64;
+  const double lb = half ? l_bytes * 12.0 / 20.0 : l_bytes;
+  const char* name = bb.elem == FactorElem::C32 ? "trisolve_fp32" : (bb.elem == FactorElem::R64 ? "trisolve_real" : "trisolve");
+  KTimer t(ctx, name, (lb + p * vec_bytes) * nb, 8.0 * p * nb * l_bytes / 20.0);
+  auto levels = [&](auto lt) {
+    using LT = decltype(lt);
+    if (p == 1) trisolve_levels<1, LT>(ctx, P, sa, B);
+    else if (p <= 4) trisolve_levels<4, LT>(ctx, P, sa, B);
+    else if (p <= 8) trisolve_levels<8, LT>(ctx, P, sa, B);
+    else trisolve_levels<16, LT>(ctx, P, sa, B);
+  };
   auto record = [&] {
-    if (bb.fp32) {
-      if (p == 1) trisolve_levels<1, float2>(ctx, P, sa, B);
-      else if (p <= 4) trisolve_levels<4, float2>(ctx, P, sa, B);
-      else if (p <= 8) trisolve_levels<8, float2>(ctx, P, sa, B);
-      else trisolve_levels<16, float2>(ctx, P, sa, B);
-    } else {
-      if (p == 1) trisolve_levels<1, double2>(ctx, P, sa, B);
-      else if (p <= 4) trisolve_levels<4, double2>(ctx, P, sa, B);
-      else if (p <= 8) trisolve_levels<8, double2>(ctx, P, sa, B);
-      else trisolve_levels<16, double2>(ctx, P, sa, B);
+    switch (bb.elem) {
+      case FactorElem::C64: levels(double2{}); break;
+      case FactorElem::C32: levels(float2{}); break;
+      case FactorElem::R64: levels(rdouble{}); break;
     }
   };
   if (graph) {
@@ -1423,7 +1463,7 @@ void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs&
     key_add(key, sa.active);
     key_add(key, sa.p);
     key_add(key, B);
-    key_add(key, bb.fp32);
+    key_add(key, bb.elem);
     run_captured(ctx, key, record);
   } else {
     record();

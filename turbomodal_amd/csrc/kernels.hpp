@@ -1,6 +1,8 @@
 // Launchers and device-side plan structures shared by the engine sources.
 #pragma once
 
+#include <cmath>
+
 #include "engine.hpp"
 
 namespace tme {
@@ -135,13 +137,31 @@ struct Operator {
 };
 
 // ---- factorization (factor.hip) ------------------------------------------------
+// element type of a batch's factor: complex FP64, complex FP32 (TM_PREC_MIXED) or
+// real FP64 (real harmonics, k = 0 and N/2, FP64)
+enum class FactorElem : int { C64 = 0, C32 = 1, R64 = 2 };
+
+// e^{i 2 pi k / N}, exactly 1 at k = 0 and -1 at k = N/2 (N even): the harmonics whose
+// K(k), M(k) are real (the reference builds them from P = +-1, src/cyclic_solver.cpp:673-723)
+inline bool real_harmonic(int N, int k) { return k == 0 || (N % 2 == 0 && 2 * k == N); }
+inline void harmonic_phase(int N, int k, double& re, double& im) {
+  if (real_harmonic(N, k)) {
+    re = k == 0 ? 1.0 : -1.0;
+    im = 0.0;
+  } else {
+    const double alpha = 2.0 * M_PI / N;
+    re = std::cos(k * alpha);
+    im = std::sin(k * alpha);
+  }
+}
+inline size_t factor_elem_bytes(FactorElem e) { return e == FactorElem::C64 ? 16 : 8; }
+
 struct BatchBufs {
-  // the factor's panels, update matrices and inversion scratch: complex FP64 (double2),
-  // or complex FP32 (float2) when fp32
+  // the factor's panels, update matrices and inversion scratch, of element type elem
   void* L;           // B * panel_entries
   void* U;           // B * upool_entries
   void* S;           // B * inv_scratch_entries
-  bool fp32 = false;
+  FactorElem elem = FactorElem::C64;
   double2* T;        // B * p * tvec_entries (p right-hand sides; FP64 in either case)
   int* status;       // B: bit 0 zero / non-finite pivot, bit 1 near-singular pivot
                      // (|d_j| <= kTinyPivot |a_jj|, a_jj the diagonal of A(k) before elimination)
