@@ -22,8 +22,13 @@
 // Triangular solves reuse the same tree: forward (leaves -> root) with update
 // vectors passed to the parent, diagonal scaling fused into the backward
 // sweep, backward (root -> leaves) gathering already-solved ancestor values.
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <shared_mutex>
+#include <tuple>
 
 #include "kernels.hpp"
 
@@ -403,11 +408,12 @@ __global__ __launch_bounds__(256) void k_panel_trsm(FactorArgs fa, const RowTask
 // (re re + im im into the real part, im re - re im into the imaginary part).
 // Operand layout of v_mfma_f64_16x16x4: lane l holds A[l % 16][l / 16] and
 // B[l / 16][l % 16]; result register j of lane l is C[4 j + l / 16][l % 16].
-// Columns are staged SCHUR_KC at a time (34 KB of LDS per block, so three to
-// four blocks share a CU and one's staging overlaps another's products).
+// Columns are staged KC at a time (KC = 16: 34 KB of LDS per block for complex FP64,
+// so three to four blocks share a CU and one's staging overlaps another's products);
+// every KC accumulates the same products in the same order (k ascending, four at a
+// time), so the variants give bit-identical results.
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
-constexpr int SCHUR_KC = 16;
 // one 16x16x4 real product on the matrix cores, FP64 or FP32 (FP32 accumulation)
 template <class RT> struct v4of;
 template <> struct v4of<double> { using type = v4d; };
@@ -433,7 +439,7 @@ __device__ __forceinline__ int xcd_task(int x, int n) {
   return c * q + min(c, rem) + (x >> 3);
 }
 
-template <class CT>
+template <class CT, int KC>
 __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTask* __restrict__ tasks,
                                                     int t0, int kb) {
   (void)kb;
@@ -447,8 +453,8 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
   using V4 = typename v4of<RT>::type;
   CT* Pn = static_cast<CT*>(fa.L) + b * fa.PE + s.poff;
   CT* Un = static_cast<CT*>(fa.U) + b * fa.UE + s.uoff;
-  __shared__ CT Li[TILE][SCHUR_KC + 1];
-  __shared__ CT Lj[TILE][SCHUR_KC + 1];
+  __shared__ CT Li[TILE][KC + 1];
+  __shared__ CT Lj[TILE][KC + 1];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int r0 = 32 * (wv & 1), q0 = 32 * (wv >> 1);  // this wave's rows / columns in the tile
@@ -471,12 +477,12 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
       const int gi1 = t.i0 + r0 + 16 * p + 15, gi0 = t.i0 + r0 + 16 * p, gj0 = t.j0 + q0 + 16 * q;
       need[p][q] = gi1 >= gj0 && gi0 < m && gj0 < jlim;
     }
-  // staging of one chunk: this thread's SCHUR_KC * TILE / 256 elements of each
+  // staging of one chunk: this thread's KC * TILE / 256 elements of each
   // panel, loaded one chunk ahead so the loads overlap the previous products
-  constexpr int PER = TILE * SCHUR_KC / 256;
+  constexpr int PER = TILE * KC / 256;
   CT ri[PER], rj[PER];
   auto fetch = [&](int c0) {
-    const int jb = min(SCHUR_KC, t.tk + t.kd - c0);
+    const int jb = min(KC, t.tk + t.kd - c0);
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
       const int idx = tid + 256 * e, ii = idx % TILE, k = idx / TILE;
@@ -487,12 +493,12 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
       rj[e] = in && gj < m ? Pn[gj + col] : mk<CT>(0.0, 0.0);
     }
   };
-  __shared__ RT dk[SCHUR_KC];
+  __shared__ RT dk[KC];
   fetch(t.tk);
-  for (int c0 = t.tk; c0 < t.tk + t.kd; c0 += SCHUR_KC) {
-    const int jb = min(SCHUR_KC, t.tk + t.kd - c0);
+  for (int c0 = t.tk; c0 < t.tk + t.kd; c0 += KC) {
+    const int jb = min(KC, t.tk + t.kd - c0);
     __syncthreads();
-    if (tid < SCHUR_KC) dk[tid] = tid < jb ? Pn[(c0 + tid) + (long long)(c0 + tid) * m].x : RT(0);
+    if (tid < KC) dk[tid] = tid < jb ? Pn[(c0 + tid) + (long long)(c0 + tid) * m].x : RT(0);
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
       const int idx = tid + 256 * e, ii = idx % TILE, k = idx / TILE;
@@ -500,7 +506,7 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
       Lj[ii][k] = rj[e];
     }
     __syncthreads();
-    if (c0 + SCHUR_KC < t.tk + t.kd) fetch(c0 + SCHUR_KC);
+    if (c0 + KC < t.tk + t.kd) fetch(c0 + KC);
     const int kend = (jb + 3) & ~3;  // entries past jb are zero
     for (int kk = 0; kk < kend; kk += 4) {
       const int kx = kk + l4;
@@ -1261,9 +1267,22 @@ static FactorArgs factor_args(const Mesh& M, const Plan& P, const OpArrays& ops,
   return fa;
 }
 
+// Staged depths of the Schur tiles that the autotuner chooses from, per tree level
+// (bit-identical results, different occupancy and barrier counts: depth 8 keeps 18 KB
+// of LDS per block, 16 keeps 35 KB, 32 keeps 68 KB with two blocks per CU).
+constexpr int kSchurDepths[] = {8, 16, 32};
+constexpr int kSchurDefault = 1;  // depth 16
+
+// One factorization's launches; `depth` gives per tree level an index into
+// kSchurDepths (null: the default), and `marks` (when given, levels + 1 events) are
+// recorded at the level boundaries.
 template <class CT>
-static void factor_levels(Context* ctx, const Plan& P, const FactorArgs& fa, const InvArgs& ia, int B) {
-  for (const LevelPlan& L : P.levels) {
+static void factor_levels(Context* ctx, const Plan& P, const FactorArgs& fa, const InvArgs& ia, int B,
+                          const std::vector<int8_t>* depth = nullptr, hipEvent_t* marks = nullptr) {
+  if (marks) TM_HIP(hipEventRecord(marks[0], ctx->stream));
+  for (size_t h = 0; h < P.levels.size(); ++h) {
+    const LevelPlan& L = P.levels[h];
+    const int dv = depth ? (*depth)[h] : kSchurDefault;
     if (L.nasm)
       hipLaunchKernelGGL(k_front_assemble<CT>, dim3(L.nasm, B), dim3(256), 0, ctx->stream, fa,
                          P.d_asm.as<AsmTask>(), L.asm0);
@@ -1274,9 +1293,17 @@ static void factor_levels(Context* ctx, const Plan& P, const FactorArgs& fa, con
       if (st.ntrsm)
         hipLaunchKernelGGL(k_panel_trsm<CT>, dim3(st.ntrsm, B), dim3(256), 0, ctx->stream, fa,
                            P.d_row.as<RowTask>(), st.trsm0, st.kb);
-      if (st.ntile)
-        hipLaunchKernelGGL(k_schur_tile<CT>, dim3(st.ntile, B), dim3(256), 0, ctx->stream, fa,
-                           P.d_tile.as<TileTask>(), st.tile0, st.kb);
+      if (st.ntile) {
+        if (kSchurDepths[dv] == 8)
+          hipLaunchKernelGGL((k_schur_tile<CT, 8>), dim3(st.ntile, B), dim3(256), 0, ctx->stream, fa,
+                             P.d_tile.as<TileTask>(), st.tile0, st.kb);
+        else if (kSchurDepths[dv] == 32)
+          hipLaunchKernelGGL((k_schur_tile<CT, 32>), dim3(st.ntile, B), dim3(256), 0, ctx->stream, fa,
+                             P.d_tile.as<TileTask>(), st.tile0, st.kb);
+        else
+          hipLaunchKernelGGL((k_schur_tile<CT, 16>), dim3(st.ntile, B), dim3(256), 0, ctx->stream, fa,
+                             P.d_tile.as<TileTask>(), st.tile0, st.kb);
+      }
     }
     for (const InvStep& iv : L.inv) {
       hipLaunchKernelGGL(k_inv_diag<CT>, dim3(iv.ndiag, B), dim3(256), 0, ctx->stream, ia, P.d_inv_diag.as<int>(),
@@ -1290,7 +1317,88 @@ static void factor_levels(Context* ctx, const Plan& P, const FactorArgs& fa, con
       hipLaunchKernelGGL(k_inv_store<CT>, dim3(iv.ndiag, B), dim3(256), 0, ctx->stream, ia, P.d_inv_diag.as<int>(),
                          iv.diag0, iv.J);
     }
+    if (marks) TM_HIP(hipEventRecord(marks[h + 1], ctx->stream));
   }
+}
+
+// ---- the frontal-update autotuner ------------------------------------------
+// The Schur tiles' staged depth is chosen per tree level by timing: the first batched
+// factorization of a (plan, element type, batch size class) runs once with each
+// depth at every level, timed per level with events, and every level keeps its
+// fastest depth (the default unless another is more than 1 % faster).  The results do
+// not depend on the choice.  While it times, the tuner holds g_launch_mu alone and the
+// device is idle otherwise (every launch sequence of the engine holds it shared), so
+// other slot groups' kernels do not enter the timings.  TMHIP_AUTOTUNE=0: the default
+// depth everywhere; TMHIP_TRACE: the timings and choices on stderr.
+namespace {
+std::shared_mutex g_launch_mu;
+std::mutex g_tune_mu;
+std::map<std::tuple<uint64_t, int, int>, std::vector<int8_t>> g_tune;
+
+template <class CT>
+std::vector<int8_t> tune_levels(Context* ctx, const Plan& P, const FactorArgs& fa, const InvArgs& ia, int B) {
+  const size_t nl = P.levels.size();
+  constexpr int nv = sizeof(kSchurDepths) / sizeof(kSchurDepths[0]);
+  std::vector<hipEvent_t> ev((nl + 1) * nv, nullptr);
+  for (auto& e : ev) TM_HIP(hipEventCreate(&e));
+  std::vector<std::vector<float>> ms(nv, std::vector<float>(nl, 0.0f));
+  try {
+    for (int v = 0; v < nv; ++v) {
+      const std::vector<int8_t> all(nl, (int8_t)v);
+      factor_levels<CT>(ctx, P, fa, ia, B, &all, &ev[(nl + 1) * v]);
+    }
+    TM_HIP(hipGetLastError());
+    TM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int v = 0; v < nv; ++v)
+      for (size_t h = 0; h < nl; ++h)
+        TM_HIP(hipEventElapsedTime(&ms[v][h], ev[(nl + 1) * v + h], ev[(nl + 1) * v + h + 1]));
+  } catch (...) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    throw;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  std::vector<int8_t> pick(nl, (int8_t)kSchurDefault);
+  const bool trace = std::getenv("TMHIP_TRACE") != nullptr;
+  for (size_t h = 0; h < nl; ++h) {
+    int best = kSchurDefault;
+    for (int v = 0; v < nv; ++v)
+      if (ms[v][h] < ms[best][h]) best = v;
+    if (ms[best][h] < 0.99f * ms[kSchurDefault][h]) pick[h] = (int8_t)best;
+    if (trace) {
+      std::fprintf(stderr, "[tmhip] tune B=%d level %zu:", B, h);
+      for (int v = 0; v < nv; ++v) std::fprintf(stderr, " depth %d %.3f ms", kSchurDepths[v], ms[v][h]);
+      std::fprintf(stderr, " -> %d\n", kSchurDepths[pick[h]]);
+    }
+  }
+  return pick;
+}
+}  // namespace
+
+// The per-level depths for this factorization, tuned on first use of its key.
+template <class CT>
+static std::vector<int8_t> schur_depths(Context* ctx, const Plan& P, const FactorArgs& fa, const InvArgs& ia,
+                                        int B, FactorElem elem) {
+  const char* env = std::getenv("TMHIP_AUTOTUNE");
+  if (env && std::strcmp(env, "0") == 0) return std::vector<int8_t>(P.levels.size(), (int8_t)kSchurDefault);
+  int bucket = 1;
+  while (bucket < B) bucket <<= 1;
+  const auto key = std::make_tuple(P.serial, (int)elem, bucket);
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) return it->second;
+  }
+  std::unique_lock<std::shared_mutex> alone(g_launch_mu);
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) return it->second;
+  }
+  TM_HIP(hipDeviceSynchronize());  // no other group's work in the timings
+  std::vector<int8_t> pick = tune_levels<CT>(ctx, P, fa, ia, B);
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune[key] = pick;
+  return pick;
 }
 
 void run_captured(Context* ctx, const std::string& key, const std::function<void()>& record) {
@@ -1336,15 +1444,26 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
   // algorithmic flops: 4 sum c_j^2 real flops for a complex factor, a quarter of that
   // for a real one
   const bool real = bb.elem == FactorElem::R64;
+  // the block method's batches: per-level Schur depths from the autotuner (tuned on
+  // first use, before the timed region); others: the default
+  std::vector<int8_t> depth(P.levels.size(), (int8_t)kSchurDefault);
+  if (graph) {
+    switch (bb.elem) {
+      case FactorElem::C64: depth = schur_depths<double2>(ctx, P, fa, ia, B, bb.elem); break;
+      case FactorElem::C32: depth = schur_depths<float2>(ctx, P, fa, ia, B, bb.elem); break;
+      case FactorElem::R64: depth = schur_depths<rdouble>(ctx, P, fa, ia, B, bb.elem); break;
+    }
+  }
   KTimer t(ctx, bb.elem == FactorElem::C32 ? "factor_fp32" : (real ? "factor_real" : "factor"), 0,
            M.S.flops * (real ? 0.25 : 1.0) * (nactive >= 0 ? nactive : B));
+  std::shared_lock<std::shared_mutex> shared(g_launch_mu);
   auto record = [&] {
     if (bb.pivinfo)
       hipLaunchKernelGGL(k_pivinfo_init, dim3((B + 63) / 64), dim3(64), 0, ctx->stream, bb.pivinfo, active, B);
     switch (bb.elem) {
-      case FactorElem::C64: factor_levels<double2>(ctx, P, fa, ia, B); break;
-      case FactorElem::C32: factor_levels<float2>(ctx, P, fa, ia, B); break;
-      case FactorElem::R64: factor_levels<rdouble>(ctx, P, fa, ia, B); break;
+      case FactorElem::C64: factor_levels<double2>(ctx, P, fa, ia, B, &depth); break;
+      case FactorElem::C32: factor_levels<float2>(ctx, P, fa, ia, B, &depth); break;
+      case FactorElem::R64: factor_levels<rdouble>(ctx, P, fa, ia, B, &depth); break;
     }
   };
   if (graph) {
@@ -1358,6 +1477,7 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
     key_add(key, B);
     key_add(key, bb.elem);
     key_add(key, bb.pivinfo);
+    key.append(reinterpret_cast<const char*>(depth.data()), depth.size());
     run_captured(ctx, key, record);
   } else {
     record();
@@ -1432,6 +1552,7 @@ void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs&
   const double lb = half ? l_bytes * 12.0 / 20.0 : l_bytes;
   const char* name = bb.elem == FactorElem::C32 ? "trisolve_fp32" : (bb.elem == FactorElem::R64 ? "trisolve_real" : "trisolve");
   KTimer t(ctx, name, (lb + p * vec_bytes) * nb, 8.0 * p * nb * l_bytes / 20.0);
+  std::shared_lock<std::shared_mutex> shared(g_launch_mu);  // not while the autotuner times
   auto levels = [&](auto lt) {
     using LT = decltype(lt);
     if (p == 1) trisolve_levels<1, LT>(ctx, P, sa, B);
