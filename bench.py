@@ -182,7 +182,7 @@ def run_c5(args):
 
     # saturation: the whole stream at once
     reqs = draw(args.c5_requests)
-    with capi.Server(ctx, max_slots=64) as srv:
+    with capi.Server(ctx, max_slots=args.c5_slots) as srv:
         ts = time.perf_counter()
         tickets = [srv.submit(p, want_shapes=True) for p in reqs]
         for t in tickets:
@@ -195,7 +195,7 @@ def run_c5(args):
     gaps = rng.exponential(1.0 / rate, len(reqs))
     arrival, done_at = {}, {}
     lock = threading.Lock()
-    with capi.Server(ctx, max_slots=64) as srv:
+    with capi.Server(ctx, max_slots=args.c5_slots) as srv:
         from concurrent.futures import ThreadPoolExecutor
         pool = ThreadPoolExecutor(max_workers=16)
 
@@ -219,7 +219,8 @@ def run_c5(args):
         results = [f.result() for f in futs]
         tend = time.perf_counter()
         pool.shutdown()
-    lat = np.array([done_at[t] - arrival[t] for t in arrival])
+    lat = np.array([done_at[t] - arrival[t] for t in arrival])  # in arrival order
+    last_arrival = max(arrival.values())
     served = len(reqs) / (tend - tstart)
     ok = sum(1 for r in results if r.status == 0)
     # the plain FP64 path: one request at a time, single-vector Lanczos, on a sample
@@ -238,6 +239,11 @@ def run_c5(args):
         "saturation_solves_per_s": round(sat, 3),
         "p50_latency_ms": round(1e3 * float(np.percentile(lat, 50)), 1),
         "p99_latency_ms": round(1e3 * float(np.percentile(lat, 99)), 1),
+        # a stream the server keeps up with has latencies that do not grow along it
+        "p50_latency_ms_by_arrival_quarter": [round(1e3 * float(np.percentile(q, 50)), 1)
+                                              for q in np.array_split(lat, 4)],
+        "served_rate_during_arrivals": round(sum(1 for t in arrival if done_at[t] <= last_arrival)
+                                             / (last_arrival - tstart), 3),
         "solved": ok,
         "plain_fp64_one_at_a_time": {"value": round(plain, 3), "unit": "ND solves/s", "sample": len(sample),
                                      "eigensolver": "lanczos"},
@@ -258,7 +264,8 @@ def main():
                          "the FETCH_SIZE / WRITE_SIZE counters to bytes")
     ap.add_argument("--coriolis", action="store_true",
                     help="include_coriolis=True: the gyroscopic QEP for harmonics k > 0")
-    ap.add_argument("--c5-requests", type=int, default=240)
+    ap.add_argument("--c5-requests", type=int, default=2000)
+    ap.add_argument("--c5-slots", type=int, default=64, help="slots per server group (tm_server_create max_slots)")
     ap.add_argument("--c5-plain", type=int, default=24)
     ap.add_argument("--c5-c3", action="store_true", help="include the ~330k-DOF C3 generator in the stream")
     ap.add_argument("--seed", type=int, default=2024)

@@ -815,27 +815,32 @@ void operator_inertia_in(Context* ctx, Operator& op, double rpm, int k, double s
 
 // M(k) blocks of the operator, cached per harmonic.  Operators are shared by
 // the slot groups (each on its own thread and stream): the first user builds the
-// blocks on its stream and waits for them on the host before it enters them, so
-// later users (on any stream) find them complete.  No event is shared between the
-// streams: a stream cannot wait on, nor the host query, an event last recorded on
-// a stream that is capturing a graph, which another group's may be doing.
+// blocks on the operator's own stream and waits for them (a few tens of
+// microseconds: nothing else runs there) before it enters them, so later users
+// (on any stream) find them complete.  No event is shared between the streams: a
+// stream cannot wait on, nor the host query, an event last recorded on a stream
+// that is capturing a graph, which another group's may be doing.
 const double2* harmonic_mass(Context* ctx, Operator& op, int k) {
   static std::mutex mu;
   std::lock_guard<std::mutex> g(mu);
   auto it = op.Mk_cache.find(k);
   if (it != op.Mk_cache.end()) return it->second.as<double2>();
+  if (!op.mk_stream) {
+    TM_HIP(hipSetDevice(ctx->device));
+    TM_HIP(hipStreamCreateWithFlags(&op.mk_stream, hipStreamNonBlocking));
+  }
   Mesh& M = *op.mesh;
   const Symbolic& S = M.S;
   DevBuf b(sizeof(double2) * 9 * op.nblk);
   double pre, pim;
   harmonic_phase(S.N, k, pre, pim);
-  hipLaunchKernelGGL(k_build_phase_blocks, dim3((9 * (long long)op.nblk + 255) / 256), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(k_build_phase_blocks, dim3((9 * (long long)op.nblk + 255) / 256), dim3(256), 0, op.mk_stream,
                      (long long)op.nblk, M.d_slot_row.as<int>(), M.d_colidx.as<int>(), M.d_tslot.as<int>(),
                      M.d_pinned.as<unsigned char>(), op.Mc.as<double>(), op.Mp.as<double>(),
                      (const double*)nullptr, (const double*)nullptr, 0.0, pre, pim, b.as<double2>(), 0,
                      make_double2(1.0, 0.0));
   TM_HIP(hipGetLastError());
-  TM_HIP(hipStreamSynchronize(ctx->stream));
+  TM_HIP(hipStreamSynchronize(op.mk_stream));
   const double2* p = b.as<double2>();
   op.Mk_cache.emplace(k, std::move(b));
   return p;
@@ -1880,6 +1885,7 @@ class BatchLanczos {
   bool qep_ = false;
   // TM_PREC_MIXED: complex-FP32 factor; every shift-invert solve refined in FP64
   bool mixed_ = false;
+  bool tune_ = true;  // the batched factorizations may run the Schur-tile autotuner
   long long nvec_ = 0;  // basis vector length: n, or 2n for the pencil
   long long cs_ = 0;    // column stride of the vectors handed to the SpMVs
   std::vector<DevBuf> dblk_;
@@ -2254,7 +2260,26 @@ class BatchLanczos {
     std::map<int, std::vector<int>> by_salt;
     for (int b = 0; b < B_; ++b)
       if (sel[b]) by_salt[salt[b]].push_back(b);
-    for (const auto& g : by_salt)
+    for (const auto& g : by_salt) {
+      if (g.first == 0) {  // every problem's first start block: made once per slot setup, then copied
+        const size_t bytes = sizeof(double2) * (size_t)pb_ * nvec_;
+        if (!rand0_.p) {
+          rand0_.alloc(pool(), bytes);
+          for (int c = 0; c < pb_; ++c) {
+            std::vector<double> v = random_vector(7919u * (uint32_t)(c + 1));
+            if (qep_) {  // the b half from its own seed
+              std::vector<double> v2 = random_vector(7919u * (uint32_t)(c + 1) + 31u);
+              v.insert(v.end(), v2.begin(), v2.end());
+            }
+            TM_HIP(hipMemcpyAsync(rand0_.as<double2>() + (long long)c * nvec_, v.data(), sizeof(double2) * nvec_,
+                                  hipMemcpyHostToDevice, stream()));
+            TM_HIP(hipStreamSynchronize(stream()));
+          }
+        }
+        for (int b : g.second)  // slot b's block: pb columns nvec apart, contiguous
+          TM_HIP(hipMemcpyAsync(Wb_.as<double2>() + b * ldw_, rand0_.p, bytes, hipMemcpyDeviceToDevice, stream()));
+        continue;
+      }
       for (int c = 0; c < pb_; ++c) {
         std::vector<double> v = random_vector(7919u * (uint32_t)(c + 1) + 104729u * (uint32_t)g.first);
         if (qep_) {  // the b half from its own seed
@@ -2270,10 +2295,14 @@ class BatchLanczos {
                                 hipMemcpyDeviceToDevice, stream()));
         TM_HIP(hipStreamSynchronize(stream()));
       }
+    }
   }
+  DevBuf rand0_;  // the salt-0 start block (pb columns, nvec apart), made on first use per slot setup
 
   // ---- slots: continuous batching ----------------------------------------
  public:
+  // a serving loop: factorizations use tuned tables but never tune (no device-wide pause)
+  void no_tuning() { tune_ = false; }
   // Allocates the workspaces of `capacity` slots for problems of at most nevmax
   // wanted pairs (one block size).
   // elem: the factor's element type (FactorElem::C32 is TM_PREC_MIXED; R64 takes real
@@ -2304,6 +2333,7 @@ class BatchLanczos {
     nv_.assign(B_, 0);
     y0_.assign(B_, 0);
     block_setup();
+    rand0_ = DevBuf();
   }
 
   int free_slots() const {
@@ -2319,6 +2349,11 @@ class BatchLanczos {
   void admit(std::vector<HostProblem*> adm) {
     std::vector<int> slots, mask(B_, 0);
     const auto now = std::chrono::steady_clock::now();
+    auto lap = [this, t = now](int part) mutable {
+      const auto t2 = std::chrono::steady_clock::now();
+      admit_s_[part] += std::chrono::duration<double>(t2 - t).count();
+      t = t2;
+    };
     for (int b = 0; b < B_ && !adm.empty(); ++b)
       if (!probs_[b]) {
         HostProblem* hp = adm.front();
@@ -2341,6 +2376,7 @@ class BatchLanczos {
       }
     if (!adm.empty()) throw ArgError("admit: more problems than free slots");
     if (slots.empty()) return;
+    lap(0);
     TM_HIP(hipMemcpyAsync(bm_.params.p, h_params_.data(), sizeof(ProbParam) * B_, hipMemcpyHostToDevice, stream()));
     std::vector<int> zero(B_, 0);
     TM_HIP(hipMemcpyAsync(bm_.status.p, zero.data(), sizeof(int) * B_, hipMemcpyHostToDevice, stream()));
@@ -2351,11 +2387,14 @@ class BatchLanczos {
       HostTimer h(ctx_, "factor");
       BatchBufs bb = bm_.bufs(P_, B_);
       put(gops_, h_ops_);
+      lap(1);
       factor_batch(ctx_, M_, P_, op_arrays(op_), bb, gmask_.as<int>(), static_cast<int>(slots.size()),
-                   gops_.as<OpArrays>(), true);
+                   gops_.as<OpArrays>(), true, tune_);
+      lap(2);
       std::vector<int> status(B_);
       TM_HIP(hipMemcpyAsync(status.data(), bm_.status.p, sizeof(int) * B_, hipMemcpyDeviceToHost, stream()));
       TM_HIP(hipStreamSynchronize(stream()));
+      lap(3);
       for (int b : slots) {
         probs_[b]->factor_flags = status[b];
         if (status[b]) {
@@ -2382,7 +2421,11 @@ class BatchLanczos {
         iter_[b] = 1;
         y0_[b] = nv_[b] - pb_;
       }
+    lap(4);
   }
+  // host seconds of the admissions so far: M(k) lookups, uploads and block builds,
+  // factorization launches (captures included), waiting for the factorizations, start blocks
+  double admit_s_[5] = {0, 0, 0, 0, 0};
 
   // One block step of every iterating slot: expansion, orthogonalisation,
   // Rayleigh-Ritz and convergence (finished slots stop iterating).
@@ -3300,7 +3343,8 @@ static std::vector<int> solve_problems_impl(Context* ctx, const tm_problem* prob
 // a server-wide HBM budget (a group holds at most a quarter of it).  A group
 // admits queued problems as slots free up (factorizations batched, masked to
 // the new slots), steps all its slots together, retires finished problems into
-// the caller's result buffers, and returns its memory when it runs dry.
+// the caller's result buffers, and returns its memory when it has been dry for
+// kIdleKeep or another group waits for memory.
 // Problems too small for the block method (and the Lanczos method's requests) run
 // as Lanczos batches on a worker thread of their own, with its own stream and pool,
 // so they never hold up the dispatcher's admissions.
@@ -3326,13 +3370,18 @@ struct Server {
     std::thread th;
     bool quit = false;               // guarded by srv->mu
     // counters (TMHIP_SERVER_STATS=1 prints them when the server stops)
-    long admits = 0, admitted = 0, steps = 0, slot_steps = 0;
-    double t_admit = 0, t_step = 0;
+    long admits = 0, admitted = 0, steps = 0, slot_steps = 0, setups = 0;
+    double t_admit = 0, t_step = 0, t_setup = 0;
+    double admit_s[5] = {0, 0, 0, 0, 0};  // the admissions' parts (BatchLanczos::admit_s_)
   };
 
+  // how long an idle group keeps its slots before it returns their memory
+  static constexpr std::chrono::milliseconds kIdleKeep{1000};
+  static constexpr int kMinSlots = 16;  // a group's smallest capacity (below max_slots)
   Context* ctx;
   int max_slots;
   double frac, budget = 0, reserved = 0;
+  int mem_waiters = 0;  // groups waiting for device memory (guarded by mu)
   std::thread th;
   std::mutex mu;
   std::condition_variable cv_in, cv_done, cv_group;
@@ -3437,8 +3486,19 @@ struct Server {
           std::unique_lock<std::mutex> lk(mu);
           const bool busy = g->bl && g->bl->busy();
           if (!busy && g->queue.empty()) {
-            if (g->bl) {  // ran dry: give the memory back
+            if (g->bl && !g->quit && mem_waiters == 0) {
+              // ran dry: keep the slots (their workspaces, captured launches) for a while,
+              // as the next request of a stream usually comes soon, unless another group
+              // waits for memory
+              cv_group.wait_for(lk, kIdleKeep, [&] {
+                return g->quit || !g->queue.empty() || !fatal.empty() || mem_waiters > 0;
+              });
+              if (!fatal.empty()) return;
+              if (!g->queue.empty()) continue;
+            }
+            if (g->bl) {  // still dry: give the memory back
               lk.unlock();
+              for (int q = 0; q < 5; ++q) g->admit_s[q] += g->bl->admit_s_[q];
               g->bl.reset();
               g->sub->c.pool.trim();
               lk.lock();
@@ -3456,9 +3516,18 @@ struct Server {
               per = std::max<int64_t>(per, problem_footprint(*g->rep, h->in));
             per = std::max<int64_t>(per, 1) * 21 / 20;
             const double avail = std::min(budget - reserved, 0.25 * budget);
-            const int cap = (int)std::min<double>(max_slots, std::floor(avail / (double)per));
+            // sized to the demand: twice the queue (at least kMinSlots), up to max_slots and
+            // the memory share.  Workspaces of a full 64 slots run to tens of GB for the
+            // bladed sector; allocating them whenever a group of a thin stream starts (and
+            // returning them when it runs dry) stalled the stream for seconds
+            int want = kMinSlots;
+            while (want < 2 * (int)g->queue.size()) want *= 2;
+            const int cap = (int)std::min<double>({(double)max_slots, (double)want, std::floor(avail / (double)per)});
             if (cap < 1) {  // no memory now: wait for another group to return some
+              ++mem_waiters;
+              cv_group.notify_all();  // idle groups release theirs
               cv_group.wait_for(lk, std::chrono::milliseconds(20));
+              --mem_waiters;
               continue;
             }
             g->reserved = (double)cap * per;
@@ -3468,8 +3537,12 @@ struct Server {
           }
         }
         if (need_setup) {
+          const auto ts = std::chrono::steady_clock::now();
           g->bl = std::make_unique<BatchLanczos>(&g->sub->c, *g->rep, std::vector<HostProblem*>{}, true);
+          g->bl->no_tuning();
           g->bl->slot_setup((int)per, g->pbclass == 8 ? kNarrowNev : std::max(24, g->nevclass), g->qep);
+          g->setups += 1;
+          g->t_setup += std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
         }
         {
           std::lock_guard<std::mutex> lk(mu);
@@ -3611,10 +3684,12 @@ struct Server {
       for (auto& kv : groups) {
         const Group& g = *kv.second;
         std::fprintf(stderr,
-                     "[tmhip] group nf=%d pb=%d qep=%d: %ld admissions of %ld problems (%.1f s), %ld steps "
-                     "(%.1f s, %.1f slots busy on average)\n",
-                     g.mesh->S.nf, g.pbclass, g.qep ? 1 : 0, g.admits, g.admitted, g.t_admit, g.steps, g.t_step,
-                     g.steps ? (double)g.slot_steps / g.steps : 0.0);
+                     "[tmhip] group nf=%d pb=%d qep=%d: %ld admissions of %ld problems (%.1f s: %.2f / %.2f / "
+                     "%.2f / %.2f / %.2f s), %ld steps (%.1f s, %.1f slots busy on average), %ld slot setups "
+                     "(%.1f s)\n",
+                     g.mesh->S.nf, g.pbclass, g.qep ? 1 : 0, g.admits, g.admitted, g.t_admit, g.admit_s[0],
+                     g.admit_s[1], g.admit_s[2], g.admit_s[3], g.admit_s[4], g.steps, g.t_step,
+                     g.steps ? (double)g.slot_steps / g.steps : 0.0, g.setups, g.t_setup);
       }
     groups.clear();
   }

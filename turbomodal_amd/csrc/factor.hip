@@ -1374,12 +1374,16 @@ std::vector<int8_t> tune_levels(Context* ctx, const Plan& P, const FactorArgs& f
 }
 }  // namespace
 
-// The per-level depths for this factorization, tuned on first use of its key.
+// The per-level depths for this factorization: tuned on first use of its key when
+// `tune` (a batched sweep, which can afford three extra factorizations and a moment of
+// the device alone); otherwise the table of the nearest batch class tuned so far for
+// this plan and element type, or the default (a serving loop never stops the device).
 template <class CT>
 static std::vector<int8_t> schur_depths(Context* ctx, const Plan& P, const FactorArgs& fa, const InvArgs& ia,
-                                        int B, FactorElem elem) {
+                                        int B, FactorElem elem, bool tune) {
+  const std::vector<int8_t> dflt(P.levels.size(), (int8_t)kSchurDefault);
   const char* env = std::getenv("TMHIP_AUTOTUNE");
-  if (env && std::strcmp(env, "0") == 0) return std::vector<int8_t>(P.levels.size(), (int8_t)kSchurDefault);
+  if (env && std::strcmp(env, "0") == 0) return dflt;
   int bucket = 1;
   while (bucket < B) bucket <<= 1;
   const auto key = std::make_tuple(P.serial, (int)elem, bucket);
@@ -1387,6 +1391,19 @@ static std::vector<int8_t> schur_depths(Context* ctx, const Plan& P, const Facto
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tune.find(key);
     if (it != g_tune.end()) return it->second;
+    if (!tune) {
+      const std::vector<int8_t>* best = &dflt;
+      int gap = 1 << 30;
+      for (const auto& kv : g_tune)
+        if (std::get<0>(kv.first) == P.serial && std::get<1>(kv.first) == (int)elem) {
+          const int b2 = std::get<2>(kv.first), d = b2 > bucket ? b2 / bucket : bucket / b2;
+          if (d < gap) {
+            gap = d;
+            best = &kv.second;
+          }
+        }
+      return *best;
+    }
   }
   std::unique_lock<std::shared_mutex> alone(g_launch_mu);
   {
@@ -1435,7 +1452,8 @@ void run_captured(Context* ctx, const std::string& key, const std::function<void
 }
 
 void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& ops,
-                  const BatchBufs& bb, const int* active, int nactive, const OpArrays* ops_b, bool graph) {
+                  const BatchBufs& bb, const int* active, int nactive, const OpArrays* ops_b, bool graph,
+                  bool tune) {
   FactorArgs fa = factor_args(M, P, ops, bb);
   fa.active = active;
   fa.ops_b = ops_b;
@@ -1444,15 +1462,14 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
   // algorithmic flops: 4 sum c_j^2 real flops for a complex factor, a quarter of that
   // for a real one
   const bool real = bb.elem == FactorElem::R64;
-  // the block method's batches: per-level Schur depths from the autotuner (tuned on
-  // first use, before the timed region); others: the default
-  std::vector<int8_t> depth(P.levels.size(), (int8_t)kSchurDefault);
-  if (graph) {
-    switch (bb.elem) {
-      case FactorElem::C64: depth = schur_depths<double2>(ctx, P, fa, ia, B, bb.elem); break;
-      case FactorElem::C32: depth = schur_depths<float2>(ctx, P, fa, ia, B, bb.elem); break;
-      case FactorElem::R64: depth = schur_depths<rdouble>(ctx, P, fa, ia, B, bb.elem); break;
-    }
+  // per-level Schur depths: tuned on first use of the key (tune: the batched sweeps,
+  // before the timed region), else a table tuned earlier for this plan and element type,
+  // else the default
+  std::vector<int8_t> depth;
+  switch (bb.elem) {
+    case FactorElem::C64: depth = schur_depths<double2>(ctx, P, fa, ia, B, bb.elem, tune); break;
+    case FactorElem::C32: depth = schur_depths<float2>(ctx, P, fa, ia, B, bb.elem, tune); break;
+    case FactorElem::R64: depth = schur_depths<rdouble>(ctx, P, fa, ia, B, bb.elem, tune); break;
   }
   KTimer t(ctx, bb.elem == FactorElem::C32 ? "factor_fp32" : (real ? "factor_real" : "factor"), 0,
            M.S.flops * (real ? 0.25 : 1.0) * (nactive >= 0 ? nactive : B));

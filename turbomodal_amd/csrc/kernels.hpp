@@ -131,9 +131,17 @@ struct Operator {
   DevBuf real_slot;  // per slot: K(k), M(k) blocks real and unpinned for every harmonic (k_real_slots)
   DevBuf real_pack;  // per slot: Kc, dKc, Mc packed for the split K/M product (k_pack_real_slots)
   std::vector<double> u_static;       // 3*nn at unit omega
-  // harmonic -> complex M(k) blocks, built on the stream of their first user, which
-  // waits for the build before it enters them
+  // harmonic -> complex M(k) blocks, built on the operator's own stream (which runs
+  // nothing else, so waiting for a build waits for that build only) before they are
+  // handed to any user
   std::map<int, DevBuf> Mk_cache;
+  hipStream_t mk_stream = nullptr;
+  Operator() = default;
+  Operator(const Operator&) = delete;
+  Operator& operator=(const Operator&) = delete;
+  ~Operator() {
+    if (mk_stream) (void)hipStreamDestroy(mk_stream);
+  }
 };
 
 // ---- factorization (factor.hip) ------------------------------------------------
@@ -182,7 +190,7 @@ constexpr double kTinyShiftPivot = 1e-3;
 
 void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& ops,
                   const BatchBufs& bb, const int* active = nullptr, int nactive = -1,
-                  const OpArrays* ops_b = nullptr, bool graph = false);
+                  const OpArrays* ops_b = nullptr, bool graph = false, bool tune = false);
 // in-place solve A X = B for each problem's block X_b = X + b*ldx: p columns of
 // length n = 3 nf, column c at X_b + c*n.  bb.T must hold p * tvec_entries per problem.
 // nactive: how many problems `active` selects (-1: all B), for the byte count only.
