@@ -800,7 +800,10 @@ __device__ __forceinline__ ChildPull child_pull(const SolveArgs& sa, const DevSn
 }
 
 // pivot rows of front s: x += the children's update entries that land on them
-// (child0 then child1).  The update rows are written by k_fwd_upd.
+// (child0 then child1).  The update rows are written by k_fwd_upd.  LT (the factor's
+// element type) is not read here: it names the launches of the real and the complex
+// sweeps apart in kernel traces.
+template <class LT>
 __global__ __launch_bounds__(256) void k_fwd_asm(SolveArgs sa, const int* __restrict__ level_sn, int s0) {
   const int b = blockIdx.y;
   if (sa.active && !sa.active[b]) return;
@@ -1117,6 +1120,7 @@ __global__ __launch_bounds__(256 * NG) void k_fwd_upd_split(SolveArgs sa, const 
 // backward: the solved ancestor values x[struct(q)] of every front of the level,
 // gathered once into the front's update-vector slots (free after the forward
 // sweep), so the column dot products below read them contiguously
+template <class LT>  // as k_fwd_asm: for the traces only
 __global__ __launch_bounds__(256) void k_bwd_gather(SolveArgs sa, const int* __restrict__ level_sn, int s0) {
   const int b = blockIdx.y;
   if (sa.active && !sa.active[b]) return;
@@ -1514,7 +1518,7 @@ template <int PM, class LT>
 static void trisolve_levels(Context* ctx, const Plan& P, const SolveArgs& sa, int B) {
   const int* lsn = P.d_level_sn.as<int>();
   for (const LevelPlan& L : P.levels) {
-    hipLaunchKernelGGL(k_fwd_asm, dim3(L.nsn, B), dim3(256), 0, ctx->stream, sa, lsn, L.sn0);
+    hipLaunchKernelGGL(k_fwd_asm<LT>, dim3(L.nsn, B), dim3(256), 0, ctx->stream, sa, lsn, L.sn0);
     // levels with few pivot-row tasks (the upper tree) split each task's columns
     // over four waves; the choice depends on the level only, never on the batch
     if (L.nfinv > kFwdInvSplit)
@@ -1534,7 +1538,7 @@ static void trisolve_levels(Context* ctx, const Plan& P, const SolveArgs& sa, in
   for (int h = static_cast<int>(P.levels.size()) - 1; h >= 0; --h) {
     const LevelPlan& L = P.levels[h];
     if (!L.nbcol) continue;
-    hipLaunchKernelGGL(k_bwd_gather, dim3(L.nsn, B), dim3(256), 0, ctx->stream, sa, lsn, L.sn0);
+    hipLaunchKernelGGL(k_bwd_gather<LT>, dim3(L.nsn, B), dim3(256), 0, ctx->stream, sa, lsn, L.sn0);
     hipLaunchKernelGGL((k_bwd_tile<PM, LT>), dim3(L.nbcol, B), dim3(256), 0, ctx->stream, sa, P.d_bcol.as<RowTask>(),
                        L.bcol0, 0);
     hipLaunchKernelGGL((k_bwd_tile<PM, LT>), dim3(L.nbcol, B), dim3(256), 0, ctx->stream, sa, P.d_bcol.as<RowTask>(),

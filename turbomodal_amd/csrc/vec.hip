@@ -687,10 +687,16 @@ __global__ void k_pack_real_slots(long long nslot, const double* __restrict__ Kc
 
 // Y1 = K(k) X and Y2 = M(k) X for the Hermitian pencil, as k_spmv_pair, with the
 // real slots read from the operator's packed real arrays (Kc + s dKc, Mc; 224 B
-// per slot in 14 16-byte loads, the same for every problem of the operator and so
+// per slot in 14 16-byte pieces, the same for every problem of the operator and so
 // found in L2 by the problems after the first) instead of the problem's complex blocks (288 B per
 // slot and problem).  A real slot's products are the complex path's with the zero
 // imaginary terms left out: bit-identical sums.
+// One wave per node row: lane = (slot lane sl, column lane cg), eight slots of the row
+// at a time.  The eight column lanes of a slot all need its whole record, so the
+// wave loads the eight records once, contiguous and coalesced (112 pieces over its 64
+// lanes), into its own LDS, and each lane reads its slot's record from there (a
+// broadcast to the eight column lanes): the vector memory path then returns each
+// record once instead of eight times.
 template <int CPL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_spmv_km(int nf, int B, const int* __restrict__ rowptr,
                                                  const int* __restrict__ colidx, const SplitKM* __restrict__ sk,
@@ -699,11 +705,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                  const double2* __restrict__ X, long long ldx, long long n,
                                                  const int* __restrict__ ncol, int c0, double2* __restrict__ Y1,
                                                  double2* __restrict__ Y2, long long ldy) {
+  constexpr int kRec = 14;  // 16-byte pieces per slot record
+  __shared__ double2 rec[4][8 * kRec];
   const int t = xcd_task(blockIdx.x, gridDim.x);
   const int b = t % B, I = (t / B) * 4 + (threadIdx.x >> 6);
   const int nc = min(8 * CPL, ncol[b] - c0);
   if (nc <= 0 || I >= nf) return;  // whole waves leave together
   const int lane = threadIdx.x & 63, cg = lane & 7, sl = lane >> 3;
+  double2* wrec = rec[threadIdx.x >> 6];
   const SplitKM q = sk[b];
   const double2* A1 = kblocks[b];
   const double2* A2 = mblocks[b];
@@ -713,17 +722,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   for (int j = 0; j < CPL; ++j)
 #pragma unroll
     for (int rr = 0; rr < 3; ++rr) a1[j][rr] = a2[j][rr] = make_double2(0.0, 0.0);
-  const int s1 = rowptr[I + 1];
-  if (cg < nc) {
-    int s = rowptr[I] + sl;
-    int Jn = s < s1 ? colidx[s] : 0, Rn = s < s1 ? q.real[s] : 0;
-    for (; s < s1; s += 8) {
-      const long long J3 = 3LL * Jn;
-      const int re = Rn;
-      if (s + 8 < s1) {
-        Jn = colidx[s + 8];
-        Rn = q.real[s + 8];
-      }
+  const int r0 = rowptr[I], s1 = rowptr[I + 1];
+  const bool col_on = cg < nc;
+  int Jn = r0 + sl < s1 ? colidx[r0 + sl] : 0, Rn = r0 + sl < s1 ? q.real[r0 + sl] : 0;
+  for (int base = r0; base < s1; base += 8) {  // wave-uniform trip count
+    const int s = base + sl;
+    const long long J3 = 3LL * Jn;
+    const int re = Rn;
+    if (s + 8 < s1) {
+      Jn = colidx[s + 8];
+      Rn = q.real[s + 8];
+    }
+    // the eight records of slots base .. base + 7 (those of the row) into this wave's LDS
+    const double2* src = q.pack + (long long)base * kRec;
+    const int npiece = min(8, s1 - base) * kRec;
+    for (int e = lane; e < npiece; e += 64) wrec[e] = src[e];
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // this wave's LDS writes done before any lane reads
+    __builtin_amdgcn_wave_barrier();
+    if (s < s1 && col_on) {
       double2 xv[CPL][3];
 #pragma unroll
       for (int j = 0; j < CPL; ++j) {
@@ -733,7 +749,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         for (int k = 0; k < 3; ++k) xv[j][k] = on ? xj[k] : make_double2(0.0, 0.0);
       }
       if (re) {
-        const double2* pk = q.pack + (long long)s * 14;
+        const double2* pk = wrec + sl * kRec;
         double mcv[10];
 #pragma unroll
         for (int u = 0; u < 5; ++u) {
@@ -772,6 +788,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           }
       }
     }
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // every lane's reads of the records done before the next writes
+    __builtin_amdgcn_wave_barrier();
   }
 #pragma unroll
   for (int j = 0; j < CPL; ++j)
