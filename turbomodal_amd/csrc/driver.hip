@@ -938,6 +938,13 @@ class BatchLanczos {
  public:
   BatchLanczos(Context* ctx, Operator& op, const std::vector<HostProblem*>& probs, bool block)
       : ctx_(ctx), op_(op), M_(*op.mesh), S_(M_.S), P_(M_.plan), probs_(probs), block_(block) {}
+  BatchLanczos(const BatchLanczos&) = delete;
+  BatchLanczos& operator=(const BatchLanczos&) = delete;
+  ~BatchLanczos() {
+    if (pend_side_) (void)hipStreamSynchronize(pend_side_->stream);  // it writes this batch's workspaces
+    for (hipEvent_t e : side_ev_)
+      if (e) (void)hipEventDestroy(e);
+  }
 
   void run() {
     auto t0 = std::chrono::steady_clock::now();
@@ -2342,18 +2349,59 @@ class BatchLanczos {
     return n;
   }
   bool busy() const { return free_slots() < B_; }
+  bool stepping() const { return std::any_of(iter_.begin(), iter_.end(), [](int v) { return v != 0; }); }
   int capacity() const { return B_; }
 
   // Puts problems into free slots, factors them (only them), and starts their
   // block Krylov bases; they then step with the problems already in flight.
   void admit(std::vector<HostProblem*> adm) {
-    std::vector<int> slots, mask(B_, 0);
+    if (!admit_prepare(std::move(adm))) return;
+    admit_factor(ctx_);
+    admit_complete();
+  }
+
+  // The same in two halves for a serving loop: admit_async puts the problems into
+  // free slots and launches their factorization on `side` (a context of its own:
+  // stream, graph cache), after the slots' uploads and block builds on this
+  // context's stream; the slots already in flight keep stepping meanwhile (the
+  // factorization writes only the new slots' workspaces).  admit_ready tells when it
+  // has finished, admit_finish then starts the new slots' bases.  One admission at
+  // a time; the new slots neither step nor retire before admit_finish.
+  void admit_async(std::vector<HostProblem*> adm, Context* side) {
+    if (admitting()) throw std::logic_error("admit_async: an admission is pending");
+    if (!admit_prepare(std::move(adm))) return;
+    if (!side_ev_[0]) {
+      TM_HIP(hipEventCreateWithFlags(&side_ev_[0], hipEventDisableTiming));
+      TM_HIP(hipEventCreateWithFlags(&side_ev_[1], hipEventDisableTiming));
+    }
+    TM_HIP(hipEventRecord(side_ev_[0], stream()));  // neither stream is capturing here
+    TM_HIP(hipStreamWaitEvent(side->stream, side_ev_[0], 0));
+    admit_factor(side);
+    TM_HIP(hipEventRecord(side_ev_[1], side->stream));
+    pend_side_ = side;
+  }
+  bool admitting() const { return pend_side_ != nullptr; }
+  bool admit_ready() const {
+    if (!pend_side_) return false;
+    const hipError_t q = hipEventQuery(side_ev_[1]);
+    if (q == hipErrorNotReady) return false;
+    TM_HIP(q);
+    return true;
+  }
+  void admit_finish() {
+    if (!pend_side_) return;
+    TM_HIP(hipEventSynchronize(side_ev_[1]));
+    pend_side_ = nullptr;
+    admit_complete();
+  }
+
+ private:
+  // slots, parameters, M(k) and K(k) blocks of the admitted problems (this context's stream)
+  bool admit_prepare(std::vector<HostProblem*> adm) {
+    adm_slots_.clear();
+    adm_mask_.assign(B_, 0);
     const auto now = std::chrono::steady_clock::now();
-    auto lap = [this, t = now](int part) mutable {
-      const auto t2 = std::chrono::steady_clock::now();
-      admit_s_[part] += std::chrono::duration<double>(t2 - t).count();
-      t = t2;
-    };
+    adm_t0_ = now;
     for (int b = 0; b < B_ && !adm.empty(); ++b)
       if (!probs_[b]) {
         HostProblem* hp = adm.front();
@@ -2371,31 +2419,39 @@ class BatchLanczos {
         nv_[b] = 0;
         y0_[b] = 0;
         prev_res_[b].clear();
-        slots.push_back(b);
-        mask[b] = 1;
+        adm_slots_.push_back(b);
+        adm_mask_[b] = 1;
       }
     if (!adm.empty()) throw ArgError("admit: more problems than free slots");
-    if (slots.empty()) return;
-    lap(0);
+    if (adm_slots_.empty()) return false;
+    admit_lap(0);
     TM_HIP(hipMemcpyAsync(bm_.params.p, h_params_.data(), sizeof(ProbParam) * B_, hipMemcpyHostToDevice, stream()));
     std::vector<int> zero(B_, 0);
     TM_HIP(hipMemcpyAsync(bm_.status.p, zero.data(), sizeof(int) * B_, hipMemcpyHostToDevice, stream()));
     mk_ptrs_ = upload(pool(), mk_, stream());
-    build_stiffness_blocks(slots);
-    put(gmask_, mask);
+    build_stiffness_blocks(adm_slots_);
+    put(gmask_, adm_mask_);
+    put(gops_, h_ops_);
+    admit_lap(1);
+    return true;
+  }
+  // the admitted slots' factorization on c's stream
+  void admit_factor(Context* c) {
+    HostTimer h(ctx_, "factor");
+    BatchBufs bb = bm_.bufs(P_, B_);
+    factor_batch(c, M_, P_, op_arrays(op_), bb, gmask_.as<int>(), static_cast<int>(adm_slots_.size()),
+                 gops_.as<OpArrays>(), true, tune_);
+    admit_lap(2);
+  }
+  // status of the factorizations, start blocks of the admitted slots
+  void admit_complete() {
+    std::vector<int>& mask = adm_mask_;
     {
-      HostTimer h(ctx_, "factor");
-      BatchBufs bb = bm_.bufs(P_, B_);
-      put(gops_, h_ops_);
-      lap(1);
-      factor_batch(ctx_, M_, P_, op_arrays(op_), bb, gmask_.as<int>(), static_cast<int>(slots.size()),
-                   gops_.as<OpArrays>(), true, tune_);
-      lap(2);
       std::vector<int> status(B_);
       TM_HIP(hipMemcpyAsync(status.data(), bm_.status.p, sizeof(int) * B_, hipMemcpyDeviceToHost, stream()));
       TM_HIP(hipStreamSynchronize(stream()));
-      lap(3);
-      for (int b : slots) {
+      admit_lap(3);
+      for (int b : adm_slots_) {
         probs_[b]->factor_flags = status[b];
         if (status[b]) {
           probs_[b]->failed = true;
@@ -2406,23 +2462,35 @@ class BatchLanczos {
     }
     HostTimer h(ctx_, "iterate");
     random_block(mask, std::vector<int>(B_, 0));
-    for (int b : slots) {  // rpm continuation: the adjacent point's Ritz vectors lead the start block
+    for (int b : adm_slots_) {  // rpm continuation: the adjacent point's Ritz vectors lead the start block
       const std::pair<DevBuf, int>* src = mask[b] ? warm_source(probs_[b]) : nullptr;
       if (!src) continue;
       TM_HIP(hipMemcpyAsync(Wb_.as<double2>() + b * ldw_, src->first.p, sizeof(double2) * (size_t)src->second * n_,
                             hipMemcpyDeviceToDevice, stream()));
       probs_[b]->warm_started = true;
     }
-    warm_prune(slots);
+    warm_prune(adm_slots_);
     if (qep_) smooth_block(mask);
     append_or_retry(mask);
-    for (int b : slots)
+    for (int b : adm_slots_)
       if (mask[b]) {
         iter_[b] = 1;
         y0_[b] = nv_[b] - pb_;
       }
-    lap(4);
+    admit_lap(4);
+    adm_slots_.clear();
   }
+  void admit_lap(int part) {
+    const auto t2 = std::chrono::steady_clock::now();
+    admit_s_[part] += std::chrono::duration<double>(t2 - adm_t0_).count();
+    adm_t0_ = t2;
+  }
+  std::vector<int> adm_slots_, adm_mask_;  // the admission in progress
+  std::chrono::steady_clock::time_point adm_t0_;
+  Context* pend_side_ = nullptr;           // the context of a pending asynchronous factorization
+  hipEvent_t side_ev_[2] = {nullptr, nullptr};
+
+ public:
   // host seconds of the admissions so far: M(k) lookups, uploads and block builds,
   // factorization launches (captures included), waiting for the factorizations, start blocks
   double admit_s_[5] = {0, 0, 0, 0, 0};
@@ -2486,7 +2554,7 @@ class BatchLanczos {
     std::vector<int> sel(B_, 0);
     std::vector<HostProblem*> out;
     for (int b = 0; b < B_; ++b)
-      if (probs_[b] && !iter_[b]) {
+      if (probs_[b] && !iter_[b] && !(pend_side_ && adm_mask_[b])) {
         sel[b] = 1;
         out.push_back(probs_[b]);
       }
@@ -3364,6 +3432,7 @@ struct Server {
     int nevclass = 0;  // a wanted-pair count of its basis class (any one: they share the basis length)
     bool qep = false;
     std::unique_ptr<SubContext> sub;
+    std::unique_ptr<SubContext> side;  // stream of the admissions' factorizations
     std::unique_ptr<BatchLanczos> bl;
     std::deque<HostProblem*> queue;  // guarded by srv->mu
     double reserved = 0;             // guarded by srv->mu
@@ -3544,7 +3613,11 @@ struct Server {
           g->setups += 1;
           g->t_setup += std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
         }
-        {
+        const auto ta = std::chrono::steady_clock::now();
+        // a finished admission's slots join the stepping ones; with nothing else to
+        // step, wait for it
+        if (g->bl->admitting() && (g->bl->admit_ready() || !g->bl->stepping())) g->bl->admit_finish();
+        if (!g->bl->admitting()) {
           std::lock_guard<std::mutex> lk(mu);
           const int nfree = g->bl->free_slots();
           const int refill = std::max(1, g->bl->capacity() / 8);
@@ -3555,26 +3628,27 @@ struct Server {
               g->queue.pop_front();
             }
         }
-        const auto ta = std::chrono::steady_clock::now();
         if (!adm.empty()) {
-          g->bl->admit(adm);
+          // the new slots' factorization runs beside the steps of those in flight
+          g->bl->admit_async(adm, &g->side->c);
           g->admits += 1;
           g->admitted += (long)adm.size();
         }
         const auto tb = std::chrono::steady_clock::now();
         g->t_admit += std::chrono::duration<double>(tb - ta).count();
-        if (g->bl->busy()) {
+        if (g->bl->stepping()) {
           g->steps += 1;
           g->slot_steps += g->bl->capacity() - g->bl->free_slots();
           g->bl->block_step();
           g->t_step += std::chrono::duration<double>(std::chrono::steady_clock::now() - tb).count();
-          std::vector<HostProblem*> fin = g->bl->retire();
-          if (!fin.empty())  // done once their mode shapes have landed
-            g->sub->c.deliverer().then(g->sub->c.stream, [this, fin](const std::string& e) {
-              if (e.empty()) finish(fin);
-              else fail("result copy: " + e);
-            });
         }
+        // finished problems, and admitted ones whose factorization failed
+        std::vector<HostProblem*> fin = g->bl->retire();
+        if (!fin.empty())  // done once their mode shapes have landed
+          g->sub->c.deliverer().then(g->sub->c.stream, [this, fin](const std::string& e) {
+            if (e.empty()) finish(fin);
+            else fail("result copy: " + e);
+          });
       }
     } catch (const std::exception& e) {
       fail(e.what());
@@ -3654,6 +3728,7 @@ struct Server {
             gp->nevclass = hp->nev;
             gp->qep = hp->qep;
             gp->sub = std::make_unique<SubContext>(ctx->device);
+            gp->side = std::make_unique<SubContext>(ctx->device);
             Group* raw = gp.get();
             gp->th = std::thread([this, raw] { run_group(raw); });
           }
