@@ -32,7 +32,7 @@ struct OpArrays {
 };
 
 // ---- plans (built once per mesh from the symbolic analysis) ------------------
-struct AsmTask {    // front assembly: node columns [n0, n1) of snode sn
+struct AsmTask {    // front assembly: pivot node columns [n0, n1) of snode sn
   int sn, n0, n1, a0, a1;
   int clo[2], chi[2];  // child struct node ranges that land in [n0, n1)
 };
@@ -62,6 +62,7 @@ struct FactorStep {
   int diag0, ndiag;    // range in diag task list (snode ids)
   int trsm0, ntrsm;    // range in row tasks
   int tile0, ntile;    // range in tile tasks
+  int pull;            // some tile is a front's first trailing pass (writes its update matrix)
 };
 
 struct LevelPlan {

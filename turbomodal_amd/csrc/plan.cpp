@@ -158,12 +158,13 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
     for (int id : lvl) {
       const Snode& s = S.sn[id];
       maxw = std::max(maxw, s.w());
-      const int nodes = s.ncol + s.nstr;
-      for (int n0 = 0; n0 < nodes; n0 += kAsmChunk) {
+      // pivot node columns only: the update matrix is written by the first
+      // trailing Schur pass from the children's update matrices
+      for (int n0 = 0; n0 < s.ncol; n0 += kAsmChunk) {
         AsmTask t{};
         t.sn = id;
         t.n0 = n0;
-        t.n1 = std::min(nodes, n0 + kAsmChunk);
+        t.n1 = std::min(s.ncol, n0 + kAsmChunk);
         // asm entries are ordered by column node (colpos = pos & 0xffff)
         int a = s.asm0, b = s.asm1;
         auto colpos = [&](int q) { return S.asm_pos[q] & 0xffff; };
@@ -226,6 +227,7 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
             // super-panel complete: trailing matrix, depth se - ks, in groups of
             // 4 x 4 tiles so neighbouring tasks share row and column panels
             const int G = 4 * TL;
+            if (ks == 0 && se < m) st.pull = 1;
             for (int J = se; J < m; J += G)
               for (int I = J; I < m; I += G)
                 for (int j0 = J; j0 < std::min(m, J + G); j0 += TL)
