@@ -551,7 +551,8 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
   // (no branch: an entry outside the lower triangle or the front, valid false, and
   // an absent child term read the front's first entry and are dropped), pre_value
   // combines.  On the first pass an update-matrix entry is the children's sum
-  // (child0's term first; adding +0 to the sum, which starts at +0, changes no bit).
+  // (child0's term first; adding +0 to the sum, which starts at +0, changes no bit)
+  // and a pivot-column entry the entry itself.
   struct Pre {
     CT a, c;
     bool ok0, ok1;
@@ -570,8 +571,10 @@ __global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTas
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       ok[k] = valid && gj >= w && cqj[k][q] >= 0 && cqi[k][p][j] >= 0;
+      // an absent term reads the front's first entry (one line, in cache), not the
+      // entry's own location: that would be a wasted HBM read per absent term
       src[k] = ok[k] ? &cu.U[k][(3 * cqi[k][p][j] + gi % 3) + (long long)(3 * cqj[k][q] + gj % 3) * cu.rc[k]]
-                     : own;
+                     : (k == 0 && gj < w ? own : Pn);
     }
     e.a = *src[0];
     e.c = *src[1];
