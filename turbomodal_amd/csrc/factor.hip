@@ -461,7 +461,7 @@ __device__ __forceinline__ int xcd_task(int x, int n) {
 }
 
 template <class CT, int KC, bool PULL>
-__global__ __launch_bounds__(256) void k_schur_tile(FactorArgs fa, const TileTask* __restrict__ tasks,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_schur_tile(FactorArgs fa, const TileTask* __restrict__ tasks,
                                                     int t0, int kb) {
   (void)kb;
   const TileTask t = tasks[t0 + xcd_task(blockIdx.x, gridDim.x)];
