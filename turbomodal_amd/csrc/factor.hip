@@ -748,7 +748,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // ---------------------------------------------------------------------------
 // in-place inversion of the unit lower L11 of every front (after its
 // factorization), 64-column blocks from the last: X_JJ = L_JJ^-1 (row by row
-// in LDS), X_IJ = -(sum_{J<K<=I} X_IK L_KJ) X_JJ.  The diagonal keeps D.
+// in LDS), X_IJ = -(sum_{J<K<=I} X_IK L_KJ) X_JJ.  The diagonal keeps D.  Block
+// column J is overwritten in place once its L is no longer read: X_JJ by
+// k_inv_diag (k_inv_z then reads L_IJ, I > J, and X_JJ from the panel), X_IJ by
+// k_inv_tile (which reads only columns K > J of the panel, already inverted, and
+// the scratch Z).
 // ---------------------------------------------------------------------------
 struct InvArgs {
   const DevSn* sn;
@@ -819,8 +823,7 @@ __global__ __launch_bounds__(256) void k_inv_diag(InvArgs ia, const int* __restr
   const DevSn s = ia.sn[id];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int j0 = 64 * J, nb = min(64, w - j0);
-  const CT* Pn = static_cast<const CT*>(ia.L) + b * ia.PE + s.poff;
-  CT* Sc = static_cast<CT*>(ia.S) + b * ia.SE + s.soff;  // w x 128, column-major (ld = w): X | Z
+  CT* Pn = static_cast<CT*>(ia.L) + b * ia.PE + s.poff;
   __shared__ CT A[64][64];
   const int tid = threadIdx.x;
   for (int idx = tid; idx < 64 * 64; idx += 256) {
@@ -848,7 +851,7 @@ __global__ __launch_bounds__(256) void k_inv_diag(InvArgs ia, const int* __restr
   inv_combine<32, 1, CT>(A);
   for (int idx = tid; idx < nb * nb; idx += 256) {
     const int i = idx % nb, t = idx / nb;
-    Sc[(j0 + i) + (long long)t * w] = (t < i) ? A[i][t] : mk<CT>(t == i ? 1.0 : 0.0, 0.0);
+    if (t < i) Pn[(j0 + i) + (long long)(j0 + t) * m] = A[i][t];
   }
 }
 
@@ -879,7 +882,7 @@ __device__ __forceinline__ void tile_gemm(CT (&acc)[4][4], int k0, int k1, FA fa
   }
 }
 
-// Z_IJ = L_IJ X_JJ (scratch columns 64..127), I > J
+// Z_IJ = L_IJ X_JJ (the scratch, w x 64 per front, column-major), I > J
 template <class CT>
 __global__ __launch_bounds__(256) void k_inv_z(InvArgs ia, const TileTask* __restrict__ tasks, int t0) {
   const TileTask tk = tasks[t0 + blockIdx.x];
@@ -895,16 +898,19 @@ __global__ __launch_bounds__(256) void k_inv_z(InvArgs ia, const TileTask* __res
     for (int q = 0; q < 4; ++q) acc[p][q] = mk<CT>(0.0, 0.0);
   tile_gemm(acc, 0, nj,
             [&](int r, int k) { return r < ni ? Pn[(i0 + r) + (long long)(jj0 + k) * m] : mk<CT>(0.0, 0.0); },
-            [&](int k, int c) { return c < nj ? Sc[(jj0 + k) + (long long)c * w] : mk<CT>(0.0, 0.0); });
+            [&](int k, int c) {  // X_JJ: unit lower, strictly lower part in the panel
+              if (c >= nj || k < c) return mk<CT>(0.0, 0.0);
+              return k == c ? mk<CT>(1.0, 0.0) : Pn[(jj0 + k) + (long long)(jj0 + c) * m];
+            });
   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
   for (int p = 0; p < 4; ++p)
     for (int q = 0; q < 4; ++q) {
       const int r = tx + 16 * p, c = ty + 16 * q;
-      if (r < ni && c < nj) Sc[(i0 + r) + (long long)(64 + c) * w] = acc[p][q];
+      if (r < ni && c < nj) Sc[(i0 + r) + (long long)c * w] = acc[p][q];
     }
 }
 
-// X_IJ = -sum_{J<K<=I} X_IK Z_KJ (scratch columns 0..63), I > J
+// X_IJ = -sum_{J<K<=I} X_IK Z_KJ into the panel, I > J
 template <class CT>
 __global__ __launch_bounds__(256) void k_inv_tile(InvArgs ia, const TileTask* __restrict__ tasks, int t0) {
   const TileTask tk = tasks[t0 + blockIdx.x];
@@ -913,7 +919,7 @@ __global__ __launch_bounds__(256) void k_inv_tile(InvArgs ia, const TileTask* __
   const DevSn s = ia.sn[tk.sn];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int i0 = 64 * tk.i0, ni = min(64, w - i0), jj0 = 64 * tk.j0, nj = min(64, w - jj0);
-  const CT* Pn = static_cast<const CT*>(ia.L) + b * ia.PE + s.poff;
+  CT* Pn = static_cast<CT*>(ia.L) + b * ia.PE + s.poff;
   CT* Sc = static_cast<CT*>(ia.S) + b * ia.SE + s.soff;
   CT acc[4][4];
   for (int p = 0; p < 4; ++p)
@@ -925,30 +931,13 @@ __global__ __launch_bounds__(256) void k_inv_tile(InvArgs ia, const TileTask* __
               if (k < gi) return Pn[gi + (long long)k * m];  // X_IK, already inverted
               return mk<CT>(k == gi ? 1.0 : 0.0, 0.0);
             },
-            [&](int k, int c) { return c < nj ? Sc[k + (long long)(64 + c) * w] : mk<CT>(0.0, 0.0); });
+            [&](int k, int c) { return c < nj ? Sc[k + (long long)c * w] : mk<CT>(0.0, 0.0); });
   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
   for (int p = 0; p < 4; ++p)
     for (int q = 0; q < 4; ++q) {
       const int r = tx + 16 * p, c = ty + 16 * q;
-      if (r < ni && c < nj) Sc[(i0 + r) + (long long)c * w] = mk<CT>(-acc[p][q].x, -acc[p][q].y);
+      if (r < ni && c < nj) Pn[(i0 + r) + (long long)(jj0 + c) * m] = mk<CT>(-acc[p][q].x, -acc[p][q].y);
     }
-}
-
-// write the inverted block column J back into the panel (diagonal keeps D)
-template <class CT>
-__global__ __launch_bounds__(256) void k_inv_store(InvArgs ia, const int* __restrict__ tasks, int t0, int J) {
-  const int id = tasks[t0 + blockIdx.x];
-  const int b = blockIdx.y;
-  if (ia.active && !ia.active[b]) return;
-  const DevSn s = ia.sn[id];
-  const int w = 3 * s.ncol, m = w + 3 * s.nstr;
-  const int j0 = 64 * J, nj = min(64, w - j0);
-  CT* Pn = static_cast<CT*>(ia.L) + b * ia.PE + s.poff;
-  const CT* Sc = static_cast<CT*>(ia.S) + b * ia.SE + s.soff;
-  // wave v copies columns v, v + 4, ...: four columns in flight per block
-  for (int c = threadIdx.x >> 6; c < nj; c += 4)
-    for (int i = j0 + c + 1 + (threadIdx.x & 63); i < w; i += 64)
-      Pn[i + (long long)(j0 + c) * m] = Sc[i + (long long)c * w];
 }
 
 // ---------------------------------------------------------------------------
@@ -1523,8 +1512,6 @@ static void factor_levels(Context* ctx, const Plan& P, const FactorArgs& fa, con
         hipLaunchKernelGGL(k_inv_tile<CT>, dim3(iv.ntile, B), dim3(256), 0, ctx->stream, ia,
                            P.d_inv_tiles.as<TileTask>(), iv.tile0);
       }
-      hipLaunchKernelGGL(k_inv_store<CT>, dim3(iv.ndiag, B), dim3(256), 0, ctx->stream, ia, P.d_inv_diag.as<int>(),
-                         iv.diag0, iv.J);
     }
     if (marks) TM_HIP(hipEventRecord(marks[h + 1], ctx->stream));
   }

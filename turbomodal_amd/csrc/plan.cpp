@@ -102,13 +102,13 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
     if (invmap.size() > (size_t)std::numeric_limits<int>::max()) throw std::length_error("invmap");
   }
   if (invmap.empty()) invmap.push_back(-1);
-  // inversion scratch: w x 128 per front (X block column | Z = L X_JJ), laid out per level
+  // inversion scratch: w x 64 per front (Z = L X_JJ of the current block column), laid out per level
   int64_t scratch = 0;
   for (const auto& lvl : S.levels) {
     int64_t o = 0;
     for (int id : lvl) {
       dsn[id].soff = o;
-      o += int64_t(S.sn[id].w()) * 128;
+      o += int64_t(S.sn[id].w()) * 64;
     }
     scratch = std::max(scratch, o);
   }
