@@ -1622,7 +1622,13 @@ void run_captured(Context* ctx, const std::string& key, const std::function<void
   }
   auto it = ctx->graphs.find(key);
   if (it == ctx->graphs.end()) {
-    if (ctx->graphs.size() >= 32) ctx->clear_graphs();  // bounded: a new batch shape starts afresh
+    if (ctx->graphs.size() >= 32) {  // bounded: a new batch shape starts afresh
+      // graphs launched earlier on this stream may still run: an executable graph is
+      // destroyed only once its launches have finished
+      TM_HIP(hipStreamSynchronize(ctx->stream));
+      ctx->clear_graphs();
+      if (std::getenv("TMHIP_TRACE")) std::fprintf(stderr, "[tmhip] graph cache full: cleared\n");
+    }
     hipGraph_t g = nullptr;
     TM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     try {
