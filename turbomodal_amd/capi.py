@@ -321,8 +321,8 @@ def _output(p, r, keep):
     nn = p["op"].mesh.n_nodes
     shapes = None
     if ms is not None:
-        z = ms[: 2 * 3 * nn * nm]
-        shapes = (z[0::2] + 1j * z[1::2]).reshape(3 * nn, nm)
+        # the engine writes (re, im) pairs: the buffer is already complex128, viewed without a copy
+        shapes = ms.view(np.complex128)[: 3 * nn * nm].reshape(3 * nn, nm)
     return SolveOutput(int(p["harmonic"]), float(p.get("rpm", 0.0)), f[:nm].copy(), shapes,
                        w[:nm].copy(), ev[: r.n_unique].copy(), rs[: r.n_unique].copy(),
                        bool(r.converged), int(r.restarts), int(r.status), float(r.seconds),
