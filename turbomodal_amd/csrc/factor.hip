@@ -34,6 +34,7 @@
 #include <mutex>
 #include <shared_mutex>
 #include <tuple>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -592,10 +593,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const CT z = mk<CT>(0.0, 0.0);
     return cadd(cadd(z, e.ok0 ? e.a : z), e.ok1 ? e.c : z);
   };
-  // A first-pass tile starts its accumulators at the entries' values before the
-  // update (gathered while the first chunk's loads are in flight) and accumulates the
-  // negated products (dk holds -d), so that its epilogue only stores.
-  if (pull) {
+  // A first-pass tile of an FP64 factor starts its accumulators at the entries' values
+  // before the update (gathered while the first chunk's loads are in flight) and
+  // accumulates the negated products (dk holds -d), so that its epilogue only stores.
+  // The complex FP32 factor keeps old - sum(products) with the sum from zero: each
+  // accumulation step rounds at the accumulator's magnitude, and starting it at the
+  // entry cost the FP32 factor enough accuracy to double the FP64 refinement steps of
+  // its solves (1049 -> 2106 sweep pairs per C4 sweep).
+  constexpr bool kAccInit = !std::is_same<CT, float2>::value;
+  if (kAccInit && pull) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int gj = t.j0 + q0 + 16 * q + l16;
@@ -624,7 +630,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     __syncthreads();
     if (tid < KC) {
       const RT dv = tid < jb ? Pn[(c0 + tid) + (long long)(c0 + tid) * m].x : RT(0);
-      dk[tid] = pull ? -dv : dv;
+      dk[tid] = kAccInit && pull ? -dv : dv;
     }
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
@@ -679,7 +685,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
   }
   // Epilogue.
-  // * First trailing pass: the accumulators hold the updated entries; stored.
+  // * First trailing pass, FP64: the accumulators hold the updated entries; stored.
+  //   FP32: the four entries of a 16 x 16 block pulled together (pre_load), combined,
+  //   and stored less the products.
   // * Later passes, 8-byte entries (the real factor, the complex FP32 factor): every
   //   old value loaded before the first store (the stores cannot alias the other
   //   entries' loads, but the compiler cannot know that and would otherwise wait on
@@ -687,7 +695,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   // * Later passes, complex FP64: one load, wait and store per entry; the batched
   //   form measured 3.5 % slower there (a burst of 16 loads per lane at the end of
   //   every tile).
-  if (pull) {
+  if (kAccInit && pull) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int gj = t.j0 + q0 + 16 * q + l16;
@@ -700,6 +708,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
           if (gi >= m || gi < gj) continue;
           *front_at(Pn, Un, m, w, r, gi, gj) = mk<CT>(cr[p][q][j], ci[p][q][j]);
         }
+    }
+  } else if (pull) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int gj = t.j0 + q0 + 16 * q + l16;
+      if (gj >= jlim) continue;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        Pre pre[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int gi = t.i0 + r0 + 16 * p + mfma_row(RT(0), j, l4);
+          pre[j] = pre_load(p, q, j, gi, gj, gi < m && gi >= gj);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) pre_hold(pre[j]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int gi = t.i0 + r0 + 16 * p + mfma_row(RT(0), j, l4);
+          if (gi >= m || gi < gj) continue;
+          const CT o = pre_value(pre[j], gj);
+          *front_at(Pn, Un, m, w, r, gi, gj) = mk<CT>(o.x - cr[p][q][j], o.y - ci[p][q][j]);
+        }
+      }
     }
   } else if constexpr (sizeof(CT) == 8) {
     CT old[2][2][4];
