@@ -27,6 +27,7 @@
 #include <random>
 #include <exception>
 #include <thread>
+#include <type_traits>
 
 #include "vec.hpp"
 
@@ -1989,14 +1990,33 @@ class BatchLanczos {
                                                                 long long len = -1) {
     if (len < 0) len = nvec_;
     KTimer t(ctx_, "gram", 0, 8.0 * len * kamax * kb * B_ * (B2 ? 2 : 1));
-    const dim3 g(nts_, B_, (kamax + 15) / 16);
-    if (B2)
-      hipLaunchKernelGGL(k_tsgemm_partial<true>, g, dim3(256), 0, stream(), A, lda, Bm, B2, ldb, len, nvec_,
-                         ka.as<int>(), kb, act.as<int>(), mmax_, tsp_.as<double2>(), tsp2_.as<double2>(), nts_);
-    else
-      hipLaunchKernelGGL(k_tsgemm_partial<false>, g, dim3(256), 0, stream(), A, lda, Bm, (const double2*)nullptr,
-                         ldb, len, nvec_, ka.as<int>(), kb, act.as<int>(), mmax_, tsp_.as<double2>(),
-                         (double2*)nullptr, nts_);
+    if (kb < 1 || kb > 16 || kamax < 1 || kamax > mmax_ || len > nvec_) throw std::logic_error("tsgemm shape");
+    // 16-column blocks of A over the block's four waves: one wave per block up to four
+    // blocks (row groups share the rest), two per wave beyond
+    const int nab = (kamax + 15) / 16;
+    const int wa = nab >= 3 ? 4 : nab, per = nab > 4 ? 2 : 1;
+    const dim3 g(nts_, B_, (nab + wa * per - 1) / (wa * per));
+    auto launch = [&](auto kern) {
+      hipLaunchKernelGGL(kern, g, dim3(256), 0, stream(), A, lda, Bm, B2, ldb, len, nvec_, ka.as<int>(), kb,
+                         act.as<int>(), mmax_, tsp_.as<double2>(), B2 ? tsp2_.as<double2>() : (double2*)nullptr,
+                         nts_);
+    };
+    auto by_shape = [&](auto two) {
+      constexpr bool T = decltype(two)::value;
+      if (kb <= 8) {
+        if (per == 2) launch(k_tsgemm_mfma<T, 1, 2, 4>);
+        else if (wa == 4) launch(k_tsgemm_mfma<T, 1, 1, 4>);
+        else if (wa == 2) launch(k_tsgemm_mfma<T, 1, 1, 2>);
+        else launch(k_tsgemm_mfma<T, 1, 1, 1>);
+      } else {
+        if (per == 2) launch(k_tsgemm_mfma<T, 2, 2, 4>);
+        else if (wa == 4) launch(k_tsgemm_mfma<T, 2, 1, 4>);
+        else if (wa == 2) launch(k_tsgemm_mfma<T, 2, 1, 2>);
+        else launch(k_tsgemm_mfma<T, 2, 1, 1>);
+      }
+    };
+    if (B2) by_shape(std::true_type{});
+    else by_shape(std::false_type{});
     hipLaunchKernelGGL(k_tsgemm_reduce, dim3((kamax * 16 + 255) / 256, B_), dim3(256), 0, stream(),
                        tsp_.as<double2>(), ka.as<int>(), kb, act.as<int>(), mmax_, nts_, tsc_.as<double2>());
     if (B2)

@@ -431,62 +431,166 @@ __global__ void k_flag_cols(const int* __restrict__ ncol, int q, int B, int* __r
   if (b < B) act[b] = q < ncol[b] ? 1 : 0;
 }
 
-// Tall-skinny C = A^H B, partial over row chunks.  A_b: n x ka (columns ldA apart,
-// problem stride lda_b), B_b: n x kb.  grid (nchunk, problems, ceil(ka/16)); each
-// block owns 16 columns of A and all kb columns of B for one chunk of rows.  With
-// TWO, also C2 = A^H B2 from the same staged rows of A.
-template <bool TWO>
-__global__ __launch_bounds__(256) void k_tsgemm_partial(const double2* __restrict__ A, long long lda_b,
-                                                        const double2* __restrict__ Bm, const double2* __restrict__ Bm2,
-                                                        long long ldb_b, long long n, long long cs,
-                                                        const int* __restrict__ ka_b, int kb,
-                                                        const int* __restrict__ act, int kamax,
-                                                        double2* __restrict__ part, double2* __restrict__ part2,
-                                                        int nchunk) {
+// Tall-skinny C = A^H B on the FP64 matrix cores, partial over row chunks.  A_b: n x ka
+// (columns cs apart, problem stride lda_b), B_b: n x kb (kb <= 16); with TWO also
+// C2 = A^H B2 from the same loads of A.  grid (nchunk, problems, column super-blocks):
+// each block takes kTsChunk rows; its four waves are WA column groups (NAB 16-column
+// blocks of A each) times 4 / WA row groups, whose sums are added in LDS in a fixed
+// order.  The complex product runs as real v_mfma_f64_16x16x4 products with the
+// B columns packed as [Re B | Im B] (eight complex columns per 16-wide operand):
+// P = Re(A)^T [Re B | Im B] and Q = Im(A)^T [Re B | Im B], then
+// Re C = P_re + Q_im and Im C = P_im - Q_re.  Lane l of a product holds A's row
+// (K index) l / 16 of column (M index) l % 16 and B's packed column l % 16 of the same
+// row; result register q of lane l is C[4 q + l / 16][l % 16].  Each lane reads its A
+// entry as one 16-byte complex and its B entry as the one double it needs.
+typedef double v4d __attribute__((ext_vector_type(4)));
+template <bool TWO, int NH, int NAB, int WA>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_tsgemm_mfma(const double2* __restrict__ A, long long lda_b,
+                                                     const double2* __restrict__ Bm, const double2* __restrict__ Bm2,
+                                                     long long ldb_b, long long n, long long cs,
+                                                     const int* __restrict__ ka_b, int kb,
+                                                     const int* __restrict__ act, int kamax,
+                                                     double2* __restrict__ part, double2* __restrict__ part2,
+                                                     int nchunk) {
+  constexpr int WR = 4 / WA, NS = (TWO ? 2 : 1) * NH, U = NAB * NS >= 8 ? 1 : 4;
   const int b = blockIdx.y;
   if (act && !act[b]) return;
   const int ka = ka_b[b];
-  const int a0 = blockIdx.z * 16;
-  if (a0 >= ka) return;
-  const int na = min(16, ka - a0);
-  __shared__ double2 As[64][17];
-  __shared__ double2 Bs[TWO ? 2 : 1][64][17];
-  const int tid = threadIdx.x;
-  const int oa = tid / 16, oc = tid % 16;  // output (a0 + oa, oc)
-  double2 u1 = make_double2(0.0, 0.0), u2 = make_double2(0.0, 0.0);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wa = wave % WA, wr = wave / WA;
+  const int l16 = lane & 15, l4 = lane >> 4;
+  const int abase = (blockIdx.z * WA + wa) * NAB * 16;
+  if (blockIdx.z * WA * NAB * 16 >= ka) return;  // the whole block (uniform)
+  // rows of this wave: a contiguous share of the block's chunk
   const long long r0 = (long long)blockIdx.x * kTsChunk;
   const long long r1 = r0 + kTsChunk < n ? r0 + kTsChunk : n;
-  const double2* Ab = A + b * lda_b;
-  const double2* Bb = Bm + b * ldb_b;
-  const double2* Bb2 = TWO ? Bm2 + b * ldb_b : nullptr;
-  for (long long rr = r0; rr < r1; rr += 64) {
-    const int nr = (int)(r1 - rr < 64 ? r1 - rr : 64);
-    __syncthreads();
-    for (int idx = tid; idx < 64 * 16; idx += 256) {
-      const int r = idx % 64, c = idx / 64;
-      As[r][c] = (r < nr && c < na) ? Ab[(long long)(a0 + c) * cs + rr + r] : make_double2(0.0, 0.0);
-      Bs[0][r][c] = (r < nr && c < kb) ? Bb[(long long)c * cs + rr + r] : make_double2(0.0, 0.0);
-      if (TWO) Bs[TWO ? 1 : 0][r][c] = (r < nr && c < kb) ? Bb2[(long long)c * cs + rr + r] : make_double2(0.0, 0.0);
+  constexpr int kSub = kTsChunk / WR;
+  const long long rs = r0 + (long long)wr * kSub;
+  const long long re = rs + kSub < r1 ? rs + kSub : r1;
+  const double2* pa[NAB];
+#pragma unroll
+  for (int i = 0; i < NAB; ++i) {
+    const int a = abase + 16 * i + l16;
+    pa[i] = A + b * lda_b + (long long)(a < ka ? a : 0) * cs;
+  }
+  const double* pb[NS];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    const int c = 8 * (j % NH) + (l16 & 7);
+    const double2* src = (TWO && j >= NH) ? Bm2 : Bm;
+    pb[j] = reinterpret_cast<const double*>(src + b * ldb_b + (long long)(c < kb ? c : 0) * cs) + (l16 >> 3);
+  }
+  v4d P[NAB][NS], Q[NAB][NS];
+#pragma unroll
+  for (int i = 0; i < NAB; ++i)
+#pragma unroll
+    for (int j = 0; j < NS; ++j) P[i][j] = Q[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+  // U row quads per step, all loads of a step issued before its products; enough
+  // blocks per CU keep HBM busy meanwhile.  Full steps read without masks: an A column
+  // at or past ka, or a B column at or past kb, reads column 0 instead and feeds only
+  // results that are not stored.  The rows left over (fewer than 4 U) take one masked
+  // step.
+  double2 av[U][NAB];
+  double bv[U][NS];
+  auto products = [&]() {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int i = 0; i < NAB; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+          P[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][i].x, bv[u][j], P[i][j], 0, 0, 0);
+          Q[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][i].y, bv[u][j], Q[i][j], 0, 0, 0);
+        }
+  };
+  if (abase < ka && rs < re) {
+    const int nfull = static_cast<int>((re - rs) / (4 * U));  // full steps (wave-uniform)
+    const long long rfull = rs + (long long)nfull * 4 * U;
+    for (int st = 0; st < nfull; ++st) {
+      const long long r = rs + l4 + (long long)st * 4 * U;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+#pragma unroll
+        for (int i = 0; i < NAB; ++i) av[u][i] = pa[i][r + 4 * u];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) bv[u][j] = pb[j][2 * (r + 4 * u)];
+      }
+      products();
     }
-    __syncthreads();
-    for (int r = 0; r < nr; ++r) {
-      const double2 av = As[r][oa];
-      cmacc(u1, av, Bs[0][r][oc]);
-      if (TWO) cmacc(u2, av, Bs[TWO ? 1 : 0][r][oc]);
+    if (rfull < re) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long long ru = rfull + l4 + 4 * u;
+        const bool rok = ru < re;
+#pragma unroll
+        for (int i = 0; i < NAB; ++i) av[u][i] = rok ? pa[i][ru] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < NS; ++j) bv[u][j] = rok ? pb[j][2 * ru] : 0.0;
+      }
+      products();
     }
   }
-  if (oa < na && oc < kb) {
-    const long long o = (((long long)b * kamax + a0 + oa) * 16 + oc) * nchunk + blockIdx.x;
-    part[o] = u1;
-    if (TWO) part2[o] = u2;
+  // lanes l % 16 < 8 (real parts of B) assemble the complex results with lane l + 8
+  // (every lane takes part in the shuffles) and store them
+  auto cval = [&](int i, int j, int q) {
+    const double po = __shfl_xor(P[i][j][q], 8, 64), qo = __shfl_xor(Q[i][j][q], 8, 64);
+    return make_double2(P[i][j][q] + qo, po - Q[i][j][q]);
+  };
+  auto store = [&](int i, int j, int q, double2 v) {
+    const int c = 8 * (j % NH) + l16, a = abase + 16 * i + 4 * q + l4;
+    double2* dst = (TWO && j >= NH) ? part2 : part;
+    if (l16 < 8 && a < ka && c < kb) dst[(((long long)b * kamax + a) * 16 + c) * nchunk + blockIdx.x] = v;
+  };
+  if constexpr (WR > 1) {
+    // the row groups' sums added in LDS, in row-group order, by row group 0
+    __shared__ double2 red[4][NAB * NS * 4][32];
+    const int li = l4 * 8 + (l16 & 7);
+#pragma unroll
+    for (int i = 0; i < NAB; ++i)
+#pragma unroll
+      for (int j = 0; j < NS; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double2 v = cval(i, j, q);
+          if (l16 < 8) red[wave][(i * NS + j) * 4 + q][li] = v;
+        }
+    __syncthreads();
+    if (wr > 0) return;
+#pragma unroll
+    for (int i = 0; i < NAB; ++i)
+#pragma unroll
+      for (int j = 0; j < NS; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          double2 v = red[wa][(i * NS + j) * 4 + q][li];
+          for (int g = 1; g < WR; ++g) {
+            const double2 o = red[wa + g * WA][(i * NS + j) * 4 + q][li];
+            v = make_double2(v.x + o.x, v.y + o.y);
+          }
+          store(i, j, q, v);
+        }
+  } else {
+#pragma unroll
+    for (int i = 0; i < NAB; ++i)
+#pragma unroll
+      for (int j = 0; j < NS; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) store(i, j, q, cval(i, j, q));
   }
 }
-template __global__ void k_tsgemm_partial<false>(const double2*, long long, const double2*, const double2*, long long,
-                                                 long long, long long, const int*, int, const int*, int, double2*,
-                                                 double2*, int);
-template __global__ void k_tsgemm_partial<true>(const double2*, long long, const double2*, const double2*, long long,
-                                                long long, long long, const int*, int, const int*, int, double2*,
-                                                double2*, int);
+#define TM_TSGEMM_MFMA(TWO, NH, NAB, WA)                                                                        \
+  template __global__ void k_tsgemm_mfma<TWO, NH, NAB, WA>(const double2*, long long, const double2*,           \
+                                                           const double2*, long long, long long, long long,     \
+                                                           const int*, int, const int*, int, double2*, double2*, \
+                                                           int);
+#define TM_TSGEMM_MFMA_ALL(TWO, NH) \
+  TM_TSGEMM_MFMA(TWO, NH, 1, 1) TM_TSGEMM_MFMA(TWO, NH, 1, 2) TM_TSGEMM_MFMA(TWO, NH, 1, 4) TM_TSGEMM_MFMA(TWO, NH, 2, 4)
+TM_TSGEMM_MFMA_ALL(false, 1)
+TM_TSGEMM_MFMA_ALL(false, 2)
+TM_TSGEMM_MFMA_ALL(true, 1)
+TM_TSGEMM_MFMA_ALL(true, 2)
+#undef TM_TSGEMM_MFMA_ALL
+#undef TM_TSGEMM_MFMA
 
 // C[b][a][c] = sum over chunks (fixed order)
 __global__ void k_tsgemm_reduce(const double2* __restrict__ part, const int* __restrict__ ka_b, int kb,

@@ -6,7 +6,7 @@
 namespace tme {
 
 constexpr int kDotChunk = 2048;  // rows per partial-dot block (8 per thread)
-constexpr int kTsChunk = 4096;   // rows per tall-skinny A^H B block
+constexpr int kTsChunk = 2048;   // rows per tall-skinny A^H B block
 constexpr int kTsMaxRows = 128;  // coefficient rows of k_tsmm (basis vectors per problem)
 
 // Device-side state of a batch of Lanczos runs.
@@ -67,11 +67,13 @@ __global__ void k_copy_col(const double2* src, long long lds, long long n, int q
                            long long ldd);
 __global__ void k_flag_cols(const int* ncol, int q, int B, int* act);
 
-// (A_b^H B_b) partial sums over row chunks of n rows; columns cs apart
-template <bool TWO>
-__global__ void k_tsgemm_partial(const double2* A, long long lda_b, const double2* Bm, const double2* Bm2,
-                                 long long ldb_b, long long n, long long cs, const int* ka_b, int kb, const int* act,
-                                 int kamax, double2* part, double2* part2, int nchunk);
+// (A_b^H B_b) partial sums over row chunks of n rows; columns cs apart.  NH: eight-column
+// parts of B (kb <= 8 NH); a block's four waves are WA column groups of NAB 16-column
+// blocks of A; gridDim.z covers ka in steps of 16 WA NAB columns
+template <bool TWO, int NH, int NAB, int WA>
+__global__ void k_tsgemm_mfma(const double2* A, long long lda_b, const double2* Bm, const double2* Bm2,
+                              long long ldb_b, long long n, long long cs, const int* ka_b, int kb, const int* act,
+                              int kamax, double2* part, double2* part2, int nchunk);
 __global__ void k_tsgemm_reduce(const double2* part, const int* ka_b, int kb, const int* act, int kamax, int nchunk,
                                 double2* C);
 __global__ void k_tsmm(const double2* A, long long lda_b, long long n, long long cs, const double2* C, const int* ka_b, int kc,
