@@ -582,8 +582,9 @@ void herm_gen_eig(int n, const std::vector<double>& A, const std::vector<double>
 // operator: assembly at unit omega + static prestress
 // ---------------------------------------------------------------------------
 struct BatchMem {
-  DevBuf L, U, T, S, status, pivinfo, params;
+  DevBuf L, U, T, S, status, pivinfo, params, lead, share_ws;
   int pcols = 1;
+  int nlead = 0;  // classes in `lead` (0: no class table for the next factorization)
   FactorElem elem = FactorElem::C64;
   BatchBufs bufs(const Plan&, int B) {
     BatchBufs bb;
@@ -596,7 +597,34 @@ struct BatchMem {
     bb.pivinfo = pivinfo.as<int>();
     bb.params = params.as<ProbParam>();
     bb.B = B;
+    if (nlead > 0) {
+      bb.lead = lead.as<int>();
+      bb.share_ws = share_ws.as<int>();
+      bb.nlead = nlead;
+    }
     return bb;
+  }
+  // Classes of the problems `fac` (slots) about to be factored: the same operator,
+  // omega^2 and shift give the same shared fronts.  lead[b] = first slot of b's class
+  // in `fac` (b itself for slots not factored); uploaded on `st`.
+  void set_classes(const std::vector<int>& fac, const std::vector<ProbParam>& prm, const std::vector<OpArrays>& ops,
+                   int B, hipStream_t st) {
+    std::vector<int> ld(B);
+    for (int b = 0; b < B; ++b) ld[b] = b;
+    nlead = 0;
+    for (size_t i = 0; i < fac.size(); ++i) {
+      const int b = fac[i];
+      for (size_t j = 0; j < i; ++j) {
+        const int c = fac[j];
+        if (ld[c] == c && ops[c].Kc == ops[b].Kc && prm[c].s == prm[b].s && prm[c].sigma == prm[b].sigma) {
+          ld[b] = c;
+          break;
+        }
+      }
+      if (ld[b] == b) ++nlead;
+    }
+    TM_HIP(hipMemcpyAsync(lead.p, ld.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+    TM_HIP(hipStreamSynchronize(st));
   }
   void alloc(DevPool* pool, const Plan& P, int B, int p = 1, FactorElem e = FactorElem::C64) {
     pcols = p;
@@ -609,6 +637,8 @@ struct BatchMem {
     status.alloc(pool, sizeof(int) * B);
     pivinfo.alloc(pool, sizeof(int) * 2 * B);
     params.alloc(pool, sizeof(ProbParam) * B);
+    lead.alloc(pool, sizeof(int) * B);
+    share_ws.alloc(pool, sizeof(int) * 4 * B);
   }
 };
 
@@ -2458,6 +2488,7 @@ class BatchLanczos {
   // the admitted slots' factorization on c's stream
   void admit_factor(Context* c) {
     HostTimer h(ctx_, "factor");
+    bm_.set_classes(adm_slots_, h_params_, h_ops_, B_, c->stream);
     BatchBufs bb = bm_.bufs(P_, B_);
     factor_batch(c, M_, P_, op_arrays(op_), bb, gmask_.as<int>(), static_cast<int>(adm_slots_.size()),
                  gops_.as<OpArrays>(), true, tune_);

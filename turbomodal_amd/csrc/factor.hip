@@ -159,6 +159,7 @@ struct FactorArgs {
   const ProbParam* params;
   const int* active;  // per problem: factor it (null: all)
   const OpArrays* ops_b;  // per problem operator arrays (null: op for all)
+  const int* lead;    // per problem: whose storage holds the shared fronts (null: its own)
 };
 
 // Diagonal entry (i, i) of A(k) = K + s dK - sigma M (+ the phase parts) at DOF comp of
@@ -191,7 +192,9 @@ struct ChildUpdates {
       const DevSn cs = fa.sn[c];
       inv[k] = fa.invmap + o;
       o += s.ncol + s.nstr;
-      U[k] = static_cast<const CT*>(fa.U) + b * fa.UE + cs.uoff;
+      // a shared child's update matrix is in its class leader's storage
+      const int cb = cs.shared && fa.lead ? fa.lead[b] : b;
+      U[k] = static_cast<const CT*>(fa.U) + cb * fa.UE + cs.uoff;
       rc[k] = 3 * cs.nstr;
     }
   }
@@ -274,6 +277,42 @@ __global__ void k_pivinfo_init(int* pivinfo, const int* active, int B) {
   if (b >= B || (active && !active[b])) return;
   pivinfo[2 * b] = 0;
   pivinfo[2 * b + 1] = (int)__float_as_uint(INFINITY);
+}
+
+// Shared fronts: ws = [leader mask B][status B][pivot statistics 2 B].  The problems
+// about to be factored that lead their class are marked, and the statistics of their
+// shared fronts cleared.
+__global__ void k_share_prep(const int* __restrict__ lead, const int* __restrict__ active, int B, int* ws) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  ws[b] = (!active || active[b]) && lead[b] == b ? 1 : 0;
+  ws[B + b] = 0;
+  ws[2 * B + 2 * b] = 0;
+  ws[2 * B + 2 * b + 1] = (int)__float_as_uint(INFINITY);
+}
+
+// Every factored problem that does not lead its class receives the leader's shared-front
+// panels (the first n8 8-byte words of a problem's panel storage, PE8 words apart), and
+// every factored problem its class's shared-front statistics: status bits, negative
+// pivots, smallest relative pivot (non-negative floats order as their bit patterns).
+__global__ __launch_bounds__(256) void k_share_finish(unsigned long long* __restrict__ L, long long PE8, long long n8,
+                                                      const int* __restrict__ lead, const int* __restrict__ active,
+                                                      int B, int* __restrict__ status, int* __restrict__ pivinfo,
+                                                      const int* __restrict__ ws) {
+  const int b = blockIdx.y;
+  if (active && !active[b]) return;
+  const int lb = lead[b];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    status[b] |= ws[B + lb];
+    if (pivinfo) {
+      pivinfo[2 * b] += ws[2 * B + 2 * lb];
+      pivinfo[2 * b + 1] = (int)min((unsigned)pivinfo[2 * b + 1], (unsigned)ws[2 * B + 2 * lb + 1]);
+    }
+  }
+  if (lb == b) return;
+  const unsigned long long* src = L + lb * PE8;
+  unsigned long long* dst = L + b * PE8;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n8; i += (long long)gridDim.x * 256) dst[i] = src[i];
 }
 
 template <class CT>
@@ -1485,6 +1524,7 @@ static FactorArgs factor_args(const Mesh& M, const Plan& P, const OpArrays& ops,
   fa.params = bb.params;
   fa.active = nullptr;
   fa.ops_b = nullptr;
+  fa.lead = nullptr;
   return fa;
 }
 
@@ -1508,42 +1548,58 @@ static void launch_schur(Context* ctx, int kc, const FactorStep& st, int B, cons
                        st.tile0, st.kb);
 }
 
+// The launches of one tree level's factorization tasks (L: all fronts of the level, or
+// the shared or the other fronts of it), Schur tiles at staged depth kSchurDepths[dv].
+template <class CT>
+static void factor_level(Context* ctx, const Plan& P, const LevelPlan& L, const FactorArgs& fa, const InvArgs& ia,
+                         int B, int dv) {
+  if (L.nasm)
+    hipLaunchKernelGGL(k_front_assemble<CT>, dim3(L.nasm, B), dim3(256), 0, ctx->stream, fa, P.d_asm.as<AsmTask>(),
+                       L.asm0);
+  for (const FactorStep& st : L.steps) {
+    if (st.ndiag)
+      hipLaunchKernelGGL(k_diag_ldl<CT>, dim3(st.ndiag, B), dim3(256), 0, ctx->stream, fa, P.d_diag.as<int>(),
+                         st.diag0, st.kb);
+    if (st.ntrsm)
+      hipLaunchKernelGGL(k_panel_trsm<CT>, dim3(st.ntrsm, B), dim3(256), 0, ctx->stream, fa,
+                         P.d_row.as<RowTask>(), st.trsm0, st.kb);
+    if (st.ntile) {
+      if (st.pull)
+        launch_schur<CT, true>(ctx, kSchurDepths[dv], st, B, fa, P.d_tile.as<TileTask>());
+      else
+        launch_schur<CT, false>(ctx, kSchurDepths[dv], st, B, fa, P.d_tile.as<TileTask>());
+    }
+  }
+  for (const InvStep& iv : L.inv) {
+    hipLaunchKernelGGL(k_inv_diag<CT>, dim3(iv.ndiag, B), dim3(256), 0, ctx->stream, ia, P.d_inv_diag.as<int>(),
+                       iv.diag0, iv.J);
+    if (iv.ntile) {
+      hipLaunchKernelGGL(k_inv_z<CT>, dim3(iv.ntile, B), dim3(256), 0, ctx->stream, ia, P.d_inv_tiles.as<TileTask>(),
+                         iv.tile0);
+      hipLaunchKernelGGL(k_inv_tile<CT>, dim3(iv.ntile, B), dim3(256), 0, ctx->stream, ia,
+                         P.d_inv_tiles.as<TileTask>(), iv.tile0);
+    }
+  }
+}
+
 // One factorization's launches; `depth` gives per tree level an index into
 // kSchurDepths (null: the default), and `marks` (when given, levels + 1 events) are
-// recorded at the level boundaries.
+// recorded at the level boundaries.  With `sfa` (shared-front arguments: the class
+// leaders' masks and statistics), every level runs its shared fronts with sfa / sia,
+// then its other fronts with fa / ia; a level's fronts read only lower levels' update
+// matrices, so the two parts of a level are independent.
 template <class CT>
 static void factor_levels(Context* ctx, const Plan& P, const FactorArgs& fa, const InvArgs& ia, int B,
-                          const std::vector<int8_t>* depth = nullptr, hipEvent_t* marks = nullptr) {
+                          const std::vector<int8_t>* depth = nullptr, hipEvent_t* marks = nullptr,
+                          const FactorArgs* sfa = nullptr, const InvArgs* sia = nullptr) {
   if (marks) TM_HIP(hipEventRecord(marks[0], ctx->stream));
   for (size_t h = 0; h < P.levels.size(); ++h) {
-    const LevelPlan& L = P.levels[h];
     const int dv = depth ? (*depth)[h] : kSchurDefault;
-    if (L.nasm)
-      hipLaunchKernelGGL(k_front_assemble<CT>, dim3(L.nasm, B), dim3(256), 0, ctx->stream, fa,
-                         P.d_asm.as<AsmTask>(), L.asm0);
-    for (const FactorStep& st : L.steps) {
-      if (st.ndiag)
-        hipLaunchKernelGGL(k_diag_ldl<CT>, dim3(st.ndiag, B), dim3(256), 0, ctx->stream, fa,
-                           P.d_diag.as<int>(), st.diag0, st.kb);
-      if (st.ntrsm)
-        hipLaunchKernelGGL(k_panel_trsm<CT>, dim3(st.ntrsm, B), dim3(256), 0, ctx->stream, fa,
-                           P.d_row.as<RowTask>(), st.trsm0, st.kb);
-      if (st.ntile) {
-        if (st.pull)
-          launch_schur<CT, true>(ctx, kSchurDepths[dv], st, B, fa, P.d_tile.as<TileTask>());
-        else
-          launch_schur<CT, false>(ctx, kSchurDepths[dv], st, B, fa, P.d_tile.as<TileTask>());
-      }
-    }
-    for (const InvStep& iv : L.inv) {
-      hipLaunchKernelGGL(k_inv_diag<CT>, dim3(iv.ndiag, B), dim3(256), 0, ctx->stream, ia, P.d_inv_diag.as<int>(),
-                         iv.diag0, iv.J);
-      if (iv.ntile) {
-        hipLaunchKernelGGL(k_inv_z<CT>, dim3(iv.ntile, B), dim3(256), 0, ctx->stream, ia,
-                           P.d_inv_tiles.as<TileTask>(), iv.tile0);
-        hipLaunchKernelGGL(k_inv_tile<CT>, dim3(iv.ntile, B), dim3(256), 0, ctx->stream, ia,
-                           P.d_inv_tiles.as<TileTask>(), iv.tile0);
-      }
+    if (sfa) {
+      factor_level<CT>(ctx, P, P.slevels[h], *sfa, *sia, B, dv);
+      factor_level<CT>(ctx, P, P.clevels[h], fa, ia, B, dv);
+    } else {
+      factor_level<CT>(ctx, P, P.levels[h], fa, ia, B, dv);
     }
     if (marks) TM_HIP(hipEventRecord(marks[h + 1], ctx->stream));
   }
@@ -1705,16 +1761,47 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
     case FactorElem::C32: depth = schur_depths<float2>(ctx, P, fa, ia, B, bb.elem, tune); break;
     case FactorElem::R64: depth = schur_depths<rdouble>(ctx, P, fa, ia, B, bb.elem, tune); break;
   }
+  // shared fronts once per class: by the class leaders, with their own masks and
+  // statistics, then copied to the other members (TMHIP_SHARE=0: every problem factors
+  // every front; the results are the same)
+  const char* share_env = std::getenv("TMHIP_SHARE");
+  const bool share = bb.lead && bb.share_ws && P.has_shared && !(share_env && std::strcmp(share_env, "0") == 0);
+  FactorArgs sfa = fa;
+  InvArgs sia = ia;
+  if (share) {
+    if (bb.nlead < 1) throw std::logic_error("factor: shared fronts without a class leader");
+    fa.lead = bb.lead;
+    sfa = fa;
+    sfa.active = bb.share_ws;
+    sfa.status = bb.share_ws + B;
+    sfa.pivinfo = bb.share_ws + 2 * B;
+    sia.active = bb.share_ws;
+  }
+  const int nf = nactive >= 0 ? nactive : B;
+  const double flops = share ? P.flops_other * nf + P.flops_shared * bb.nlead : M.S.flops * nf;
   KTimer t(ctx, bb.elem == FactorElem::C32 ? "factor_fp32" : (real ? "factor_real" : "factor"), 0,
-           M.S.flops * (real ? 0.25 : 1.0) * (nactive >= 0 ? nactive : B));
+           flops * (real ? 0.25 : 1.0));
   std::shared_lock<std::shared_mutex> shared(g_launch_mu);
   auto record = [&] {
     if (bb.pivinfo)
       hipLaunchKernelGGL(k_pivinfo_init, dim3((B + 63) / 64), dim3(64), 0, ctx->stream, bb.pivinfo, active, B);
+    if (share)
+      hipLaunchKernelGGL(k_share_prep, dim3((B + 255) / 256), dim3(256), 0, ctx->stream, bb.lead, active, B,
+                         bb.share_ws);
+    const FactorArgs* sf = share ? &sfa : nullptr;
+    const InvArgs* si = share ? &sia : nullptr;
     switch (bb.elem) {
-      case FactorElem::C64: factor_levels<double2>(ctx, P, fa, ia, B, &depth); break;
-      case FactorElem::C32: factor_levels<float2>(ctx, P, fa, ia, B, &depth); break;
-      case FactorElem::R64: factor_levels<rdouble>(ctx, P, fa, ia, B, &depth); break;
+      case FactorElem::C64: factor_levels<double2>(ctx, P, fa, ia, B, &depth, nullptr, sf, si); break;
+      case FactorElem::C32: factor_levels<float2>(ctx, P, fa, ia, B, &depth, nullptr, sf, si); break;
+      case FactorElem::R64: factor_levels<rdouble>(ctx, P, fa, ia, B, &depth, nullptr, sf, si); break;
+    }
+    if (share) {
+      const long long w8 = (long long)factor_elem_bytes(bb.elem) / 8;
+      const long long n8 = P.shared_panel_entries * w8;
+      const int gx = static_cast<int>(std::min<long long>(1024, (n8 + 1023) / 1024));
+      hipLaunchKernelGGL(k_share_finish, dim3(gx, B), dim3(256), 0, ctx->stream,
+                         static_cast<unsigned long long*>(bb.L), P.panel_entries * w8, n8, bb.lead, active, B,
+                         bb.status, bb.pivinfo, bb.share_ws);
     }
   };
   if (graph) {
@@ -1728,6 +1815,8 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
     key_add(key, B);
     key_add(key, bb.elem);
     key_add(key, bb.pivinfo);
+    key_add(key, share);
+    key_add(key, bb.share_ws);
     key.append(reinterpret_cast<const char*>(depth.data()), depth.size());
     run_captured(ctx, key, record);
   } else {

@@ -77,6 +77,12 @@ struct LevelPlan {
 
 struct Plan {
   uint64_t serial = 0;     // distinct for every plan built in the process (graph keys)
+  // the factorization's task ranges per level for the shared fronts and for the others
+  // (both index the same task tables as `levels`); shared fronts' panels come first
+  std::vector<LevelPlan> slevels, clevels;
+  bool has_shared = false;
+  int64_t shared_panel_entries = 0;
+  double flops_shared = 0, flops_other = 0;  // real flops of one complex factorization
   int nb = 32;             // diagonal block width
   int tile = 64;           // Schur tile
   int kw = 128;            // super-panel width: depth of the trailing Schur update
@@ -102,6 +108,9 @@ struct Plan {
 struct DevSn {
   int col0, ncol, nstr, str0, rel0, child0, child1;
   int inv0;  // offset into invmap: per child, front node -> child update node (-1: none)
+  int shared;  // the front and its subtree assemble no phase-coupled slot: A(k) there is real and
+               // the same for every harmonic at one (operator, rpm, shift)
+  int pad_;
   long long poff, uoff;
   long long toff;   // per column: [r update-vector entries][w pivot temp entries]
   long long soff;   // inversion scratch (w x 64) within its level
@@ -177,6 +186,13 @@ struct BatchBufs {
   int* pivinfo;      // 2 B (may be null): negative pivots, bits of the smallest |d_j| / |a_jj| (float)
   const ProbParam* params;  // B
   int B;
+  // Shared fronts (DevSn::shared) factored once per class of problems with the same
+  // operator, rpm and shift: lead[b] is the first problem of b's class (itself for a
+  // class's first problem), nlead the number of classes among the factored problems,
+  // share_ws 4 B ints of workspace.  lead null: every problem factors every front.
+  const int* lead = nullptr;
+  int* share_ws = nullptr;
+  int nlead = 0;
 };
 
 // A factorization is flagged near-singular (status bit 1) when a pivot d_j has

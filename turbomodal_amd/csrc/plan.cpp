@@ -65,8 +65,36 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
   static std::atomic<uint64_t> plans{0};
   P.serial = ++plans;
   const int nsn = static_cast<int>(S.sn.size());
+  // Shared fronts: a front whose assembly lists (entries (I, J), J a pivot node, and
+  // their transposes) touch no slot with phase contributions, and whose children are
+  // shared, has A(k) = K + s dK - sigma M real and independent of the harmonic, and so
+  // has its factor and update matrix.  Levels run from the leaves up.
+  std::vector<char> shared(nsn, 1);
+  auto phase_slot = [&](int sl) { return S.kp_ptr[sl + 1] > S.kp_ptr[sl]; };
+  for (const auto& lvl : S.levels)
+    for (int id : lvl) {
+      const Snode& s = S.sn[id];
+      for (int q = s.asm0; q < s.asm1 && shared[id]; ++q)
+        if (phase_slot(S.asm_slot[q]) || phase_slot(S.tslot[S.asm_slot[q]])) shared[id] = 0;
+      for (int c : s.child)
+        if (c >= 0 && !shared[c]) shared[id] = 0;
+    }
   std::vector<DevSn> dsn(nsn);
   int64_t poff = 0, toff = 0;
+  P.flops_shared = P.flops_other = 0;
+  for (int pass = 1; pass >= 0; --pass) {  // shared fronts' panels first
+    for (int id = 0; id < nsn; ++id) {
+      if (shared[id] != pass) continue;
+      const Snode& s = S.sn[id];
+      dsn[id].poff = poff;
+      poff += int64_t(s.m()) * s.w();
+      double fl = 0;
+      for (int64_t j = 0; j < s.w(); ++j) fl += 4.0 * double(s.m() - j) * double(s.m() - j);
+      (pass ? P.flops_shared : P.flops_other) += fl;
+    }
+    if (pass) P.shared_panel_entries = poff;
+  }
+  P.has_shared = P.shared_panel_entries > 0;
   for (int id = 0; id < nsn; ++id) {
     const Snode& s = S.sn[id];
     DevSn& d = dsn[id];
@@ -78,8 +106,8 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
     d.child0 = s.child[0];
     d.child1 = s.child[1];
     d.inv0 = 0;
-    d.poff = poff;
-    poff += int64_t(s.m()) * s.w();
+    d.shared = shared[id];
+    d.pad_ = 0;
     d.toff = toff;
     toff += s.r() + s.w();
     d.soff = 0;
@@ -129,6 +157,8 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
   P.upool_entries = std::max<int64_t>(pool.peak(), 1);
 
   P.levels.clear();
+  P.slevels.clear();
+  P.clevels.clear();
   P.asm_tasks.clear();
   P.diag_tasks.clear();
   P.row_tasks.clear();
@@ -144,18 +174,11 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
   constexpr int kRowChunk = 256;  // rows per panel-solve / forward-update task
   constexpr int kInvChunk = 64;   // rows per L11^-1 task
   constexpr int kColChunk = 64;   // pivot columns per backward task
-  double tri_bytes = 0;
-  for (const auto& lvl : S.levels) {
-    LevelPlan L;
-    L.sn0 = static_cast<int>(P.level_sn.size());
-    L.nsn = static_cast<int>(lvl.size());
-    P.level_sn.insert(P.level_sn.end(), lvl.begin(), lvl.end());
+  // the factorization's tasks of the fronts `fr` of one level, appended to the task tables
+  auto factor_tasks = [&](const std::vector<int>& fr, LevelPlan& L) {
     L.asm0 = static_cast<int>(P.asm_tasks.size());
-    L.finv0 = static_cast<int>(P.finv_tasks.size());
-    L.fup0 = static_cast<int>(P.fup_tasks.size());
-    L.bcol0 = static_cast<int>(P.bcol_tasks.size());
     int maxw = 0;
-    for (int id : lvl) {
+    for (int id : fr) {
       const Snode& s = S.sn[id];
       maxw = std::max(maxw, s.w());
       // pivot node columns only: the update matrix is written by the first
@@ -187,20 +210,8 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
         }
         P.asm_tasks.push_back(t);
       }
-      for (int r0 = 0; r0 < s.w(); r0 += kInvChunk)
-        P.finv_tasks.push_back({id, r0, std::min(s.w(), r0 + kInvChunk)});
-      for (int r0 = 0; r0 < s.m(); r0 += kRowChunk)
-        P.fup_tasks.push_back({id, r0, std::min(s.m(), r0 + kRowChunk)});
-      for (int c0 = 0; c0 < s.w(); c0 += kColChunk)
-        P.bcol_tasks.push_back({id, c0, std::min(s.w(), c0 + kColChunk)});
-      // per sweep pair: L21 read twice (forward, backward), L11^-1 triangle read twice
-      const double w = s.w(), r = s.r();
-      tri_bytes += 2.0 * (r * w + w * (w - 1) / 2) * 20.0;
     }
     L.nasm = static_cast<int>(P.asm_tasks.size()) - L.asm0;
-    L.nfinv = static_cast<int>(P.finv_tasks.size()) - L.finv0;
-    L.nfup = static_cast<int>(P.fup_tasks.size()) - L.fup0;
-    L.nbcol = static_cast<int>(P.bcol_tasks.size()) - L.bcol0;
     // super-panels of KW columns made of NB-wide sub-panels: each sub-panel
     // updates the rest of its super-panel (narrow update, depth NB); the last
     // one updates the trailing matrix with the whole super-panel (depth <= KW)
@@ -213,7 +224,7 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
         st.diag0 = static_cast<int>(P.diag_tasks.size());
         st.trsm0 = static_cast<int>(P.row_tasks.size());
         st.tile0 = static_cast<int>(P.tile_tasks.size());
-        for (int id : lvl) {
+        for (int id : fr) {
           const Snode& s = S.sn[id];
           const int w = s.w(), m = s.m();
           if (w <= kb) continue;
@@ -251,7 +262,7 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
       iv.J = J;
       iv.diag0 = static_cast<int>(P.inv_diag.size());
       iv.tile0 = static_cast<int>(P.inv_tiles.size());
-      for (int id : lvl) {
+      for (int id : fr) {
         const int nblk = (S.sn[id].w() + 63) / 64;
         if (nblk <= J) continue;
         P.inv_diag.push_back(id);
@@ -261,7 +272,40 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
       iv.ntile = static_cast<int>(P.inv_tiles.size()) - iv.tile0;
       L.inv.push_back(iv);
     }
+  };
+  double tri_bytes = 0;
+  for (const auto& lvl : S.levels) {
+    LevelPlan L;
+    L.sn0 = static_cast<int>(P.level_sn.size());
+    L.nsn = static_cast<int>(lvl.size());
+    P.level_sn.insert(P.level_sn.end(), lvl.begin(), lvl.end());
+    L.finv0 = static_cast<int>(P.finv_tasks.size());
+    L.fup0 = static_cast<int>(P.fup_tasks.size());
+    L.bcol0 = static_cast<int>(P.bcol_tasks.size());
+    for (int id : lvl) {
+      const Snode& s = S.sn[id];
+      for (int r0 = 0; r0 < s.w(); r0 += kInvChunk)
+        P.finv_tasks.push_back({id, r0, std::min(s.w(), r0 + kInvChunk)});
+      for (int r0 = 0; r0 < s.m(); r0 += kRowChunk)
+        P.fup_tasks.push_back({id, r0, std::min(s.m(), r0 + kRowChunk)});
+      for (int c0 = 0; c0 < s.w(); c0 += kColChunk)
+        P.bcol_tasks.push_back({id, c0, std::min(s.w(), c0 + kColChunk)});
+      // per sweep pair: L21 read twice (forward, backward), L11^-1 triangle read twice
+      const double w = s.w(), r = s.r();
+      tri_bytes += 2.0 * (r * w + w * (w - 1) / 2) * 20.0;
+    }
+    L.nfinv = static_cast<int>(P.finv_tasks.size()) - L.finv0;
+    L.nfup = static_cast<int>(P.fup_tasks.size()) - L.fup0;
+    L.nbcol = static_cast<int>(P.bcol_tasks.size()) - L.bcol0;
+    factor_tasks(lvl, L);
+    std::vector<int> sh, ot;
+    for (int id : lvl) (shared[id] ? sh : ot).push_back(id);
+    LevelPlan Ls, Lo;
+    factor_tasks(sh, Ls);
+    factor_tasks(ot, Lo);
     P.levels.push_back(L);
+    P.slevels.push_back(Ls);
+    P.clevels.push_back(Lo);
   }
   P.trisolve_bytes_c16 = tri_bytes + 5.0 * 3 * S.nf * 16.0;
   P.d_sn = upload(dsn, stream);
