@@ -584,7 +584,8 @@ void herm_gen_eig(int n, const std::vector<double>& A, const std::vector<double>
 struct BatchMem {
   DevBuf L, U, T, S, status, pivinfo, params, lead, share_ws;
   int pcols = 1;
-  int nlead = 0;  // classes in `lead` (0: no class table for the next factorization)
+  int nlead = 0;           // classes among the last factorization's problems (0: no class table)
+  std::vector<int> lead_h;  // per slot: the slot holding its shared fronts (host copy of `lead`)
   FactorElem elem = FactorElem::C64;
   BatchBufs bufs(const Plan&, int B) {
     BatchBufs bb;
@@ -606,11 +607,15 @@ struct BatchMem {
   }
   // Classes of the problems `fac` (slots) about to be factored: the same operator,
   // omega^2 and shift give the same shared fronts.  lead[b] = first slot of b's class
-  // in `fac` (b itself for slots not factored); uploaded on `st`.
+  // in `fac`; the other slots keep theirs.  Uploaded on `st`.
   void set_classes(const std::vector<int>& fac, const std::vector<ProbParam>& prm, const std::vector<OpArrays>& ops,
                    int B, hipStream_t st) {
-    std::vector<int> ld(B);
-    for (int b = 0; b < B; ++b) ld[b] = b;
+    if ((int)lead_h.size() != B) {
+      lead_h.resize(B);
+      for (int b = 0; b < B; ++b) lead_h[b] = b;
+    }
+    std::vector<int>& ld = lead_h;
+    for (int b : fac) ld[b] = b;
     nlead = 0;
     for (size_t i = 0; i < fac.size(); ++i) {
       const int b = fac[i];
@@ -639,6 +644,9 @@ struct BatchMem {
     params.alloc(pool, sizeof(ProbParam) * B);
     lead.alloc(pool, sizeof(int) * B);
     share_ws.alloc(pool, sizeof(int) * 4 * B);
+    nlead = 0;
+    lead_h.assign(B, 0);
+    for (int b = 0; b < B; ++b) lead_h[b] = b;
   }
 };
 
@@ -1008,6 +1016,7 @@ class BatchLanczos {
   const Symbolic& S_;
   const Plan& P_;
   std::vector<HostProblem*> probs_;  // per slot (nullptr: free)
+  std::vector<int> hold_;            // per slot: problems in flight whose shared fronts it holds
   int B_ = 0, ncvmax_ = 0, kmax_ = 0, nevmax_ = 0, nchunk_ = 0;
   long long n_ = 0, ldb_ = 0;
   BatchMem bm_;
@@ -2368,6 +2377,7 @@ class BatchLanczos {
     HostTimer h(ctx_, "setup");
     B_ = capacity;
     probs_.assign(B_, nullptr);
+    hold_.assign(B_, 0);
     n_ = 3LL * S_.nf;
     qep_ = qep;
     mixed_ = elem == FactorElem::C32;
@@ -2393,9 +2403,12 @@ class BatchLanczos {
     rand0_ = DevBuf();
   }
 
+  // a slot is free once it holds no problem and no problem in flight reads the shared
+  // fronts its last factorization left there
+  bool slot_free(int b) const { return !probs_[b] && hold_[b] == 0; }
   int free_slots() const {
     int n = 0;
-    for (auto* hp : probs_) n += hp ? 0 : 1;
+    for (int b = 0; b < B_; ++b) n += slot_free(b) ? 1 : 0;
     return n;
   }
   bool busy() const { return free_slots() < B_; }
@@ -2453,7 +2466,7 @@ class BatchLanczos {
     const auto now = std::chrono::steady_clock::now();
     adm_t0_ = now;
     for (int b = 0; b < B_ && !adm.empty(); ++b)
-      if (!probs_[b]) {
+      if (slot_free(b)) {
         HostProblem* hp = adm.front();
         adm.erase(adm.begin());
         probs_[b] = hp;
@@ -2489,6 +2502,7 @@ class BatchLanczos {
   void admit_factor(Context* c) {
     HostTimer h(ctx_, "factor");
     bm_.set_classes(adm_slots_, h_params_, h_ops_, B_, c->stream);
+    for (int b : adm_slots_) hold_[bm_.lead_h[b]] += 1;
     BatchBufs bb = bm_.bufs(P_, B_);
     factor_batch(c, M_, P_, op_arrays(op_), bb, gmask_.as<int>(), static_cast<int>(adm_slots_.size()),
                  gops_.as<OpArrays>(), true, tune_);
@@ -2621,6 +2635,7 @@ class BatchLanczos {
         HostProblem* hp = probs_[b];
         if (hp->out) hp->out->seconds = std::chrono::duration<double>(now - hp->t_admit).count();
         probs_[b] = nullptr;
+        hold_[bm_.lead_h[b]] -= 1;
       }
     return out;
   }

@@ -291,28 +291,20 @@ __global__ void k_share_prep(const int* __restrict__ lead, const int* __restrict
   ws[2 * B + 2 * b + 1] = (int)__float_as_uint(INFINITY);
 }
 
-// Every factored problem that does not lead its class receives the leader's shared-front
-// panels (the first n8 8-byte words of a problem's panel storage, PE8 words apart), and
-// every factored problem its class's shared-front statistics: status bits, negative
-// pivots, smallest relative pivot (non-negative floats order as their bit patterns).
-__global__ __launch_bounds__(256) void k_share_finish(unsigned long long* __restrict__ L, long long PE8, long long n8,
-                                                      const int* __restrict__ lead, const int* __restrict__ active,
-                                                      int B, int* __restrict__ status, int* __restrict__ pivinfo,
-                                                      const int* __restrict__ ws) {
-  const int b = blockIdx.y;
-  if (active && !active[b]) return;
+// Every factored problem receives its class's shared-front statistics: status bits,
+// negative pivots, smallest relative pivot (non-negative floats order as their bit
+// patterns).  The shared fronts' panels stay in the leader's storage, where the
+// triangular sweeps of every member read them.
+__global__ void k_share_finish(const int* __restrict__ lead, const int* __restrict__ active, int B,
+                               int* __restrict__ status, int* __restrict__ pivinfo, const int* __restrict__ ws) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B || (active && !active[b])) return;
   const int lb = lead[b];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    status[b] |= ws[B + lb];
-    if (pivinfo) {
-      pivinfo[2 * b] += ws[2 * B + 2 * lb];
-      pivinfo[2 * b + 1] = (int)min((unsigned)pivinfo[2 * b + 1], (unsigned)ws[2 * B + 2 * lb + 1]);
-    }
+  status[b] |= ws[B + lb];
+  if (pivinfo) {
+    pivinfo[2 * b] += ws[2 * B + 2 * lb];
+    pivinfo[2 * b + 1] = (int)min((unsigned)pivinfo[2 * b + 1], (unsigned)ws[2 * B + 2 * lb + 1]);
   }
-  if (lb == b) return;
-  const unsigned long long* src = L + lb * PE8;
-  unsigned long long* dst = L + b * PE8;
-  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n8; i += (long long)gridDim.x * 256) dst[i] = src[i];
 }
 
 template <class CT>
@@ -1030,7 +1022,15 @@ struct SolveArgs {
   long long ldx, n;
   const int* active;
   int p;
+  const int* lead;    // per problem: whose panels hold the shared fronts (null: its own)
 };
+
+// the factor panel of front s for problem b: a shared front's in its class leader's storage
+template <class LT>
+__device__ __forceinline__ const LT* panel(const SolveArgs& sa, const DevSn& s, int b) {
+  const long long pb = s.shared && sa.lead ? sa.lead[b] : b;
+  return static_cast<const LT*>(sa.L) + pb * sa.PE + s.poff;
+}
 
 __device__ __forceinline__ double2* tcol(const SolveArgs& sa, int b, int c, long long off) {
   return sa.T + ((long long)b * sa.p + c) * sa.TE + off;
@@ -1089,7 +1089,7 @@ __global__ __launch_bounds__(64) void k_fwd_inv(SolveArgs sa, const RowTask* __r
   const RowTask t = tasks[t0 + blockIdx.x];
   const DevSn s = sa.sn[t.sn];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr, p = sa.p;
-  const LT* Pn = static_cast<const LT*>(sa.L) + b * sa.PE + s.poff;
+  const LT* Pn = panel<LT>(sa, s, b);
   const double2* x = sa.X + b * sa.ldx + 3LL * s.col0;
   __shared__ double2 xs[64][PM];
   const int i = t.r0 + threadIdx.x;
@@ -1160,7 +1160,7 @@ __global__ __launch_bounds__(256) void k_fwd_inv4(SolveArgs sa, const RowTask* _
   const RowTask t = tasks[t0 + blockIdx.x];
   const DevSn s = sa.sn[t.sn];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr, p = sa.p;
-  const LT* Pn = static_cast<const LT*>(sa.L) + b * sa.PE + s.poff;
+  const LT* Pn = panel<LT>(sa, s, b);
   const double2* x = sa.X + b * sa.ldx + 3LL * s.col0;
   __shared__ double2 xs[4][64][PM];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1238,7 +1238,7 @@ __global__ __launch_bounds__(256) void k_fwd_upd(SolveArgs sa, const RowTask* __
   const RowTask t = tasks[t0 + blockIdx.x];
   const DevSn s = sa.sn[t.sn];
   const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r, p = sa.p;
-  const LT* Pn = static_cast<const LT*>(sa.L) + b * sa.PE + s.poff;
+  const LT* Pn = panel<LT>(sa, s, b);
   constexpr int CH = 2048 / PM;
   __shared__ double2 ys[CH][PM];
   const int i = t.r0 + threadIdx.x;
@@ -1304,7 +1304,7 @@ __global__ __launch_bounds__(256 * NG) void k_fwd_upd_split(SolveArgs sa, const 
   const RowTask t = tasks[t0 + blockIdx.x];
   const DevSn s = sa.sn[t.sn];
   const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r, p = sa.p;
-  const LT* Pn = static_cast<const LT*>(sa.L) + b * sa.PE + s.poff;
+  const LT* Pn = panel<LT>(sa, s, b);
   constexpr int CH = 64;
   constexpr int SH = NG * CH * PM > 256 * PM ? NG * CH * PM : 256 * PM;
   __shared__ double2 sh[SH];  // NG tiles of CH x PM solution entries, then the partial sums
@@ -1410,7 +1410,7 @@ __global__ __launch_bounds__(256) void k_bwd_tile(SolveArgs sa, const RowTask* _
   const DevSn s = sa.sn[t.sn];
   const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r, p = sa.p;
   const int j0 = t.r0, nj = t.r1 - t.r0;
-  const LT* Pn = static_cast<const LT*>(sa.L) + b * sa.PE + s.poff;
+  const LT* Pn = panel<LT>(sa, s, b);
   __shared__ double2 At[TR][64 + 1];
   __shared__ double2 vs[TR][PM];
   const int tid = threadIdx.x, jj = tid % 64, part = tid / 64;
@@ -1762,8 +1762,9 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
     case FactorElem::R64: depth = schur_depths<rdouble>(ctx, P, fa, ia, B, bb.elem, tune); break;
   }
   // shared fronts once per class: by the class leaders, with their own masks and
-  // statistics, then copied to the other members (TMHIP_SHARE=0: every problem factors
-  // every front; the results are the same)
+  // statistics, into the leaders' storage, where the members' factorizations and
+  // triangular sweeps read them (TMHIP_SHARE=0: every problem factors every front into
+  // its own storage; the results are the same)
   const char* share_env = std::getenv("TMHIP_SHARE");
   const bool share = bb.lead && bb.share_ws && P.has_shared && !(share_env && std::strcmp(share_env, "0") == 0);
   FactorArgs sfa = fa;
@@ -1795,14 +1796,9 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
       case FactorElem::C32: factor_levels<float2>(ctx, P, fa, ia, B, &depth, nullptr, sf, si); break;
       case FactorElem::R64: factor_levels<rdouble>(ctx, P, fa, ia, B, &depth, nullptr, sf, si); break;
     }
-    if (share) {
-      const long long w8 = (long long)factor_elem_bytes(bb.elem) / 8;
-      const long long n8 = P.shared_panel_entries * w8;
-      const int gx = static_cast<int>(std::min<long long>(1024, (n8 + 1023) / 1024));
-      hipLaunchKernelGGL(k_share_finish, dim3(gx, B), dim3(256), 0, ctx->stream,
-                         static_cast<unsigned long long*>(bb.L), P.panel_entries * w8, n8, bb.lead, active, B,
+    if (share)
+      hipLaunchKernelGGL(k_share_finish, dim3((B + 255) / 256), dim3(256), 0, ctx->stream, bb.lead, active, B,
                          bb.status, bb.pivinfo, bb.share_ws);
-    }
   };
   if (graph) {
     // every launch argument: the plan (its task tables and level structure), the
@@ -1882,6 +1878,7 @@ void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs&
   sa.n = colstride > 0 ? colstride : 3LL * M.S.nf;  // column stride of X
   sa.active = active;
   sa.p = p;
+  sa.lead = bb.lead;
   const int B = bb.B;
   // algorithmic bytes: the factor once (L21 + inverted pivot triangles) and ~5 n-vectors per right-hand side
   const double vec_bytes = 5.0 * 3 * M.S.nf * 16.0;
@@ -1915,6 +1912,7 @@ void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs&
     key_add(key, sa.relmap);
     key_add(key, sa.invmap);
     key_add(key, sa.L);
+    key_add(key, sa.lead);
     key_add(key, sa.T);
     key_add(key, sa.PE);
     key_add(key, sa.TE);

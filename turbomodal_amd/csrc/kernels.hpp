@@ -189,7 +189,10 @@ struct BatchBufs {
   // Shared fronts (DevSn::shared) factored once per class of problems with the same
   // operator, rpm and shift: lead[b] is the first problem of b's class (itself for a
   // class's first problem), nlead the number of classes among the factored problems,
-  // share_ws 4 B ints of workspace.  lead null: every problem factors every front.
+  // share_ws 4 B ints of workspace.  The shared fronts' panels live in the leader's
+  // storage only: every member's triangular sweeps read them there, so the caller keeps
+  // a leader's storage in place until no member of its class is solved any more.
+  // lead null: every problem factors every front into its own storage.
   const int* lead = nullptr;
   int* share_ws = nullptr;
   int nlead = 0;
