@@ -1553,7 +1553,7 @@ class BatchLanczos {
       for (int b = 0; b < B_; ++b) nact += act[b];
       if (!nact) break;
       DevBuf dact2 = ints(act);
-      trisolve_batch(ctx_, M_, P_, bb, KWb_.as<double2>(), ldw_, dact2.as<int>(), ncol, -1, nact);
+      trisolve_batch(ctx_, M_, P_, bb, KWb_.as<double2>(), ldw_, dact2.as<int>(), ncol, -1, nact, false, classes(act));
       KTimer t(ctx_, "vec", 0, 0);
       hipLaunchKernelGGL(k_add_cols, g, dim3(256), 0, stream(), W, KWb_.as<double2>(), ldw_, ncol, n_,
                          dact2.as<int>());
@@ -1572,7 +1572,7 @@ class BatchLanczos {
                        act_.as<int>());
     for (int q0 = 0; q0 < nemax; q0 += bm_.pcols)
       trisolve_batch(ctx_, M_, P_, bb, Y + (long long)q0 * n_, ldx, act_.as<int>(), std::min(bm_.pcols, nemax - q0),
-                     -1, nact);
+                     -1, nact, false, classes(nr));
     if (mixed_)  // refine the columns in W-block chunks: copy in, refine against M X, copy back
       for (int q0 = 0; q0 < nemax; q0 += pb_) {
         const int kc = std::min(pb_, nemax - q0);
@@ -2406,6 +2406,18 @@ class BatchLanczos {
   // a slot is free once it holds no problem and no problem in flight reads the shared
   // fronts its last factorization left there
   bool slot_free(int b) const { return !probs_[b] && hold_[b] == 0; }
+  // classes (shared-front leaders) of the slots in `mask`; -1 without a class table
+  int classes(const std::vector<int>& mask) const {
+    if (bm_.nlead == 0 || (int)bm_.lead_h.size() != B_) return -1;
+    std::vector<char> seen(B_, 0);
+    int n = 0;
+    for (int b = 0; b < B_; ++b)
+      if (mask[b] && !seen[bm_.lead_h[b]]) {
+        seen[bm_.lead_h[b]] = 1;
+        ++n;
+      }
+    return n;
+  }
   int free_slots() const {
     int n = 0;
     for (int b = 0; b < B_; ++b) n += slot_free(b) ? 1 : 0;
@@ -2579,7 +2591,7 @@ class BatchLanczos {
         hipLaunchKernelGGL(k_copy_cols, dim3((n_ + 255) / 256, B_), dim3(256), 0, stream(), MVb_.as<double2>(), ldv_,
                            dy0.as<int>(), Wb_.as<double2>(), ldw_, (const int*)nullptr, pb_, n_, dact.as<int>());
       }
-      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>(), ldw_, dact.as<int>(), pb_, -1, nit, true);
+      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>(), ldw_, dact.as<int>(), pb_, -1, nit, true, classes(iter_));
       if (mixed_) {
         std::vector<int> nc(B_, 0);
         for (int b = 0; b < B_; ++b) nc[b] = iter_[b] ? pb_ : 0;
@@ -2602,7 +2614,8 @@ class BatchLanczos {
       }
       // W_b += (-D) Y_b
       spmv_cols(dptr_, Wb_.as<double2>(), ldw_, dpb, pb_, Wb_.as<double2>() + n_, true);
-      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>() + n_, ldw_, dact.as<int>(), pb_, nvec_, nit, true);
+      trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>() + n_, ldw_, dact.as<int>(), pb_, nvec_, nit, true,
+                     classes(iter_));
     }
     for (int b = 0; b < B_; ++b)
       if (iter_[b]) probs_[b]->steps += 1;
@@ -2763,7 +2776,7 @@ class BatchLanczos {
       }
       spmv_cols(dptr_, Wb_.as<double2>(), ldw_, dpb, pb_, Wb_.as<double2>() + n_, true);  // += (-D) b
       trisolve_batch(ctx_, M_, P_, bb, Wb_.as<double2>() + n_, ldw_, dsel.as<int>(), pb_, nvec_,
-                     (int)std::count_if(sel.begin(), sel.end(), [](int v) { return v != 0; }));
+                     (int)std::count_if(sel.begin(), sel.end(), [](int v) { return v != 0; }), false, classes(sel));
     }
   }
 

@@ -1862,7 +1862,8 @@ static void trisolve_levels(Context* ctx, const Plan& P, const SolveArgs& sa, in
 }
 
 void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs& bb, double2* X,
-                    long long ldx, const int* active, int p, long long colstride, int nactive, bool graph) {
+                    long long ldx, const int* active, int p, long long colstride, int nactive, bool graph,
+                    int nclass) {
   if (p < 1 || p > PMAX) throw ArgError("trisolve: 1 <= p <= 16 right-hand sides");
   SolveArgs sa;
   sa.sn = P.d_sn.as<DevSn>();
@@ -1880,15 +1881,19 @@ void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs&
   sa.p = p;
   sa.lead = bb.lead;
   const int B = bb.B;
-  // algorithmic bytes: the factor once (L21 + inverted pivot triangles) and ~5 n-vectors per right-hand side
+  // algorithmic bytes: the factor once (L21 + inverted pivot triangles; the shared
+  // fronts' part once per class) and ~5 n-vectors per right-hand side
   const double vec_bytes = 5.0 * 3 * M.S.nf * 16.0;
   const double l_bytes = P.trisolve_bytes_c16 - vec_bytes;
   const int nb = nactive >= 0 ? std::min(nactive, B) : B;  // problems that do the work
+  const double l_read = bb.lead && nclass >= 0
+                            ? (l_bytes - P.trisolve_shared_bytes_c16) * nb + P.trisolve_shared_bytes_c16 * nclass
+                            : l_bytes * nb;
   // a complex-FP32 or real-FP64 factor entry is 8 + 4 B instead of 16 + 4
   const bool half = bb.elem != FactorElem::C64;
-  const double lb = half ? l_bytes * 12.0 / 20.0 : l_bytes;
+  const double lb = half ? l_read * 12.0 / 20.0 : l_read;
   const char* name = bb.elem == FactorElem::C32 ? "trisolve_fp32" : (bb.elem == FactorElem::R64 ? "trisolve_real" : "trisolve");
-  KTimer t(ctx, name, (lb + p * vec_bytes) * nb, 8.0 * p * nb * l_bytes / 20.0);
+  KTimer t(ctx, name, lb + p * vec_bytes * nb, 8.0 * p * nb * l_bytes / 20.0);
   std::shared_lock<std::shared_mutex> shared(g_launch_mu);  // not while the autotuner times
   auto levels = [&](auto lt) {
     using LT = decltype(lt);

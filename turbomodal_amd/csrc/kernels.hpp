@@ -103,6 +103,7 @@ struct Plan {
   DevBuf d_invmap;         // per front and child: front node position -> child update node, or -1
   int64_t panel_entries = 0, upool_entries = 0, tvec_entries = 0;
   double trisolve_bytes_c16 = 0;  // algorithmic bytes of one L/D/L^H pair (complex fp64)
+  double trisolve_shared_bytes_c16 = 0;  // the shared fronts' part of it (factor entries only)
 };
 
 struct DevSn {
@@ -213,12 +214,14 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
                   const OpArrays* ops_b = nullptr, bool graph = false, bool tune = false);
 // in-place solve A X = B for each problem's block X_b = X + b*ldx: p columns of
 // length n = 3 nf, column c at X_b + c*n.  bb.T must hold p * tvec_entries per problem.
-// nactive: how many problems `active` selects (-1: all B), for the byte count only.
+// nactive: how many problems `active` selects (-1: all B), for the byte count only;
+// nclass: how many classes (bb.lead) they form, whose shared fronts count once each
+// (-1: once per problem).
 // graph: replay the level launches from ctx's graph cache (run_captured); for callers
 // whose argument buffers stay in place from call to call.
 void trisolve_batch(Context* ctx, const Mesh& M, const Plan& P, const BatchBufs& bb, double2* X,
                     long long ldx, const int* active, int p = 1, long long colstride = -1, int nactive = -1,
-                    bool graph = false);
+                    bool graph = false, int nclass = -1);
 
 struct HostProblem;
 void create_operator(Context* ctx, Mesh& M, Operator& op);

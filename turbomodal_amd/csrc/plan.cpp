@@ -273,7 +273,7 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
       L.inv.push_back(iv);
     }
   };
-  double tri_bytes = 0;
+  double tri_bytes = 0, tri_shared = 0;
   for (const auto& lvl : S.levels) {
     LevelPlan L;
     L.sn0 = static_cast<int>(P.level_sn.size());
@@ -292,7 +292,9 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
         P.bcol_tasks.push_back({id, c0, std::min(s.w(), c0 + kColChunk)});
       // per sweep pair: L21 read twice (forward, backward), L11^-1 triangle read twice
       const double w = s.w(), r = s.r();
-      tri_bytes += 2.0 * (r * w + w * (w - 1) / 2) * 20.0;
+      const double fb = 2.0 * (r * w + w * (w - 1) / 2) * 20.0;
+      tri_bytes += fb;
+      if (shared[id]) tri_shared += fb;
     }
     L.nfinv = static_cast<int>(P.finv_tasks.size()) - L.finv0;
     L.nfup = static_cast<int>(P.fup_tasks.size()) - L.fup0;
@@ -308,6 +310,7 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
     P.clevels.push_back(Lo);
   }
   P.trisolve_bytes_c16 = tri_bytes + 5.0 * 3 * S.nf * 16.0;
+  P.trisolve_shared_bytes_c16 = tri_shared;
   P.d_sn = upload(dsn, stream);
   P.d_invmap = upload(invmap, stream);
   P.d_asm = upload(P.asm_tasks, stream);
