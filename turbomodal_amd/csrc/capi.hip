@@ -129,6 +129,7 @@ void tm_context_destroy(tm_context* ctx) {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  ctx->release_side();
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

@@ -190,6 +190,28 @@ void Deliverer::drain() {
   }
 }
 
+hipStream_t Context::side_stream() {
+  if (!side) TM_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+  return side;
+}
+
+hipEvent_t Context::fork_event(size_t i) {
+  while (fork_events.size() <= i) {
+    hipEvent_t e = nullptr;
+    TM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    fork_events.push_back(e);
+  }
+  return fork_events[i];
+}
+
+void Context::release_side() {
+  if (side) (void)hipStreamSynchronize(side);
+  for (auto e : fork_events) (void)hipEventDestroy(e);
+  fork_events.clear();
+  if (side) (void)hipStreamDestroy(side);
+  side = nullptr;
+}
+
 hipEvent_t Context::get_event() {
   if (!event_pool.empty()) {
     hipEvent_t e = event_pool.back();
@@ -3170,6 +3192,7 @@ static void release_context(Context& c) {
   c.pending.clear();
   for (auto ev : c.event_pool) (void)hipEventDestroy(ev);
   c.event_pool.clear();
+  c.release_side();
   (void)hipStreamDestroy(c.stream);
   c.stream = nullptr;
 }

@@ -243,6 +243,14 @@ struct Context {
   }
   hipEvent_t get_event();
   void flush_timing();  // synchronises and folds pending events into stats
+  // A second stream (default priority) and timing-free events for launch sequences that
+  // fork from `stream` and join it again within one recorded sequence; both created on
+  // first use and released with the context (release_side).
+  hipStream_t side = nullptr;
+  std::vector<hipEvent_t> fork_events;
+  hipStream_t side_stream();
+  hipEvent_t fork_event(size_t i);
+  void release_side();
   Deliverer& deliverer() {
     if (!deliver) deliver = std::make_unique<Deliverer>(device);
     return *deliver;

@@ -1535,49 +1535,50 @@ constexpr int kSchurDepths[] = {8, 16, 32};
 constexpr int kSchurDefault = 1;  // depth 16
 
 template <class CT, bool PULL>
-static void launch_schur(Context* ctx, int kc, const FactorStep& st, int B, const FactorArgs& fa,
+static void launch_schur(hipStream_t stream, int kc, const FactorStep& st, int B, const FactorArgs& fa,
                          const TileTask* tasks) {
   if (kc == 8)
-    hipLaunchKernelGGL((k_schur_tile<CT, 8, PULL>), dim3(st.ntile, B), dim3(256), 0, ctx->stream, fa, tasks,
-                       st.tile0, st.kb);
+    hipLaunchKernelGGL((k_schur_tile<CT, 8, PULL>), dim3(st.ntile, B), dim3(256), 0, stream, fa, tasks, st.tile0,
+                       st.kb);
   else if (kc == 32)
-    hipLaunchKernelGGL((k_schur_tile<CT, 32, PULL>), dim3(st.ntile, B), dim3(256), 0, ctx->stream, fa, tasks,
-                       st.tile0, st.kb);
+    hipLaunchKernelGGL((k_schur_tile<CT, 32, PULL>), dim3(st.ntile, B), dim3(256), 0, stream, fa, tasks, st.tile0,
+                       st.kb);
   else
-    hipLaunchKernelGGL((k_schur_tile<CT, 16, PULL>), dim3(st.ntile, B), dim3(256), 0, ctx->stream, fa, tasks,
-                       st.tile0, st.kb);
+    hipLaunchKernelGGL((k_schur_tile<CT, 16, PULL>), dim3(st.ntile, B), dim3(256), 0, stream, fa, tasks, st.tile0,
+                       st.kb);
 }
 
 // The launches of one tree level's factorization tasks (L: all fronts of the level, or
-// the shared or the other fronts of it), Schur tiles at staged depth kSchurDepths[dv].
+// the shared or the other fronts of it) on `stream`, Schur tiles at staged depth
+// kSchurDepths[dv].
 template <class CT>
-static void factor_level(Context* ctx, const Plan& P, const LevelPlan& L, const FactorArgs& fa, const InvArgs& ia,
-                         int B, int dv) {
+static void factor_level(hipStream_t stream, const Plan& P, const LevelPlan& L, const FactorArgs& fa,
+                         const InvArgs& ia, int B, int dv) {
   if (L.nasm)
-    hipLaunchKernelGGL(k_front_assemble<CT>, dim3(L.nasm, B), dim3(256), 0, ctx->stream, fa, P.d_asm.as<AsmTask>(),
+    hipLaunchKernelGGL(k_front_assemble<CT>, dim3(L.nasm, B), dim3(256), 0, stream, fa, P.d_asm.as<AsmTask>(),
                        L.asm0);
   for (const FactorStep& st : L.steps) {
     if (st.ndiag)
-      hipLaunchKernelGGL(k_diag_ldl<CT>, dim3(st.ndiag, B), dim3(256), 0, ctx->stream, fa, P.d_diag.as<int>(),
-                         st.diag0, st.kb);
+      hipLaunchKernelGGL(k_diag_ldl<CT>, dim3(st.ndiag, B), dim3(256), 0, stream, fa, P.d_diag.as<int>(), st.diag0,
+                         st.kb);
     if (st.ntrsm)
-      hipLaunchKernelGGL(k_panel_trsm<CT>, dim3(st.ntrsm, B), dim3(256), 0, ctx->stream, fa,
-                         P.d_row.as<RowTask>(), st.trsm0, st.kb);
+      hipLaunchKernelGGL(k_panel_trsm<CT>, dim3(st.ntrsm, B), dim3(256), 0, stream, fa, P.d_row.as<RowTask>(),
+                         st.trsm0, st.kb);
     if (st.ntile) {
       if (st.pull)
-        launch_schur<CT, true>(ctx, kSchurDepths[dv], st, B, fa, P.d_tile.as<TileTask>());
+        launch_schur<CT, true>(stream, kSchurDepths[dv], st, B, fa, P.d_tile.as<TileTask>());
       else
-        launch_schur<CT, false>(ctx, kSchurDepths[dv], st, B, fa, P.d_tile.as<TileTask>());
+        launch_schur<CT, false>(stream, kSchurDepths[dv], st, B, fa, P.d_tile.as<TileTask>());
     }
   }
   for (const InvStep& iv : L.inv) {
-    hipLaunchKernelGGL(k_inv_diag<CT>, dim3(iv.ndiag, B), dim3(256), 0, ctx->stream, ia, P.d_inv_diag.as<int>(),
+    hipLaunchKernelGGL(k_inv_diag<CT>, dim3(iv.ndiag, B), dim3(256), 0, stream, ia, P.d_inv_diag.as<int>(),
                        iv.diag0, iv.J);
     if (iv.ntile) {
-      hipLaunchKernelGGL(k_inv_z<CT>, dim3(iv.ntile, B), dim3(256), 0, ctx->stream, ia, P.d_inv_tiles.as<TileTask>(),
+      hipLaunchKernelGGL(k_inv_z<CT>, dim3(iv.ntile, B), dim3(256), 0, stream, ia, P.d_inv_tiles.as<TileTask>(),
                          iv.tile0);
-      hipLaunchKernelGGL(k_inv_tile<CT>, dim3(iv.ntile, B), dim3(256), 0, ctx->stream, ia,
-                         P.d_inv_tiles.as<TileTask>(), iv.tile0);
+      hipLaunchKernelGGL(k_inv_tile<CT>, dim3(iv.ntile, B), dim3(256), 0, stream, ia, P.d_inv_tiles.as<TileTask>(),
+                         iv.tile0);
     }
   }
 }
@@ -1585,24 +1586,43 @@ static void factor_level(Context* ctx, const Plan& P, const LevelPlan& L, const 
 // One factorization's launches; `depth` gives per tree level an index into
 // kSchurDepths (null: the default), and `marks` (when given, levels + 1 events) are
 // recorded at the level boundaries.  With `sfa` (shared-front arguments: the class
-// leaders' masks and statistics), every level runs its shared fronts with sfa / sia,
-// then its other fronts with fa / ia; a level's fronts read only lower levels' update
-// matrices, so the two parts of a level are independent.
+// leaders' masks and statistics) the shared fronts run on the context's side stream
+// with sfa / sia and the other fronts on its stream with fa / ia, level h of each side
+// after level h - 1 of both: a front reads only lower levels' update matrices, and the
+// level-scheduled update pool and inversion scratch are reused only from one level to
+// a later one, so the two parts of one level run side by side (the leaders' shared
+// fronts are small launches, latency rather than throughput).  The side stream forks
+// from the stream first and joins it at the end, so the sequence records into one graph.
 template <class CT>
 static void factor_levels(Context* ctx, const Plan& P, const FactorArgs& fa, const InvArgs& ia, int B,
                           const std::vector<int8_t>* depth = nullptr, hipEvent_t* marks = nullptr,
                           const FactorArgs* sfa = nullptr, const InvArgs* sia = nullptr) {
-  if (marks) TM_HIP(hipEventRecord(marks[0], ctx->stream));
-  for (size_t h = 0; h < P.levels.size(); ++h) {
-    const int dv = depth ? (*depth)[h] : kSchurDefault;
-    if (sfa) {
-      factor_level<CT>(ctx, P, P.slevels[h], *sfa, *sia, B, dv);
-      factor_level<CT>(ctx, P, P.clevels[h], fa, ia, B, dv);
-    } else {
-      factor_level<CT>(ctx, P, P.levels[h], fa, ia, B, dv);
+  const size_t H = P.levels.size();
+  if (!sfa) {
+    if (marks) TM_HIP(hipEventRecord(marks[0], ctx->stream));
+    for (size_t h = 0; h < H; ++h) {
+      factor_level<CT>(ctx->stream, P, P.levels[h], fa, ia, B, depth ? (*depth)[h] : kSchurDefault);
+      if (marks) TM_HIP(hipEventRecord(marks[h + 1], ctx->stream));
     }
-    if (marks) TM_HIP(hipEventRecord(marks[h + 1], ctx->stream));
+    return;
   }
+  if (H == 0) return;
+  hipStream_t ms = ctx->stream, ss = ctx->side_stream();
+  TM_HIP(hipEventRecord(ctx->fork_event(0), ms));
+  TM_HIP(hipStreamWaitEvent(ss, ctx->fork_event(0), 0));
+  for (size_t h = 0; h < H; ++h) {
+    const int dv = depth ? (*depth)[h] : kSchurDefault;
+    hipEvent_t sdone = ctx->fork_event(1 + 2 * h), cdone = ctx->fork_event(2 + 2 * h);
+    if (h > 0) {
+      TM_HIP(hipStreamWaitEvent(ss, ctx->fork_event(2 + 2 * (h - 1)), 0));
+      TM_HIP(hipStreamWaitEvent(ms, ctx->fork_event(1 + 2 * (h - 1)), 0));
+    }
+    factor_level<CT>(ss, P, P.slevels[h], *sfa, *sia, B, dv);
+    TM_HIP(hipEventRecord(sdone, ss));
+    factor_level<CT>(ms, P, P.clevels[h], fa, ia, B, dv);
+    TM_HIP(hipEventRecord(cdone, ms));
+  }
+  TM_HIP(hipStreamWaitEvent(ms, ctx->fork_event(1 + 2 * (H - 1)), 0));
 }
 
 // ---- the frontal-update autotuner ------------------------------------------
