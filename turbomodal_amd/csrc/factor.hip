@@ -1552,8 +1552,23 @@ static void launch_schur(hipStream_t stream, int kc, const FactorStep& st, int B
 // the shared or the other fronts of it) on `stream`, Schur tiles at staged depth
 // kSchurDepths[dv].
 template <class CT>
+static void invert_level(hipStream_t stream, const Plan& P, const LevelPlan& L, const InvArgs& ia, int B) {
+  for (const InvStep& iv : L.inv) {
+    hipLaunchKernelGGL(k_inv_diag<CT>, dim3(iv.ndiag, B), dim3(256), 0, stream, ia, P.d_inv_diag.as<int>(),
+                       iv.diag0, iv.J);
+    if (iv.ntile) {
+      hipLaunchKernelGGL(k_inv_z<CT>, dim3(iv.ntile, B), dim3(256), 0, stream, ia, P.d_inv_tiles.as<TileTask>(),
+                         iv.tile0);
+      hipLaunchKernelGGL(k_inv_tile<CT>, dim3(iv.ntile, B), dim3(256), 0, stream, ia, P.d_inv_tiles.as<TileTask>(),
+                         iv.tile0);
+    }
+  }
+}
+
+// (inv: the level's pivot-block inversion too)
+template <class CT>
 static void factor_level(hipStream_t stream, const Plan& P, const LevelPlan& L, const FactorArgs& fa,
-                         const InvArgs& ia, int B, int dv) {
+                         const InvArgs& ia, int B, int dv, bool inv = true) {
   if (L.nasm)
     hipLaunchKernelGGL(k_front_assemble<CT>, dim3(L.nasm, B), dim3(256), 0, stream, fa, P.d_asm.as<AsmTask>(),
                        L.asm0);
@@ -1571,28 +1586,22 @@ static void factor_level(hipStream_t stream, const Plan& P, const LevelPlan& L, 
         launch_schur<CT, false>(stream, kSchurDepths[dv], st, B, fa, P.d_tile.as<TileTask>());
     }
   }
-  for (const InvStep& iv : L.inv) {
-    hipLaunchKernelGGL(k_inv_diag<CT>, dim3(iv.ndiag, B), dim3(256), 0, stream, ia, P.d_inv_diag.as<int>(),
-                       iv.diag0, iv.J);
-    if (iv.ntile) {
-      hipLaunchKernelGGL(k_inv_z<CT>, dim3(iv.ntile, B), dim3(256), 0, stream, ia, P.d_inv_tiles.as<TileTask>(),
-                         iv.tile0);
-      hipLaunchKernelGGL(k_inv_tile<CT>, dim3(iv.ntile, B), dim3(256), 0, stream, ia, P.d_inv_tiles.as<TileTask>(),
-                         iv.tile0);
-    }
-  }
+  if (inv) invert_level<CT>(stream, P, L, ia, B);
 }
 
 // One factorization's launches; `depth` gives per tree level an index into
 // kSchurDepths (null: the default), and `marks` (when given, levels + 1 events) are
 // recorded at the level boundaries.  With `sfa` (shared-front arguments: the class
 // leaders' masks and statistics) the shared fronts run on the context's side stream
-// with sfa / sia and the other fronts on its stream with fa / ia, level h of each side
-// after level h - 1 of both: a front reads only lower levels' update matrices, and the
-// level-scheduled update pool and inversion scratch are reused only from one level to
-// a later one, so the two parts of one level run side by side (the leaders' shared
-// fronts are small launches, latency rather than throughput).  The side stream forks
-// from the stream first and joins it at the end, so the sequence records into one graph.
+// with sfa / sia and the other fronts on its stream with fa / ia, level h of each after
+// level h - 1 of both: a front reads only lower levels' update matrices, and the
+// level-scheduled update pool is reused only from one level to a later one, so the two
+// parts of one level run side by side (the leaders' shared fronts are small launches,
+// latency rather than throughput).  The other fronts' pivot-block inversion of level h
+// runs on the side stream too, before its shared fronts of level h + 1: it touches only
+// level h's panels, which no later level reads while factoring, and the inversion
+// scratch, which only the side stream uses.  The side stream forks from the stream first
+// and joins it at the end, so the sequence records into one graph.
 template <class CT>
 static void factor_levels(Context* ctx, const Plan& P, const FactorArgs& fa, const InvArgs& ia, int B,
                           const std::vector<int8_t>* depth = nullptr, hipEvent_t* marks = nullptr,
@@ -1615,14 +1624,18 @@ static void factor_levels(Context* ctx, const Plan& P, const FactorArgs& fa, con
     hipEvent_t sdone = ctx->fork_event(1 + 2 * h), cdone = ctx->fork_event(2 + 2 * h);
     if (h > 0) {
       TM_HIP(hipStreamWaitEvent(ss, ctx->fork_event(2 + 2 * (h - 1)), 0));
+      invert_level<CT>(ss, P, P.clevels[h - 1], ia, B);
       TM_HIP(hipStreamWaitEvent(ms, ctx->fork_event(1 + 2 * (h - 1)), 0));
     }
     factor_level<CT>(ss, P, P.slevels[h], *sfa, *sia, B, dv);
     TM_HIP(hipEventRecord(sdone, ss));
-    factor_level<CT>(ms, P, P.clevels[h], fa, ia, B, dv);
+    factor_level<CT>(ms, P, P.clevels[h], fa, ia, B, dv, false);
     TM_HIP(hipEventRecord(cdone, ms));
   }
-  TM_HIP(hipStreamWaitEvent(ms, ctx->fork_event(1 + 2 * (H - 1)), 0));
+  TM_HIP(hipStreamWaitEvent(ss, ctx->fork_event(2 + 2 * (H - 1)), 0));
+  invert_level<CT>(ss, P, P.clevels[H - 1], ia, B);
+  TM_HIP(hipEventRecord(ctx->fork_event(1 + 2 * H), ss));
+  TM_HIP(hipStreamWaitEvent(ms, ctx->fork_event(1 + 2 * H), 0));
 }
 
 // ---- the frontal-update autotuner ------------------------------------------
