@@ -3266,6 +3266,12 @@ static void run_block_groups(Context* ctx, Operator& op, std::vector<std::vector
   const bool serial = runs.size() == 1 || (ser && std::atoi(ser) != 0);
   size_t free_b = 0, total_b = 0;
   if (serial) {
+    struct Fork {  // one group at a time: the factorization may fork onto the side stream
+      Context* c;
+      bool was;
+      explicit Fork(Context* x) : c(x), was(x->fork) { c->fork = true; }
+      ~Fork() { c->fork = was; }
+    } fork(ctx);
     for (auto& g : ctx->groups) g->pool.trim();  // cached there, of no use here
     for (size_t g = 0; g < runs.size(); ++g) {
       auto& q = runs[g];

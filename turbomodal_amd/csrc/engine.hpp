@@ -248,6 +248,10 @@ struct Context {
   // first use and released with the context (release_side).
   hipStream_t side = nullptr;
   std::vector<hipEvent_t> fork_events;
+  // the factorization forks its shared fronts and inversions onto the side stream (set
+  // while one slot group has the device: beside other groups' streams the hardware
+  // queues are shared and the fork measured no faster)
+  bool fork = false;
   hipStream_t side_stream();
   hipEvent_t fork_event(size_t i);
   void release_side();

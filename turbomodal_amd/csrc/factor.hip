@@ -1592,8 +1592,9 @@ static void factor_level(hipStream_t stream, const Plan& P, const LevelPlan& L, 
 // One factorization's launches; `depth` gives per tree level an index into
 // kSchurDepths (null: the default), and `marks` (when given, levels + 1 events) are
 // recorded at the level boundaries.  With `sfa` (shared-front arguments: the class
-// leaders' masks and statistics) the shared fronts run on the context's side stream
-// with sfa / sia and the other fronts on its stream with fa / ia, level h of each after
+// leaders' masks and statistics) each level runs its shared fronts with sfa / sia and
+// its other fronts with fa / ia.  With ctx->fork, the shared fronts run on the context's
+// side stream and the other fronts on its stream, level h of each after
 // level h - 1 of both: a front reads only lower levels' update matrices, and the
 // level-scheduled update pool is reused only from one level to a later one, so the two
 // parts of one level run side by side (the leaders' shared fronts are small launches,
@@ -1607,10 +1608,16 @@ static void factor_levels(Context* ctx, const Plan& P, const FactorArgs& fa, con
                           const std::vector<int8_t>* depth = nullptr, hipEvent_t* marks = nullptr,
                           const FactorArgs* sfa = nullptr, const InvArgs* sia = nullptr) {
   const size_t H = P.levels.size();
-  if (!sfa) {
+  if (!sfa || !ctx->fork) {  // one stream: a level's shared fronts, then its other fronts
     if (marks) TM_HIP(hipEventRecord(marks[0], ctx->stream));
     for (size_t h = 0; h < H; ++h) {
-      factor_level<CT>(ctx->stream, P, P.levels[h], fa, ia, B, depth ? (*depth)[h] : kSchurDefault);
+      const int dv = depth ? (*depth)[h] : kSchurDefault;
+      if (sfa) {
+        factor_level<CT>(ctx->stream, P, P.slevels[h], *sfa, *sia, B, dv);
+        factor_level<CT>(ctx->stream, P, P.clevels[h], fa, ia, B, dv);
+      } else {
+        factor_level<CT>(ctx->stream, P, P.levels[h], fa, ia, B, dv);
+      }
       if (marks) TM_HIP(hipEventRecord(marks[h + 1], ctx->stream));
     }
     return;
@@ -1845,6 +1852,7 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
     key_add(key, bb.elem);
     key_add(key, bb.pivinfo);
     key_add(key, share);
+    key_add(key, ctx->fork);
     key_add(key, bb.share_ws);
     key.append(reinterpret_cast<const char*>(depth.data()), depth.size());
     run_captured(ctx, key, record);
