@@ -119,6 +119,28 @@ __device__ __forceinline__ void cmacc(double2& acc, double2 a, double2 b) {
   acc.y = fma(a.x, b.y, acc.y);
   acc.y = fma(-a.y, b.x, acc.y);
 }
+
+// A factor entry as the triangular sweeps use it: complex (double2) for complex
+// factors, real (double) for a real factor, whose products with the complex vectors
+// then take two FMAs instead of four and whose staged tiles half the LDS.  Leaving out
+// the zero imaginary terms changes no result but the sign of an exact zero.
+template <class LT> struct sweep_entry { using type = double2; };
+template <> struct sweep_entry<rdouble> { using type = double; };
+template <class LT> using se_t = typename sweep_entry<LT>::type;
+__device__ __forceinline__ double2 lde(double2 v) { return v; }
+__device__ __forceinline__ double2 lde(float2 v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ double lde(rdouble v) { return v.x; }
+template <class E> __device__ __forceinline__ E eunit(bool one);
+template <> __device__ __forceinline__ double2 eunit<double2>(bool one) { return make_double2(one ? 1.0 : 0.0, 0.0); }
+template <> __device__ __forceinline__ double eunit<double>(bool one) { return one ? 1.0 : 0.0; }
+// acc += a b and acc += conj(a) b
+__device__ __forceinline__ void emac(double2& acc, double2 a, double2 b) { cmac(acc, a, b); }
+__device__ __forceinline__ void emac(double2& acc, double a, double2 b) {
+  acc.x = fma(a, b.x, acc.x);
+  acc.y = fma(a, b.y, acc.y);
+}
+__device__ __forceinline__ void emacc(double2& acc, double2 a, double2 b) { cmacc(acc, a, b); }
+__device__ __forceinline__ void emacc(double2& acc, double a, double2 b) { emac(acc, a, b); }
 __device__ __forceinline__ void cmac(float2& acc, float2 a, float2 b) {
   acc.x = fmaf(a.x, b.x, acc.x);
   acc.x = fmaf(-a.y, b.y, acc.x);
@@ -1090,6 +1112,7 @@ __global__ __launch_bounds__(64) void k_fwd_inv(SolveArgs sa, const RowTask* __r
   const DevSn s = sa.sn[t.sn];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr, p = sa.p;
   const LT* Pn = panel<LT>(sa, s, b);
+  using E = se_t<LT>;
   const double2* x = sa.X + b * sa.ldx + 3LL * s.col0;
   __shared__ double2 xs[64][PM];
   const int i = t.r0 + threadIdx.x;
@@ -1108,39 +1131,39 @@ __global__ __launch_bounds__(64) void k_fwd_inv(SolveArgs sa, const RowTask* __r
         const LT* lp = Pn + i + (long long)j0 * m;
         int jj = 0;
         for (; jj + 4 <= nj; jj += 4) {  // four loads in flight; entries past p are zero
-          double2 l[4];
+          E l[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) l[u] = ldc(lp[(long long)(jj + u) * m]);
+          for (int u = 0; u < 4; ++u) l[u] = lde(lp[(long long)(jj + u) * m]);
 #pragma unroll
           for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], xs[jj + u][c]);
+            for (int c = 0; c < PM; ++c) emac(acc[c], l[u], xs[jj + u][c]);
         }
         for (; jj < nj; ++jj) {
-          const double2 l = ldc(lp[(long long)jj * m]);
+          const E l = lde(lp[(long long)jj * m]);
 #pragma unroll
-          for (int c = 0; c < PM; ++c) cmac(acc[c], l, xs[jj][c]);
+          for (int c = 0; c < PM; ++c) emac(acc[c], l, xs[jj][c]);
         }
       } else {  // the diagonal chunk: the same pass, entries at or above the diagonal not loaded
         const LT* lp = Pn + i + (long long)j0 * m;
         int jj = 0;
         for (; jj + 4 <= nj; jj += 4) {
-          double2 l[4];
+          E l[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const int j = j0 + jj + u;
-            l[u] = j < i ? ldc(lp[(long long)(jj + u) * m]) : make_double2(j == i ? 1.0 : 0.0, 0.0);
+            l[u] = j < i ? lde(lp[(long long)(jj + u) * m]) : eunit<E>(j == i);
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], xs[jj + u][c]);
+            for (int c = 0; c < PM; ++c) emac(acc[c], l[u], xs[jj + u][c]);
         }
         for (; jj < nj; ++jj) {
           const int j = j0 + jj;
-          const double2 l = j < i ? ldc(lp[(long long)jj * m]) : make_double2(j == i ? 1.0 : 0.0, 0.0);
+          const E l = j < i ? lde(lp[(long long)jj * m]) : eunit<E>(j == i);
 #pragma unroll
-          for (int c = 0; c < PM; ++c) cmac(acc[c], l, xs[jj][c]);
+          for (int c = 0; c < PM; ++c) emac(acc[c], l, xs[jj][c]);
         }
       }
     }
@@ -1161,6 +1184,7 @@ __global__ __launch_bounds__(256) void k_fwd_inv4(SolveArgs sa, const RowTask* _
   const DevSn s = sa.sn[t.sn];
   const int w = 3 * s.ncol, m = w + 3 * s.nstr, p = sa.p;
   const LT* Pn = panel<LT>(sa, s, b);
+  using E = se_t<LT>;
   const double2* x = sa.X + b * sa.ldx + 3LL * s.col0;
   __shared__ double2 xs[4][64][PM];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1182,38 +1206,38 @@ __global__ __launch_bounds__(256) void k_fwd_inv4(SolveArgs sa, const RowTask* _
       if (j0 + nj <= t.r0) {  // strictly below this task's diagonal
         int jj = 0;
         for (; jj + 4 <= nj; jj += 4) {
-          double2 l[4];
+          E l[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) l[u] = ldc(lp[(long long)(jj + u) * m]);
+          for (int u = 0; u < 4; ++u) l[u] = lde(lp[(long long)(jj + u) * m]);
 #pragma unroll
           for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], xs[wv][jj + u][c]);
+            for (int c = 0; c < PM; ++c) emac(acc[c], l[u], xs[wv][jj + u][c]);
         }
         for (; jj < nj; ++jj) {
-          const double2 l = ldc(lp[(long long)jj * m]);
+          const E l = lde(lp[(long long)jj * m]);
 #pragma unroll
-          for (int c = 0; c < PM; ++c) cmac(acc[c], l, xs[wv][jj][c]);
+          for (int c = 0; c < PM; ++c) emac(acc[c], l, xs[wv][jj][c]);
         }
       } else {  // the diagonal chunk: entries at or above the diagonal not loaded
         int jj = 0;
         for (; jj + 4 <= nj; jj += 4) {
-          double2 l[4];
+          E l[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const int j = j0 + jj + u;
-            l[u] = j < i ? ldc(lp[(long long)(jj + u) * m]) : make_double2(j == i ? 1.0 : 0.0, 0.0);
+            l[u] = j < i ? lde(lp[(long long)(jj + u) * m]) : eunit<E>(j == i);
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], xs[wv][jj + u][c]);
+            for (int c = 0; c < PM; ++c) emac(acc[c], l[u], xs[wv][jj + u][c]);
         }
         for (; jj < nj; ++jj) {
           const int j = j0 + jj;
-          const double2 l = j < i ? ldc(lp[(long long)jj * m]) : make_double2(j == i ? 1.0 : 0.0, 0.0);
+          const E l = j < i ? lde(lp[(long long)jj * m]) : eunit<E>(j == i);
 #pragma unroll
-          for (int c = 0; c < PM; ++c) cmac(acc[c], l, xs[wv][jj][c]);
+          for (int c = 0; c < PM; ++c) emac(acc[c], l, xs[wv][jj][c]);
         }
       }
     }
@@ -1239,6 +1263,7 @@ __global__ __launch_bounds__(256) void k_fwd_upd(SolveArgs sa, const RowTask* __
   const DevSn s = sa.sn[t.sn];
   const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r, p = sa.p;
   const LT* Pn = panel<LT>(sa, s, b);
+  using E = se_t<LT>;
   constexpr int CH = 2048 / PM;
   __shared__ double2 ys[CH][PM];
   const int i = t.r0 + threadIdx.x;
@@ -1258,18 +1283,18 @@ __global__ __launch_bounds__(256) void k_fwd_upd(SolveArgs sa, const RowTask* __
         const LT* lp = Pn + i + (long long)j0 * m;
         int jj = 0;
         for (; jj + 8 <= nj; jj += 8) {  // eight loads in flight; entries past p are zero
-          double2 l[8];
+          E l[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) l[u] = ldc(lp[(long long)(jj + u) * m]);
+          for (int u = 0; u < 8; ++u) l[u] = lde(lp[(long long)(jj + u) * m]);
 #pragma unroll
           for (int u = 0; u < 8; ++u)
 #pragma unroll
-            for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], ys[jj + u][c]);
+            for (int c = 0; c < PM; ++c) emac(acc[c], l[u], ys[jj + u][c]);
         }
         for (; jj < nj; ++jj) {
-          const double2 l = ldc(lp[(long long)jj * m]);
+          const E l = lde(lp[(long long)jj * m]);
 #pragma unroll
-          for (int c = 0; c < PM; ++c) cmac(acc[c], l, ys[jj][c]);
+          for (int c = 0; c < PM; ++c) emac(acc[c], l, ys[jj][c]);
         }
       }
     }
@@ -1305,6 +1330,7 @@ __global__ __launch_bounds__(256 * NG) void k_fwd_upd_split(SolveArgs sa, const 
   const DevSn s = sa.sn[t.sn];
   const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r, p = sa.p;
   const LT* Pn = panel<LT>(sa, s, b);
+  using E = se_t<LT>;
   constexpr int CH = 64;
   constexpr int SH = NG * CH * PM > 256 * PM ? NG * CH * PM : 256 * PM;
   __shared__ double2 sh[SH];  // NG tiles of CH x PM solution entries, then the partial sums
@@ -1329,18 +1355,18 @@ __global__ __launch_bounds__(256 * NG) void k_fwd_upd_split(SolveArgs sa, const 
         const LT* lp = Pn + i + (long long)j0 * m;
         int jj = 0;
         for (; jj + 8 <= nj; jj += 8) {
-          double2 l[8];
+          E l[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) l[u] = ldc(lp[(long long)(jj + u) * m]);
+          for (int u = 0; u < 8; ++u) l[u] = lde(lp[(long long)(jj + u) * m]);
 #pragma unroll
           for (int u = 0; u < 8; ++u)
 #pragma unroll
-            for (int c = 0; c < PM; ++c) cmac(acc[c], l[u], ys[(jj + u) * PM + c]);
+            for (int c = 0; c < PM; ++c) emac(acc[c], l[u], ys[(jj + u) * PM + c]);
         }
         for (; jj < nj; ++jj) {
-          const double2 l = ldc(lp[(long long)jj * m]);
+          const E l = lde(lp[(long long)jj * m]);
 #pragma unroll
-          for (int c = 0; c < PM; ++c) cmac(acc[c], l, ys[jj * PM + c]);
+          for (int c = 0; c < PM; ++c) emac(acc[c], l, ys[jj * PM + c]);
         }
       }
     }
@@ -1411,7 +1437,8 @@ __global__ __launch_bounds__(256) void k_bwd_tile(SolveArgs sa, const RowTask* _
   const int w = 3 * s.ncol, r = 3 * s.nstr, m = w + r, p = sa.p;
   const int j0 = t.r0, nj = t.r1 - t.r0;
   const LT* Pn = panel<LT>(sa, s, b);
-  __shared__ double2 At[TR][64 + 1];
+  using E = se_t<LT>;
+  __shared__ E At[TR][64 + 1];
   __shared__ double2 vs[TR][PM];
   const int tid = threadIdx.x, jj = tid % 64, part = tid / 64;
   const int j = j0 + jj;
@@ -1422,13 +1449,14 @@ __global__ __launch_bounds__(256) void k_bwd_tile(SolveArgs sa, const RowTask* _
   // a tile's loads are issued one tile ahead (registers), so they overlap the
   // products of the previous tile
   constexpr int APER = TR * 64 / 256, VPER = (TR * PM + 255) / 256;
-  double2 ra[APER], rv[VPER];
+  E ra[APER];
+  double2 rv[VPER];
   auto fetch = [&](int i0) {
     const int ni = min(TR, ie - i0);
 #pragma unroll
     for (int e = 0; e < APER; ++e) {
       const int idx = tid + 256 * e, ii = idx % TR, jc = idx / TR;
-      ra[e] = (ii < ni && jc < nj) ? ldc(Pn[(i0 + ii) + (long long)(j0 + jc) * m]) : make_double2(0.0, 0.0);
+      ra[e] = (ii < ni && jc < nj) ? lde(Pn[(i0 + ii) + (long long)(j0 + jc) * m]) : E();
     }
 #pragma unroll
     for (int e = 0; e < VPER; ++e) {
@@ -1457,25 +1485,25 @@ __global__ __launch_bounds__(256) void k_bwd_tile(SolveArgs sa, const RowTask* _
     const int kmin = stage == 1 ? max(0, min(TR, j - i0 + 1)) : 0;
     if (ROWSPLIT) {
       for (int k = part; k < TR; k += NP) {
-        const double2 a = k >= kmin ? At[k][jj] : make_double2(0.0, 0.0);
+        const E a = k >= kmin ? At[k][jj] : E();
 #pragma unroll
-        for (int u = 0; u < CPT; ++u) cmacc(acc[u], a, vs[k][u]);
+        for (int u = 0; u < CPT; ++u) emacc(acc[u], a, vs[k][u]);
       }
     } else {
 #pragma unroll 8
       for (int k = 0; k < TR; ++k) {
-        const double2 a = k >= kmin ? At[k][jj] : make_double2(0.0, 0.0);
+        const E a = k >= kmin ? At[k][jj] : E();
 #pragma unroll
-        for (int u = 0; u < CPT; ++u) cmacc(acc[u], a, vs[k][part + NP * u]);
+        for (int u = 0; u < CPT; ++u) emacc(acc[u], a, vs[k][part + NP * u]);
       }
     }
   }
-  if (ROWSPLIT) {  // fixed-order sum of the four row partials (PM < 4: CPT = 1, columns 0..p-1 ... p = 1)
-    __syncthreads();
-    At[part][jj] = acc[0];
+  if constexpr (ROWSPLIT) {  // fixed-order sum of the four row partials (PM < 4: CPT = 1, columns 0..p-1 ... p = 1)
+    __shared__ double2 red[NP][64];
+    red[part][jj] = acc[0];
     __syncthreads();
     if (part != 0) return;
-    acc[0] = cadd(cadd(At[0][jj], At[1][jj]), cadd(At[2][jj], At[3][jj]));
+    acc[0] = cadd(cadd(red[0][jj], red[1][jj]), cadd(red[2][jj], red[3][jj]));
   }
   if (jj >= nj) return;
   const double2* x = sa.X + b * sa.ldx + 3LL * s.col0;
