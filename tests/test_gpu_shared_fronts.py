@@ -42,19 +42,21 @@ def _solve(ctx, probs, share):
     return out, stats
 
 
-@pytest.mark.parametrize("case,rpms,ks", [
-    ("flat_disk_c1", (0.0, 5000.0, 12000.0), None),
-    ("bladed_c2", (3000.0, 10263.0), (0, 1, 5, 9, 13, 17, 18)),
+@pytest.mark.parametrize("case,rpms,ks,method", [
+    ("flat_disk_c1", (0.0, 5000.0, 12000.0), None, "block"),
+    ("flat_disk_c1", (0.0, 5000.0), None, "lanczos"),
+    ("bladed_c2", (3000.0, 10263.0), (0, 1, 5, 9, 13, 17, 18), "block"),
 ])
-def test_shared_fronts_bit_identical(ctx, case, rpms, ks):
+def test_shared_fronts_bit_identical(ctx, case, rpms, ks, method):
     from turbomodal_amd import capi, meshgen
     X, T, S, N = generated(case)
     m = oracle_model(X, T, S, N)
     em = engine_mesh(ctx, m)
     op = capi.EngineOperator(em, **meshgen.STEEL, prestress=True)
     ks = ks if ks is not None else range(N // 2 + 1)
-    probs = [dict(op=op, rpm=r, harmonic=k, num_modes=10, method=capi.TM_METHOD_BLOCK) for r in rpms for k in ks]
-    print(f"[shared] {case}: {len(probs)} problems", flush=True)
+    meth = capi.TM_METHOD_BLOCK if method == "block" else capi.TM_METHOD_LANCZOS
+    probs = [dict(op=op, rpm=r, harmonic=k, num_modes=10, method=meth) for r in rpms for k in ks]
+    print(f"[shared] {case} {method}: {len(probs)} problems", flush=True)
     shared, s1 = _solve(ctx, probs, True)
     alone, s0 = _solve(ctx, probs, False)
     for a, b in zip(shared, alone):
@@ -64,7 +66,7 @@ def test_shared_fronts_bit_identical(ctx, case, rpms, ks):
         assert np.array_equal(a.mode_shapes, b.mode_shapes)
     # the complex harmonics' factorizations did less work
     f1, f0 = s1["factor"]["alg_flops"], s0["factor"]["alg_flops"]
-    print(f"[shared] {case}: factor flops {f1:.3e} shared, {f0:.3e} every front", flush=True)
+    print(f"[shared] {case} {method}: factor flops {f1:.3e} shared, {f0:.3e} every front", flush=True)
     assert f1 < 0.95 * f0
     op.close()
     em.close()

@@ -1122,6 +1122,16 @@ class BatchLanczos {
   }
 
   void factor() {
+    {  // the batch's problems stay to the end of the run: its classes share fronts
+      std::vector<int> all(B_);
+      std::vector<ProbParam> pp(B_);
+      for (int b = 0; b < B_; ++b) {
+        all[b] = b;
+        const HostProblem* hp = probs_[b];
+        pp[b] = {hp->pre, hp->pim, hp->s, hp->in.shift};
+      }
+      bm_.set_classes(all, pp, std::vector<OpArrays>(B_, op_arrays(op_)), B_, stream());
+    }
     BatchBufs bb = bm_.bufs(P_, B_);
     factor_batch(ctx_, M_, P_, op_arrays(op_), bb);
     std::vector<int> status(B_);
