@@ -9,7 +9,10 @@
 //     children of any front;
 //   * every front's structure is that of symbolic elimination: the later nodes
 //     adjacent to its pivots or in its children's structures;
-//   * nnz(L) and the factor flop count from those structures.
+//   * nnz(L) and the factor flop count from those structures;
+//   * the shared fronts: a front is shared exactly when no slot of the pattern whose
+//     earlier node lies in its subtree's node range has phase contributions (its own
+//     or its transpose's), compared with shared_fronts().
 // Prints one JSON line.
 #include <algorithm>
 #include <cstdio>
@@ -115,12 +118,37 @@ int main(int argc, char** argv) {
     for (long long j = 0; j < w; ++j) flops += 4.0 * double(m - j) * double(m - j);
   }
   const bool root_ok = nsn > 0 && S.sn[nsn - 1].parent < 0 && st[nsn - 1].empty();
+  // shared fronts from the node ranges: entry (u, v) of the pattern is assembled in the
+  // front that pivots min(u, v); a subtree is phase-free when none of its nodes is the
+  // earlier node of a phase-coupled slot
+  std::vector<char> phase_node(nf, 0);
+  for (int u = 0; u < nf; ++u)
+    for (int q = S.rowptr[u]; q < S.rowptr[u + 1]; ++q) {
+      const int v = S.colidx[q], t = S.tslot[q];
+      if (S.kp_ptr[q + 1] > S.kp_ptr[q] || S.kp_ptr[t + 1] > S.kp_ptr[t]) phase_node[std::min(u, v)] = 1;
+    }
+  const std::vector<char> sh = shared_fronts(S);
+  bool shared_ok = (int)sh.size() == nsn;
+  int n_shared = 0;
+  double flops_shared = 0;
+  for (int id = 0; id < nsn && shared_ok; ++id) {
+    const Snode& s = S.sn[id];
+    bool free = true;
+    for (int u = lo[id]; u < s.col0 + s.ncol && free; ++u) free = !phase_node[u];
+    shared_ok = shared_ok && (sh[id] != 0) == free;
+    if (free) {
+      ++n_shared;
+      const long long w = 3LL * s.ncol, m = w + 3LL * s.nstr;
+      for (long long j = 0; j < w; ++j) flops_shared += 4.0 * double(m - j) * double(m - j);
+    }
+  }
   std::printf(
       "{\"nf\": %d, \"n_free\": %d, \"nsn\": %d, \"max_front\": %d, \"nnz_L\": %lld, \"flops\": %.6e, "
       "\"nnz_L_recomputed\": %lld, \"flops_recomputed\": %.6e, \"tile_ok\": %s, \"tree_ok\": %s, \"sep_ok\": %s, "
-      "\"sep_edges_checked\": %lld, \"struct_ok\": %s, \"root_ok\": %s}\n",
+      "\"sep_edges_checked\": %lld, \"struct_ok\": %s, \"root_ok\": %s, \"shared_ok\": %s, \"n_shared\": %d, "
+      "\"flops_shared\": %.6e}\n",
       nf, S.n_free_dofs, nsn, S.max_front, (long long)S.nnz_L, S.flops, nnz, flops, tile_ok ? "true" : "false",
       tree_ok ? "true" : "false", sep_ok ? "true" : "false", sep_checked, struct_ok ? "true" : "false",
-      root_ok ? "true" : "false");
+      root_ok ? "true" : "false", shared_ok ? "true" : "false", n_shared, flops_shared);
   return 0;
 }

@@ -348,5 +348,10 @@ struct Symbolic {
 };
 
 void build_symbolic(Symbolic& S);
+// Per front: shared (1) when neither it nor any front below it assembles an entry of a
+// slot with phase contributions (its own or its transposed slot): A(k) = K + s dK - sigma M
+// is then real on the whole subtree and the same for every harmonic, and so are the
+// front's factor and update matrix.
+std::vector<char> shared_fronts(const Symbolic& S);
 
 }  // namespace tme

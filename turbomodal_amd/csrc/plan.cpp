@@ -65,20 +65,7 @@ void build_plan(const Symbolic& S, Plan& P, hipStream_t stream) {
   static std::atomic<uint64_t> plans{0};
   P.serial = ++plans;
   const int nsn = static_cast<int>(S.sn.size());
-  // Shared fronts: a front whose assembly lists (entries (I, J), J a pivot node, and
-  // their transposes) touch no slot with phase contributions, and whose children are
-  // shared, has A(k) = K + s dK - sigma M real and independent of the harmonic, and so
-  // has its factor and update matrix.  Levels run from the leaves up.
-  std::vector<char> shared(nsn, 1);
-  auto phase_slot = [&](int sl) { return S.kp_ptr[sl + 1] > S.kp_ptr[sl]; };
-  for (const auto& lvl : S.levels)
-    for (int id : lvl) {
-      const Snode& s = S.sn[id];
-      for (int q = s.asm0; q < s.asm1 && shared[id]; ++q)
-        if (phase_slot(S.asm_slot[q]) || phase_slot(S.tslot[S.asm_slot[q]])) shared[id] = 0;
-      for (int c : s.child)
-        if (c >= 0 && !shared[c]) shared[id] = 0;
-    }
+  const std::vector<char> shared = shared_fronts(S);
   std::vector<DevSn> dsn(nsn);
   int64_t poff = 0, toff = 0;
   P.flops_shared = P.flops_other = 0;

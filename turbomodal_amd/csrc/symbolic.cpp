@@ -264,6 +264,21 @@ class NestedDissection {
 
 }  // namespace
 
+std::vector<char> shared_fronts(const Symbolic& S) {
+  const int nsn = static_cast<int>(S.sn.size());
+  std::vector<char> shared(nsn, 1);
+  auto phase_slot = [&](int sl) { return S.kp_ptr[sl + 1] > S.kp_ptr[sl]; };
+  for (const auto& lvl : S.levels)  // leaves first
+    for (int id : lvl) {
+      const Snode& s = S.sn[id];
+      for (int q = s.asm0; q < s.asm1 && shared[id]; ++q)
+        if (phase_slot(S.asm_slot[q]) || phase_slot(S.tslot[S.asm_slot[q]])) shared[id] = 0;
+      for (int c : s.child)
+        if (c >= 0 && !shared[c]) shared[id] = 0;
+    }
+  return shared;
+}
+
 void build_symbolic(Symbolic& S) {
   const int nn = S.nn;
   if (S.axis == 0) { S.c1 = 1; S.c2 = 2; S.cax = 0; }
