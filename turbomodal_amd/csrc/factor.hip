@@ -1868,6 +1868,10 @@ void factor_batch(Context* ctx, const Mesh& M, const Plan& P, const OpArrays& op
       hipLaunchKernelGGL(k_share_finish, dim3((B + 255) / 256), dim3(256), 0, ctx->stream, bb.lead, active, B,
                          bb.status, bb.pivinfo, bb.share_ws);
   };
+  if (share && ctx->fork) {  // the side stream and its events exist before any capture
+    ctx->side_stream();
+    ctx->fork_event(2 * P.levels.size() + 1);
+  }
   if (graph) {
     // every launch argument: the plan (its task tables and level structure), the
     // argument blocks (all 8-byte fields, no padding), the batch and the element type
