@@ -70,7 +70,8 @@ extern "C" {
                                 (src/hermitian_lanczos.cpp), conv_policy applies */
 #define TM_METHOD_BLOCK 1    /* block shift-invert Krylov (8 or 16 vectors per triangular sweep),
                                 FP64 Rayleigh-Ritz with K(k), M(k); converged when
-                                ||K x - lambda M x|| / ||K x|| <= max(1e-9, 4 x its FP64 floor).
+                                ||K x - lambda M x|| / ||K x|| <= max(5e-9, 4 x its FP64 floor),
+                                or below 1e-8 and no longer improving.
                                 Problems with fewer than 256 free DOFs, and TM_CONV_REFERENCE,
                                 run the Lanczos method */
 
