@@ -342,10 +342,11 @@ __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __re
   using RT = real_t<CT>;
   __shared__ CT A[NBMAX][NBMAX + 1];
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < jb * jb; idx += 256) {
-    int i = idx % jb, j = idx / jb;
-    if (i >= j) A[i][j] = Pn[(kb + i) + (long long)(kb + j) * m];
-  }
+  // thread (tx, ty) of a 32 x 8 grid takes row tx of columns ty, ty + 8, ...
+  constexpr int NY = 256 / NBMAX;
+  const int tx = tid % NBMAX, ty = tid / NBMAX;
+  for (int j = ty; j < jb; j += NY)
+    if (tx >= j && tx < jb) A[tx][j] = Pn[(kb + tx) + (long long)(kb + j) * m];
   // the diagonals of A(k) and of sigma M(k) at these columns before elimination, for the
   // pivot-size test
   __shared__ double2 a0[NBMAX];
@@ -354,33 +355,52 @@ __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __re
     a0[tid] = diag_entry(fa, fa.ops_b ? fa.ops_b[b] : fa.op, fa.params[b], s.col0 + g / 3, g % 3);
   }
   __syncthreads();
-  int nneg = 0, tiny = 0;
-  float rmin = INFINITY;
+  // Step j: L[i][j] = A[i][j] / d_j below the pivot, and the trailing lower triangle
+  // less L[i][j] d_j conj(L[k][j]).  Each thread scales the two column entries it uses
+  // itself (the same product as the stored one), so a step needs one barrier: column j
+  // is read and never written during step j, and its scaled values are stored during
+  // step j + 1, when no thread reads it any more.
+  RT inv_prev = RT(0);
   for (int j = 0; j < jb; ++j) {
     const RT d = A[j][j].x;
-    if (tid == 0) {
-      if (d == 0.0 || !isfinite(d)) atomicOr(&fa.status[b], 1);
-      nneg += d < 0.0 ? 1 : 0;
-      rmin = fminf(rmin, (float)(fabs((double)d) / fabs(a0[j].x)));
-      // near-singular: the pivot has all but vanished (relative to the entry it came
-      // from) and what cancelled it is the shift (the pivot is small against sigma M_jj
-      // too): the shift sits on an eigenvalue.  A pivot that is small only against
-      // K_jj, as sigma = 1 leaves above the rigid-body modes of a free structure, is not
-      tiny |= fabs((double)d) <= kTinyPivot * fabs(a0[j].x) && fabs((double)d) <= kTinyShiftPivot * a0[j].y;
-    }
     const RT inv = RT(1) / d;
-    for (int i = j + 1 + tid; i < jb; i += 256) A[i][j] = mk<CT>(A[i][j].x * inv, A[i][j].y * inv);
-    __syncthreads();
-    const int nt = jb - j - 1;
-    for (int idx = tid; idx < nt * nt; idx += 256) {
-      int ii = j + 1 + idx % nt, kk = j + 1 + idx / nt;
-      if (ii >= kk) {
-        CT l = A[ii][j], lk = A[kk][j];
+    if (j > 0 && tid >= j && tid < jb)
+      A[tid][j - 1] = mk<CT>(A[tid][j - 1].x * inv_prev, A[tid][j - 1].y * inv_prev);
+    const int ii = j + 1 + tx;  // entries (ii, kk), j < kk <= ii < jb, kk - j - 1 = ty mod NY
+    if (ii < jb) {
+      const CT l = mk<CT>(A[ii][j].x * inv, A[ii][j].y * inv);
+      for (int kk = j + 1 + ty; kk <= ii; kk += NY) {
+        const CT lk = mk<CT>(A[kk][j].x * inv, A[kk][j].y * inv);
         CT u = cmulc(l, lk);
         A[ii][kk] = mk<CT>(A[ii][kk].x - u.x * d, A[ii][kk].y - u.y * d);
       }
     }
+    inv_prev = inv;
     __syncthreads();
+  }
+  // pivot statistics, one lane of the last wave per pivot (its columns of the inverse
+  // below are the shortest; d_j = A[j][j], which no step after j changes): zero or
+  // non-finite pivots, the negative-pivot count, the smallest pivot relative to its
+  // entry before elimination, and near-singularity: the pivot has all but vanished
+  // (relative to the entry it came from) and what cancelled it is the shift (the pivot
+  // is small against sigma M_jj too): the shift sits on an eigenvalue.  A pivot that is
+  // small only against K_jj, as sigma = 1 leaves above the rigid-body modes of a free
+  // structure, is not.  (Lanes past jb: none of these; the first shuffle step pairs
+  // every lane below 32 with one that holds +inf, so the minimum skips NaN ratios as a
+  // sequential fminf would.)
+  constexpr int kStatLane0 = 256 - 64;
+  const int pl = tid - kStatLane0;
+  int nneg = 0, tiny = 0, bad = 0;
+  float rmin = INFINITY;
+  if (pl >= 0) {
+    const bool on = pl < jb;
+    const RT d = on ? A[pl][pl].x : RT(1);
+    const double d0 = on ? a0[pl].x : 1.0, s0 = on ? a0[pl].y : 1.0;
+    bad = __ballot(on && (d == 0.0 || !isfinite(d))) != 0;
+    nneg = __popcll(__ballot(on && d < 0.0));
+    tiny = __ballot(on && fabs((double)d) <= kTinyPivot * fabs(d0) && fabs((double)d) <= kTinyShiftPivot * s0) != 0;
+    rmin = on ? (float)(fabs((double)d) / fabs(d0)) : INFINITY;
+    for (int o = 32; o > 0; o >>= 1) rmin = fminf(rmin, __shfl_xor(rmin, o, 64));
   }
   // the inverse of the unit lower block, column c by forward substitution, kept
   // transposed in the block's strictly upper triangle (no other kernel reads it)
@@ -411,7 +431,8 @@ __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __re
     }
   }
   __syncthreads();
-  if (tid == 0) {
+  if (pl == 0) {
+    if (bad) atomicOr(&fa.status[b], 1);
     if (tiny) atomicOr(&fa.status[b], 2);
     if (fa.pivinfo) {
       if (nneg) atomicAdd(&fa.pivinfo[2 * b], nneg);
@@ -419,9 +440,9 @@ __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __re
       atomicMin(reinterpret_cast<unsigned int*>(&fa.pivinfo[2 * b + 1]), __float_as_uint(rmin));
     }
   }
-  for (int idx = tid; idx < jb * jb; idx += 256) {
-    int i = idx % jb, j = idx / jb;
-    if (i > j) {
+  for (int j = ty; j < jb; j += NY) {
+    const int i = tx;
+    if (i > j && i < jb) {
       Pn[(kb + i) + (long long)(kb + j) * m] = A[i][j];
       Pn[(kb + j) + (long long)(kb + i) * m] = Li[i][j];
     } else if (i == j) {
@@ -854,8 +875,11 @@ struct InvArgs {
 //   X21 = -X22 (L21 X11)
 // in two passes (T = L21 X11 into L21's place, then X21), 256 threads, outputs
 // with the column fastest (row reads broadcast, column reads contiguous).
+// the 64 x 64 block in LDS, rows padded by one entry so that a column is read
+// without bank conflicts
+constexpr int kInvLd = 65;
 template <int h, int nblk, class CT>
-__device__ __forceinline__ void inv_combine(CT (*A)[64]) {
+__device__ __forceinline__ void inv_combine(CT (*A)[kInvLd]) {
   constexpr int nout = nblk * h * h, per = (nout + 255) / 256;
   static_assert(nout % 256 == 0, "whole passes of 256 outputs");
   const int tid = threadIdx.x;
@@ -899,7 +923,7 @@ __device__ __forceinline__ void inv_combine(CT (*A)[64]) {
 // Inverse of the unit lower 64-block J of a front's pivot triangle (nb <= 64
 // rows; the padding is an identity block), in place in LDS: the four 16 x 16
 // diagonal blocks by the row recurrence, one wave each (lanes 0..15 own the
-// columns), then two levels of inv_combine; written to the scratch X columns.
+// columns), then two levels of inv_combine; written back to the panel.
 template <class CT>
 __global__ __launch_bounds__(256) void k_inv_diag(InvArgs ia, const int* __restrict__ tasks, int t0, int J) {
   const int id = tasks[t0 + blockIdx.x];
@@ -909,34 +933,37 @@ __global__ __launch_bounds__(256) void k_inv_diag(InvArgs ia, const int* __restr
   const int w = 3 * s.ncol, m = w + 3 * s.nstr;
   const int j0 = 64 * J, nb = min(64, w - j0);
   CT* Pn = static_cast<CT*>(ia.L) + b * ia.PE + s.poff;
-  __shared__ CT A[64][64];
+  __shared__ CT A[64][kInvLd];
   const int tid = threadIdx.x;
+  // rows fastest over the lanes: each column is a contiguous piece of the panel
   for (int idx = tid; idx < 64 * 64; idx += 256) {
-    const int i = idx / 64, t = idx % 64;
+    const int i = idx % 64, t = idx / 64;
     A[i][t] = (t < i && i < nb) ? Pn[(j0 + i) + (long long)(j0 + t) * m] : mk<CT>(0.0, 0.0);
   }
   __syncthreads();
   {
     // row i of the 16-block o: X[i][t] = -L[i][t] - sum_{t<k<i} L[i][k] X[k][t]; rows
-    // k < i already hold X
+    // k < i already hold X.  Each wave works in its own block, and a lane writes its
+    // entry of row i after its reads of that row (the stored value depends on them), so
+    // the wave's own LDS order orders the steps: the waits keep the compiler from moving
+    // the next step's reads ahead of this step's stores.
     const int o = 16 * (tid >> 6), t = tid & 63;
     for (int i = 1; i < 16; ++i) {
-      CT v = mk<CT>(0.0, 0.0);
       if (t < i) {
-        v = mk<CT>(-A[o + i][o + t].x, -A[o + i][o + t].y);
+        CT v = mk<CT>(-A[o + i][o + t].x, -A[o + i][o + t].y);
         for (int k = t + 1; k < i; ++k) v = csub(v, cmul(A[o + i][o + k], A[o + k][o + t]));
+        A[o + i][o + t] = v;
       }
-      __syncthreads();
-      if (t < i) A[o + i][o + t] = v;
-      __syncthreads();
+      __builtin_amdgcn_s_waitcnt(0xc07f);
+      __builtin_amdgcn_wave_barrier();
     }
   }
   __syncthreads();
   inv_combine<16, 2, CT>(A);
   inv_combine<32, 1, CT>(A);
-  for (int idx = tid; idx < nb * nb; idx += 256) {
-    const int i = idx % nb, t = idx / nb;
-    if (t < i) Pn[(j0 + i) + (long long)(j0 + t) * m] = A[i][t];
+  for (int idx = tid; idx < 64 * 64; idx += 256) {
+    const int i = idx % 64, t = idx / 64;
+    if (t < i && i < nb) Pn[(j0 + i) + (long long)(j0 + t) * m] = A[i][t];
   }
 }
 
@@ -950,10 +977,12 @@ __device__ __forceinline__ void tile_gemm(CT (&acc)[4][4], int k0, int k1, FA fa
   for (int kc = k0; kc < k1; kc += 16) {
     const int kn = min(16, k1 - kc);
     __syncthreads();
+    // A's rows and B's k fastest over the lanes: both operands are column-major, so
+    // a wave's loads are contiguous pieces of their columns
     for (int idx = tid; idx < 64 * 16; idx += 256) {
       const int r = idx % 64, kk = idx / 64;
       As[r][kk] = kk < kn ? fa(r, kc + kk) : mk<CT>(0.0, 0.0);
-      const int c = idx % 64, kr = idx / 64;
+      const int kr = idx % 16, c = idx / 16;
       Bs[kr][c] = kr < kn ? fb(kc + kr, c) : mk<CT>(0.0, 0.0);
     }
     __syncthreads();
