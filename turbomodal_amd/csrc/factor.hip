@@ -480,19 +480,25 @@ __global__ __launch_bounds__(256) void k_panel_trsm(FactorArgs fa, const RowTask
   CT x[NBMAX];
 #pragma unroll
   for (int k = 0; k < NBMAX; ++k) x[k] = k < jb ? Pn[i + (long long)(kb + k) * m] : mk<CT>(0.0, 0.0);
-  for (int j = 0; j < jb; ++j) {
-    CT v = mk<CT>(0.0, 0.0);
+  // v = sum_{k <= j} x_k conj(Li_jk): the terms above the diagonal of Li are exact
+  // zeros, and adding their products changed nothing (the sum starts at +0 and never
+  // becomes -0, and v + (+-0) = v otherwise), so the sums stop at the diagonal
 #pragma unroll
-    for (int k = 0; k < NBMAX; ++k) {  // v += x_k conj(L_jk)
-      v.x = fma(x[k].x, Li[j][k].x, v.x);
-      if constexpr (!is_real_elem<CT>) {
-        v.x = fma(x[k].y, Li[j][k].y, v.x);
-        v.y = fma(x[k].y, Li[j][k].x, v.y);
-        v.y = fma(-x[k].x, Li[j][k].y, v.y);
+  for (int j = 0; j < NBMAX; ++j) {
+    if (j < jb) {  // uniform; a constant trip count keeps x[] in registers
+      CT v = mk<CT>(0.0, 0.0);
+#pragma unroll
+      for (int k = 0; k <= j; ++k) {  // v += x_k conj(L_jk)
+        v.x = fma(x[k].x, Li[j][k].x, v.x);
+        if constexpr (!is_real_elem<CT>) {
+          v.x = fma(x[k].y, Li[j][k].y, v.x);
+          v.y = fma(x[k].y, Li[j][k].x, v.y);
+          v.y = fma(-x[k].x, Li[j][k].y, v.y);
+        }
       }
+      const RT inv = RT(1) / dk[j];
+      Pn[i + (long long)(kb + j) * m] = mk<CT>(v.x * inv, v.y * inv);
     }
-    const RT inv = RT(1) / dk[j];
-    Pn[i + (long long)(kb + j) * m] = mk<CT>(v.x * inv, v.y * inv);
   }
 }
 
