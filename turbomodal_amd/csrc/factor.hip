@@ -345,14 +345,25 @@ __global__ __launch_bounds__(256) void k_diag_ldl(FactorArgs fa, const int* __re
   // thread (tx, ty) of a 32 x 8 grid takes row tx of columns ty, ty + 8, ...
   constexpr int NY = 256 / NBMAX;
   const int tx = tid % NBMAX, ty = tid / NBMAX;
-  for (int j = ty; j < jb; j += NY)
-    if (tx >= j && tx < jb) A[tx][j] = Pn[(kb + tx) + (long long)(kb + j) * m];
+  // the block's loads issued together, ahead of the diagonal-entry lookups below, and
+  // stored to LDS after them, so that their latencies overlap
+  CT ra[NBMAX / NY];
+#pragma unroll
+  for (int u = 0; u < NBMAX / NY; ++u) {
+    const int j = ty + NY * u;
+    ra[u] = j < jb && tx >= j && tx < jb ? Pn[(kb + tx) + (long long)(kb + j) * m] : mk<CT>(0.0, 0.0);
+  }
   // the diagonals of A(k) and of sigma M(k) at these columns before elimination, for the
   // pivot-size test
   __shared__ double2 a0[NBMAX];
   if (tid < jb) {
     const int g = kb + tid;
     a0[tid] = diag_entry(fa, fa.ops_b ? fa.ops_b[b] : fa.op, fa.params[b], s.col0 + g / 3, g % 3);
+  }
+#pragma unroll
+  for (int u = 0; u < NBMAX / NY; ++u) {
+    const int j = ty + NY * u;
+    if (j < jb && tx >= j && tx < jb) A[tx][j] = ra[u];
   }
   __syncthreads();
   // Step j: L[i][j] = A[i][j] / d_j below the pivot, and the trailing lower triangle
@@ -941,10 +952,21 @@ __global__ __launch_bounds__(256) void k_inv_diag(InvArgs ia, const int* __restr
   CT* Pn = static_cast<CT*>(ia.L) + b * ia.PE + s.poff;
   __shared__ CT A[64][kInvLd];
   const int tid = threadIdx.x;
-  // rows fastest over the lanes: each column is a contiguous piece of the panel
-  for (int idx = tid; idx < 64 * 64; idx += 256) {
-    const int i = idx % 64, t = idx / 64;
-    A[i][t] = (t < i && i < nb) ? Pn[(j0 + i) + (long long)(j0 + t) * m] : mk<CT>(0.0, 0.0);
+  // rows fastest over the lanes: each column is a contiguous piece of the panel.  All
+  // sixteen loads of a thread are issued before the first is stored, so their
+  // latencies overlap (the LDS block already limits the kernel to two blocks per CU,
+  // so the registers cost no occupancy)
+  constexpr int kPer = 64 * 64 / 256;
+  CT ra[kPer];
+#pragma unroll
+  for (int e = 0; e < kPer; ++e) {
+    const int idx = tid + 256 * e, i = idx % 64, t = idx / 64;
+    ra[e] = (t < i && i < nb) ? Pn[(j0 + i) + (long long)(j0 + t) * m] : mk<CT>(0.0, 0.0);
+  }
+#pragma unroll
+  for (int e = 0; e < kPer; ++e) {
+    const int idx = tid + 256 * e;
+    A[idx % 64][idx / 64] = ra[e];
   }
   __syncthreads();
   {
